@@ -1,0 +1,97 @@
+"""ctypes binding of libcuvslam_sba.so (the C ABI declared in include/cuvslam_sba.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuvslam_sba.so")
+
+SBA_OK, SBA_E_EMPTY, SBA_E_INFEASIBLE, SBA_E_DEVICE, SBA_E_CAPACITY, SBA_E_INVALID = range(6)
+LOSS_STUDENT_T, LOSS_HUBER = 0, 1
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class ProblemView(C.Structure):
+    _fields_ = [
+        ("num_points", C.c_int32), ("num_fixed_points", C.c_int32), ("num_observations", C.c_int32),
+        ("num_poses", C.c_int32), ("num_fixed_key_frames", C.c_int32), ("num_cameras", C.c_int32),
+        ("points", _fp), ("observation_xys", _fp), ("observation_infos", _fp),
+        ("point_ids", _ip), ("pose_ids", _ip), ("camera_ids", C.POINTER(C.c_int8)),
+        ("rig_from_world", _fp), ("camera_from_rig", _fp),
+        ("max_iterations", C.c_int32), ("robustifier_scale", C.c_float),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("points", _fp), ("rig_from_world", _fp), ("iterations", C.c_int32), ("initial_cost", C.c_float),
+        ("last_cost", C.c_float), ("status", C.c_int32), ("initial_cost_f64", C.c_double),
+        ("last_cost_f64", C.c_double), ("lambda_", C.c_float), ("accepted", C.c_int32),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("loss", C.c_int32), ("fixed_iterations", C.c_int32), ("use_graph", C.c_int32),
+        ("chunk_pairs", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [
+        ("num_keyframes", C.c_int32), ("num_points", C.c_int32), ("sample_observations", C.c_int32),
+        ("num_cameras", C.c_int32), ("num_fixed_keyframes", C.c_int32), ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+        ("width", C.c_int32), ("height", C.c_int32), ("keyframe_spacing", C.c_double),
+        ("keyframe_rotation_deg", C.c_double), ("pixel_noise", C.c_double), ("outlier_rate", C.c_double),
+        ("pose_sigma_m", C.c_double), ("pose_sigma_deg", C.c_double), ("point_sigma_m", C.c_double),
+        ("huber_px", C.c_double), ("xy_extent", C.c_double), ("z_min", C.c_double), ("z_max", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+# Every exported symbol of include/cuvslam_sba.h with its signature.
+SIGNATURES = {
+    "sba_default_options": (None, [C.POINTER(Options)]),
+    "sba_create": (C.c_int, [C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "sba_destroy": (C.c_int, [C.c_void_p]),
+    "sba_last_error": (C.c_char_p, [C.c_void_p]),
+    "sba_abi_version": (C.c_int, []),
+    "sba_solve": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.POINTER(Result)]),
+    "sba_solve_batch": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.c_int32, C.POINTER(Result)]),
+    "sba_batch_create": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.c_int32, C.POINTER(C.c_void_p)]),
+    "sba_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sba_batch_sync": (C.c_int, [C.c_void_p]),
+    "sba_batch_read": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Result)]),
+    "sba_batch_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sba_batch_info": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip, _ip]),
+    "sba_batch_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, _fp, _ip]),
+    "sba_batch_debug_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+    "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
+    "sba_scene_default_config": (None, [C.POINTER(SceneConfig)]),
+    "sba_scene_generate": (C.c_int, [C.POINTER(SceneConfig), C.POINTER(C.c_void_p)]),
+    "sba_scene_view": (C.c_int, [C.c_void_p, C.POINTER(ProblemView)]),
+    "sba_scene_ground_truth": (C.c_int, [C.c_void_p, C.POINTER(_fp), C.POINTER(_fp)]),
+    "sba_scene_free": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads the native library; raises if it has not been built (there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(the HIP extension is the only implementation)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,103 @@
+// Device-side data layout of a packed SBA batch (shared by the HIP kernels and the host runtime).
+//
+// A batch is any number of independent windows packed back to back in structure-of-arrays streams:
+//   observations  sorted by (point, key frame, camera) inside each window:
+//                   obs_xy   float2  normalized image coordinates
+//                   obs_info float4  (L00, (L01+L10)/2, L11, camera id as int bits)
+//   pairs         one per distinct (point, key frame) of a window, i.e. one block W_ij of the
+//                 Hessian's off-diagonal part; a point's pairs are sorted by key frame, so its pairs with
+//                 free key frames come first:
+//                   pair_obs  int32 first observation of the pair (pair_obs[p+1] ends it)
+//                   pair_pose int32 key frame index inside the window
+//   points          point_pair int32 first pair of the point (point_pair[i+1] ends it)
+//                   point_nfree int32 number of the point's pairs with free key frames
+//                   pts[2]   float4 (x, y, z, 0), double-buffered: current and trial state
+//                   paux     3 x float4: damped inverse V*^-1 (6 upper entries), V*^-1 e_b (3),
+//                            diag(V) (3), written by the Schur kernel for the back-substitution
+//   key frames      poses[2] 3 x float4 rows of [R | t] of rig_from_world, double-buffered
+//   cameras         cams     3 x float4 rows of [R | t] of camera_from_rig
+//   reduced system  s_acc    FP32 accumulator of the reduced camera system S (upper 6x6 blocks, packed
+//                            block-row-major), then e (6 per free key frame), then diag(U)
+//                   A        FP64 Cholesky workspace (packed lower triangle) for windows whose factor
+//                            does not fit in LDS
+//                   dc       FP32 pose step (6 per free key frame)
+// Windows carry their offsets into every stream, so one launch covers all windows.
+#pragma once
+
+#include <stdint.h>
+
+namespace sba_dev {
+
+constexpr int kBlock = 256;        // threads of the per-tile kernels
+constexpr int kTilePairs = 256;    // max (point, key frame) pairs per tile (one per thread)
+constexpr int kTilePoints = 256;   // max points per tile (one per thread)
+constexpr int kMaxFreePoses = 40;  // free key frames per window the shared-memory Schur partial supports
+constexpr int kMaxPoses = 256;
+constexpr int kMaxCameras = 32;    // camera::Rig::kMaxCameras (libs/camera/rig.h:30)
+constexpr int kFactorThreads = 512;
+constexpr int kPanel = 8;          // Cholesky panel width
+
+struct WindowDesc {
+  int32_t np, npf, nc, ncf, ncam, no;
+  int32_t point_off, pair_off, obs_off, pose_off, cam_off;
+  int32_t s_off;     // floats into s_acc
+  int64_t a_off;     // doubles into A (only when the factor lives in global memory)
+  int32_t dc_off;    // floats into dc
+  int32_t max_iterations;
+  float robust;
+  int32_t factor_in_lds;
+};
+
+struct WindowState {
+  double current_cost, initial_cost, last_cost;
+  double acc_cost, acc_hess, acc_scal_pt, scal_pose;
+  float lambda, maxdc;
+  uint32_t acc_maxdp;  // float bits of a non-negative max
+  int32_t acc_skipped;
+  int32_t chol_ok, done, status, iterations, accepted, cur;
+  int32_t pad[2];
+};
+
+struct Tile {
+  int32_t window, point_begin, point_end, pad;
+};
+
+struct Chunk {
+  int32_t tile_begin, tile_end;
+};
+
+struct Batch {
+  int32_t n_windows, n_chunks, n_points, n_poses, n_cams;
+  int32_t loss, fixed_iterations, pad0;
+  int64_t s_acc_size;
+  const WindowDesc* win;
+  WindowState* st;
+  const Tile* tiles;
+  const Chunk* chunks;
+  const float2* obs_xy;
+  const float4* obs_info;
+  const int32_t* pair_obs;
+  const int32_t* pair_pose;
+  const int32_t* point_pair;
+  const int32_t* point_nfree;
+  float4* pts0;
+  float4* pts1;
+  const float4* pts_init;
+  float4* paux;      // 3 per point
+  float4* poses0;    // 3 per pose
+  float4* poses1;
+  const float4* poses_init;
+  const float4* cams;
+  float* s_acc;
+  double* A;
+  float* dc;
+  double* dc64;
+  float4* out_pts;
+  float4* out_poses;
+  float4* dbg_dp;    // optional: the point step of the last back-substitution (phase-level tests)
+  // shared-memory layout of the Schur kernel (offsets in 4-byte words)
+  int32_t lds_pose, lds_cam, lds_S, lds_e, lds_du, lds_acc, lds_Z, lds_pp, lds_pre, lds_words;
+  int32_t factor_lds_words;  // LDS words of the factor kernel
+};
+
+}  // namespace sba_dev

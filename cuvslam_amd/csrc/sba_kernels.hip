@@ -1,0 +1,1095 @@
+// HIP kernels of the MI355X sliding-window SBA (gfx950).
+//
+// One Levenberg-Marquardt iteration of every window of a batch is four launches:
+//   k_schur     per chunk of a window's points: linearize each (point, key frame) pair (residual, 2x6 pose and
+//               2x3 point Jacobians, robust weight), accumulate V_i, e_b, W_ij, U_j, e_a, damp and
+//               pseudo-invert V_i, and subtract the Schur pair products W_ij V_i^-1 W_ik^T from a shared-memory
+//               partial of the reduced camera system S, flushed once per chunk with FP32 atomics.
+//               (reference phases: calc_jacobians_kernel sba_v1.cu:361-460, build_full_system_1/2 :486-631,
+//               reduced_system_stage_1/12/2/3 :1016-1204; CPU twin schur_complement_bundler_cpu.cpp:64-287)
+//   k_factor    per window: assemble S + lambda diag(U) in FP64, blocked right-looking Cholesky with the
+//               forward substitution fused, backward substitution, then Exp(dc) * T re-orthonormalized in
+//               FP64 as the trial key frames (reference: cusolverDnSpotrf/Spotrs sba.cpp:606-640,
+//               calc_pose_update_kernel sba_v1.cu:823-829, UpdateState cpu.cpp:367-383)
+//   k_backsub   per chunk: dp_i = V_i^-1 (e_b - sum_j W_ij^T dc_j), the trial points, the trial cost in FP64,
+//               and the predicted-reduction terms dT H d and dT diag(H) d accumulated per observation
+//               (reference: calc_point_update_kernel sba_v1.cu:749-786, evaluate_cost_kernel :633-712,
+//               predict_reduction sba.cpp:688-727)
+//   k_decide    per window: the LM accept/reject decision, damping schedule and stop tests on the device
+//               (reference host loop schur_complement_bundler_gpu.cpp:175-253)
+// Nothing returns to the host between iterations, so the iterations of a batch replay as one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+
+#include "sba_device.h"
+
+using namespace sba_dev;
+
+namespace {
+
+constexpr float kHither = 0.1f;                  // epipolar/near_plane.h:33-38
+constexpr float kFltEps = 1.1920928955078125e-7f;  // common/types.h:203-206
+constexpr float kSqrtFltEps = 3.4526698300124393e-4f;
+
+__device__ __forceinline__ float drho(int kind, float s, float d) {
+  // math/robust_cost_function.cpp:62-95 (derivative w.r.t. the squared residual)
+  if (kind == 0) return (d + 1.f) / (d + s);
+  if (d <= 0.f) return 0.5f;
+  if (s < d * d) return 0.5f;
+  return 0.5f * d * rsqrtf(s);
+}
+
+__device__ __forceinline__ double rho64(int kind, double s, double d) {
+  if (kind == 0) return (d + 1.0) * log1p(s / d);
+  if (d <= 0.0) return 0.5 * s;
+  if (s < d * d) return 0.5 * s;
+  return d * sqrt(s) - 0.5 * d * d;
+}
+
+__device__ __forceinline__ float4 ld_pt(const float4* p, int i) { return p[i]; }
+
+// A pose in shared memory: 12 floats, rows of [R | t].
+struct Lin {
+  float r0, r1, w;
+  float jc[2][6];
+  float jp[2][3];
+};
+
+// Residual, Jacobians and robust weight of one observation (cpu.cpp:84-125). J_c is with respect to a
+// left-multiplied twist (omega, v) of rig_from_world, computed as dproj * R_cr * [-[p_r]x, I]: the same
+// matrix as the reference's dproj * [-[p_c]x, I] * Adjoint(camera_from_rig).
+__device__ __forceinline__ bool linearize(const float* P, const float* C, float px, float py, float pz, float ox,
+                                          float oy, float la, float lb, float lc, int kind, float delta, Lin& L) {
+  const float prx = P[0] * px + P[1] * py + P[2] * pz + P[3];
+  const float pry = P[4] * px + P[5] * py + P[6] * pz + P[7];
+  const float prz = P[8] * px + P[9] * py + P[10] * pz + P[11];
+  const float pcx = C[0] * prx + C[1] * pry + C[2] * prz + C[3];
+  const float pcy = C[4] * prx + C[5] * pry + C[6] * prz + C[7];
+  const float pcz = C[8] * prx + C[9] * pry + C[10] * prz + C[11];
+  if (!(pcz > kHither)) return false;
+  const float iz = 1.f / pcz;
+  const float u = pcx * iz, v = pcy * iz;
+  L.r0 = ox - u;
+  L.r1 = oy - v;
+  // A = dproj * R_cr (2x3)
+  float A[2][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    A[0][k] = iz * (C[k] - u * C[8 + k]);
+    A[1][k] = iz * (C[4 + k] - v * C[8 + k]);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    // row a of A * (-[p_r]x) = p_r x A_a
+    L.jc[a][0] = pry * A[a][2] - prz * A[a][1];
+    L.jc[a][1] = prz * A[a][0] - prx * A[a][2];
+    L.jc[a][2] = prx * A[a][1] - pry * A[a][0];
+    L.jc[a][3] = A[a][0];
+    L.jc[a][4] = A[a][1];
+    L.jc[a][5] = A[a][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L.jp[a][k] = A[a][0] * P[k] + A[a][1] * P[4 + k] + A[a][2] * P[8 + k];
+  }
+  const float s = L.r0 * (la * L.r0 + lb * L.r1) + L.r1 * (lb * L.r0 + lc * L.r1);
+  L.w = drho(kind, s, delta);
+  return true;
+}
+
+// Cyclic Jacobi on a symmetric 3x3 (Golub & Van Loan, Algorithm 8.5.3); a becomes diagonal, v holds the
+// eigenvectors as columns.
+__device__ __forceinline__ void jrot(float a[3][3], float v[3][3], int p, int q) {
+  const float apq = a[p][q];
+  if (apq == 0.f) return;
+  const float tau = (a[q][q] - a[p][p]) / (2.f * apq);
+  const float t = (tau >= 0.f ? 1.f : -1.f) / (fabsf(tau) + sqrtf(1.f + tau * tau));
+  const float c = 1.f / sqrtf(1.f + t * t);
+  const float s = t * c;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float akp = a[k][p], akq = a[k][q];
+    a[k][p] = c * akp - s * akq;
+    a[k][q] = s * akp + c * akq;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float apk = a[p][k], aqk = a[q][k];
+    a[p][k] = c * apk - s * aqk;
+    a[q][k] = s * apk + c * aqk;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float vkp = v[k][p], vkq = v[k][q];
+    v[k][p] = c * vkp - s * vkq;
+    v[k][q] = s * vkp + c * vkq;
+  }
+}
+
+// Symmetric factor G of the point-block pseudo-inverse V*^+ = G G^T with the reference's rank rule
+// (cpu.cpp:218-262, sba_v1.cu:1016-1069): eigenvalues above 1e-6 of the largest count; rank 3 inverts,
+// rank 2 drops the smallest direction, rank <= 1 gives zero.
+__device__ __forceinline__ void point_factor(const float V[6], float lambda, float G[3][3]) {
+  float a[3][3] = {{V[0] * (1.f + lambda), V[1], V[2]}, {V[1], V[3] * (1.f + lambda), V[4]},
+                   {V[2], V[4], V[5] * (1.f + lambda)}};
+  float v[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+  const float fro = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2] +
+                    2.f * (a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2]);
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    const float off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    if (off <= 1e-14f * fro) break;
+    jrot(a, v, 0, 1);
+    jrot(a, v, 0, 2);
+    jrot(a, v, 1, 2);
+  }
+  const float e[3] = {a[0][0], a[1][1], a[2][2]};
+  const float smax = fmaxf(fabsf(e[0]), fmaxf(fabsf(e[1]), fabsf(e[2])));
+  int rank = 0;
+  float sc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool keep = smax > 0.f && fabsf(e[k]) > 1e-6f * smax && e[k] > 0.f;
+    rank += keep;
+    sc[k] = keep ? rsqrtf(e[k]) : 0.f;
+  }
+  if (rank < 2) sc[0] = sc[1] = sc[2] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G[i][k] = v[i][k] * sc[k];
+}
+
+__device__ __forceinline__ int upper_block(int j, int k, int ncf) {  // j <= k < ncf
+  return j * ncf - (j * (j - 1)) / 2 + (k - j);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
+
+__device__ __forceinline__ void load_pose_lds(float* dst, const float4* src, int n3, int tid, int nthreads) {
+  for (int k = tid; k < n3; k += nthreads) {
+    const float4 q = src[k];
+    dst[4 * k + 0] = q.x;
+    dst[4 * k + 1] = q.y;
+    dst[4 * k + 2] = q.z;
+    dst[4 * k + 3] = q.w;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------
+// Linearize + Schur complement. One workgroup per chunk (consecutive tiles of one window's points).
+// ---------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
+  extern __shared__ float sm[];
+  const Chunk ch = b.chunks[blockIdx.x];
+  const int w = b.tiles[ch.tile_begin].window;
+  const WindowDesc W = b.win[w];
+  const WindowState* St = b.st + w;
+  if (St->done) return;
+  const int cur = St->cur;
+  const float lam = St->lambda;
+  const float4* __restrict__ pts = cur ? b.pts1 : b.pts0;
+  const float4* __restrict__ poses = cur ? b.poses1 : b.poses0;
+  const int tid = threadIdx.x;
+  const int ncf = W.ncf;
+  const int nb = ncf * (ncf + 1) / 2;
+  float* sPose = sm + b.lds_pose;
+  float* sCam = sm + b.lds_cam;
+  float* sS = sm + b.lds_S;
+  float* sE = sm + b.lds_e;
+  float* sDu = sm + b.lds_du;
+  float* sAcc = sm + b.lds_acc;  // 12 per tile point: V (6), e_b (3) then G (9) and G^T e_b (3)
+  float* sZ = sm + b.lds_Z;      // 18 per tile pair: W_ij (6x3) then Z_ij = W_ij G_i
+  int* sPP = reinterpret_cast<int*>(sm + b.lds_pp);
+  int* sPre = reinterpret_cast<int*>(sm + b.lds_pre);
+  __shared__ int sWave[kBlock / 64];
+
+  load_pose_lds(sPose, poses + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+  load_pose_lds(sCam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
+  for (int k = tid; k < 36 * nb; k += kBlock) sS[k] = 0.f;
+  for (int k = tid; k < 6 * ncf; k += kBlock) {
+    sE[k] = 0.f;
+    sDu[k] = 0.f;
+  }
+  const int kind = b.loss;
+  const float delta = W.robust;
+  const int pf_end = W.point_off + W.npf;  // global index one past the last free point
+
+  for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
+    const Tile T = b.tiles[t];
+    const int p0 = b.point_pair[T.point_begin];
+    const int npair = b.point_pair[T.point_end] - p0;
+    const int npts = T.point_end - T.point_begin;
+    __syncthreads();  // previous tile's shared arrays are free; pose/S init visible
+    // (a) pair -> tile point map, zero the point accumulators, pair-pair counts
+    int q = 0;
+    if (tid < npts) {
+      const int i = T.point_begin + tid;
+      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
+      for (int k = a; k < e; ++k) sPP[k] = tid;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) sAcc[12 * tid + k] = 0.f;
+      q = (i < pf_end) ? b.point_nfree[i] : 0;
+    }
+    {
+      const int cnt = q * (q + 1) / 2;
+      const int incl = wave_incl_scan(cnt);
+      if ((tid & 63) == 63) sWave[tid >> 6] = incl;
+      __syncthreads();
+      int base = 0;
+      for (int k = 0; k < (tid >> 6); ++k) base += sWave[k];
+      if (tid < npts) sPre[tid + 1] = base + incl;
+      if (tid == 0) sPre[0] = 0;
+    }
+    __syncthreads();
+    // (b) one thread per pair: linearize its observations (one per camera) and accumulate
+    if (tid < npair) {
+      const int pair = p0 + tid;
+      const int tp = sPP[tid];
+      const int i = T.point_begin + tp;
+      const int j = b.pair_pose[pair];
+      const bool point_free = i < pf_end;
+      const bool pose_free = j < ncf;
+      const float4 p = ld_pt(pts, i);
+      float Wm[6][3] = {}, U[21] = {}, ea[6] = {}, V[6] = {}, eb[3] = {};
+      const int o_end = b.pair_obs[pair + 1];
+      for (int o = b.pair_obs[pair]; o < o_end; ++o) {
+        const float2 xy = b.obs_xy[o];
+        const float4 inf = b.obs_info[o];
+        const int cam = __float_as_int(inf.w);
+        Lin L;
+        if (!linearize(sPose + 12 * j, sCam + 12 * cam, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L))
+          continue;
+        // Lambda * J and Lambda * r, weighted
+        float LJc[2][6], LJp[2][3];
+        const float wr0 = L.w * (inf.x * L.r0 + inf.y * L.r1);
+        const float wr1 = L.w * (inf.y * L.r0 + inf.z * L.r1);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          LJc[0][k] = L.w * (inf.x * L.jc[0][k] + inf.y * L.jc[1][k]);
+          LJc[1][k] = L.w * (inf.y * L.jc[0][k] + inf.z * L.jc[1][k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          LJp[0][k] = L.w * (inf.x * L.jp[0][k] + inf.y * L.jp[1][k]);
+          LJp[1][k] = L.w * (inf.y * L.jp[0][k] + inf.z * L.jp[1][k]);
+        }
+        if (pose_free) {
+          int u = 0;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+#pragma unroll
+            for (int c = r; c < 6; ++c) U[u++] += L.jc[0][r] * LJc[0][c] + L.jc[1][r] * LJc[1][c];
+            ea[r] += L.jc[0][r] * wr0 + L.jc[1][r] * wr1;
+          }
+          if (point_free) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Wm[r][c] += L.jc[0][r] * LJp[0][c] + L.jc[1][r] * LJp[1][c];
+          }
+        }
+        if (point_free) {
+          V[0] += L.jp[0][0] * LJp[0][0] + L.jp[1][0] * LJp[1][0];
+          V[1] += L.jp[0][0] * LJp[0][1] + L.jp[1][0] * LJp[1][1];
+          V[2] += L.jp[0][0] * LJp[0][2] + L.jp[1][0] * LJp[1][2];
+          V[3] += L.jp[0][1] * LJp[0][1] + L.jp[1][1] * LJp[1][1];
+          V[4] += L.jp[0][1] * LJp[0][2] + L.jp[1][1] * LJp[1][2];
+          V[5] += L.jp[0][2] * LJp[0][2] + L.jp[1][2] * LJp[1][2];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) eb[k] += L.jp[0][k] * wr0 + L.jp[1][k] * wr1;
+        }
+      }
+      if (point_free) {
+        float* acc = sAcc + 12 * tp;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lds_add(acc + k, V[k]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lds_add(acc + 6 + k, eb[k]);
+      }
+      if (pose_free) {
+        float* blk = sS + 36 * upper_block(j, j, ncf);
+        int u = 0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = r; c < 6; ++c) lds_add(blk + 6 * r + c, U[u++]);
+        int d = 0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          lds_add(sE + 6 * j + r, ea[r]);
+          lds_add(sDu + 6 * j + r, U[d]);
+          d += 6 - r;
+        }
+        if (point_free) {
+          float* z = sZ + 18 * tid;
+#pragma unroll
+          for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) z[3 * r + c] = Wm[r][c];
+        }
+      }
+    }
+    __syncthreads();
+    // (c) one thread per free point: damped pseudo-inverse V*^+ = G G^T
+    if (tid < npts) {
+      const int i = T.point_begin + tid;
+      if (i < pf_end) {
+        float* acc = sAcc + 12 * tid;
+        const float V[6] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]};
+        const float eb[3] = {acc[6], acc[7], acc[8]};
+        float G[3][3];
+        point_factor(V, lam, G);
+        float tv[3], Vi[6], gb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) tv[k] = G[0][k] * eb[0] + G[1][k] * eb[1] + G[2][k] * eb[2];
+        Vi[0] = G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2];
+        Vi[1] = G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2];
+        Vi[2] = G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2];
+        Vi[3] = G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2];
+        Vi[4] = G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2];
+        Vi[5] = G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gb[k] = G[k][0] * tv[0] + G[k][1] * tv[1] + G[k][2] * tv[2];
+        float4* ax = b.paux + 3 * i;
+        ax[0] = make_float4(Vi[0], Vi[1], Vi[2], Vi[3]);
+        ax[1] = make_float4(Vi[4], Vi[5], gb[0], gb[1]);
+        ax[2] = make_float4(gb[2], V[0], V[3], V[5]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[3 * r + c] = G[r][c];
+        acc[9] = tv[0];
+        acc[10] = tv[1];
+        acc[11] = tv[2];
+      }
+    }
+    __syncthreads();
+    // (d) one thread per (free point, free key frame) pair: Z = W G, e_j -= Z G^T e_b
+    if (tid < npair) {
+      const int pair = p0 + tid;
+      const int tp = sPP[tid];
+      const int i = T.point_begin + tp;
+      const int j = b.pair_pose[pair];
+      if (i < pf_end && j < ncf) {
+        const float* acc = sAcc + 12 * tp;
+        float* z = sZ + 18 * tid;
+        float Wm[6][3];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Wm[r][c] = z[3 * r + c];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          float zr[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) zr[c] = Wm[r][0] * acc[c] + Wm[r][1] * acc[3 + c] + Wm[r][2] * acc[6 + c];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) z[3 * r + c] = zr[c];
+          lds_add(sE + 6 * j + r, -(zr[0] * acc[9] + zr[1] * acc[10] + zr[2] * acc[11]));
+        }
+      }
+    }
+    __syncthreads();
+    // (e) Schur pair products S_{j_a j_b} -= Z_a Z_b^T over each point's free pairs a <= b
+    const int total = sPre[npts];
+    for (int uidx = tid; uidx < total; uidx += kBlock) {
+      int lo = 0, hi = npts;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sPre[mid] <= uidx) lo = mid; else hi = mid;
+      }
+      const int l = uidx - sPre[lo];
+      int bb = static_cast<int>((sqrtf(8.f * l + 1.f) - 1.f) * 0.5f);
+      while ((bb + 1) * (bb + 2) / 2 <= l) ++bb;
+      while (bb * (bb + 1) / 2 > l) --bb;
+      const int aa = l - bb * (bb + 1) / 2;
+      const int pbase = b.point_pair[T.point_begin + lo] - p0;
+      const int ka = pbase + aa, kb = pbase + bb;
+      const int ja = b.pair_pose[p0 + ka], jb = b.pair_pose[p0 + kb];
+      const float* za = sZ + 18 * ka;
+      const float* zb = sZ + 18 * kb;
+      float A[6][3], B[6][3];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          A[r][c] = za[3 * r + c];
+          B[r][c] = zb[3 * r + c];
+        }
+      float* blk = sS + 36 * upper_block(ja, jb, ncf);
+      if (aa == bb) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = r; c < 6; ++c) lds_add(blk + 6 * r + c, -(A[r][0] * B[c][0] + A[r][1] * B[c][1] + A[r][2] * B[c][2]));
+      } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) lds_add(blk + 6 * r + c, -(A[r][0] * B[c][0] + A[r][1] * B[c][1] + A[r][2] * B[c][2]));
+      }
+    }
+  }
+  __syncthreads();
+  // flush the chunk's partial reduced system into the window's accumulator
+  float* g = b.s_acc + W.s_off;
+  for (int k = tid; k < 36 * nb; k += kBlock) {
+    const float v = sS[k];
+    if (v != 0.f) unsafeAtomicAdd(g + k, v);
+  }
+  for (int k = tid; k < 6 * ncf; k += kBlock) {
+    const float ve = sE[k], vd = sDu[k];
+    if (ve != 0.f) unsafeAtomicAdd(g + 36 * nb + k, ve);
+    if (vd != 0.f) unsafeAtomicAdd(g + 36 * nb + 6 * ncf + k, vd);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// FP64 factor/solve of the reduced camera system, one workgroup per window.
+// ---------------------------------------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int64_t pk(int i, int j) { return static_cast<int64_t>(i) * (i + 1) / 2 + j; }
+
+// Exp of a twist (omega, v) in FP64, the formula of math/twist.cpp:23-41 (theta clamped at FLT_EPSILON).
+__device__ void twist_exp64(const double* tw, double R[3][3], double t[3]) {
+  const double w0 = tw[0], w1 = tw[1], w2 = tw[2];
+  const double theta = fmax(sqrt(w0 * w0 + w1 * w1 + w2 * w2), static_cast<double>(kFltEps));
+  const double c = cos(theta), st = sin(theta) / theta;
+  const double n[3] = {w0 / theta, w1 / theta, w2 / theta};
+  const double ws[3] = {w0 * st, w1 * st, w2 * st};
+  const double K[3][3] = {{c, -ws[2], ws[1]}, {ws[2], c, -ws[0]}, {-ws[1], ws[0], c}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = K[i][j] + (1.0 - c) * n[i] * n[j];
+  const double* v = tw + 3;
+  const double nxv[3] = {n[1] * v[2] - n[2] * v[1], n[2] * v[0] - n[0] * v[2], n[0] * v[1] - n[1] * v[0]};
+  const double nnv[3] = {n[1] * nxv[2] - n[2] * nxv[1], n[2] * nxv[0] - n[0] * nxv[2], n[0] * nxv[1] - n[1] * nxv[0]};
+  for (int i = 0; i < 3; ++i) t[i] = v[i] + ((1.0 - c) / theta * nxv[i] + (1.0 - st) * nnv[i]);
+}
+
+// Orthonormal polar factor of a near-rotation (Newton iteration X <- (X + X^-T) / 2, Higham 1986); for a
+// matrix with positive determinant it is U V^T of its SVD, which common/rotation_utils.h:36-50 returns.
+__device__ void polar3(double X[3][3]) {
+  for (int it = 0; it < 3; ++it) {
+    const double c00 = X[1][1] * X[2][2] - X[1][2] * X[2][1];
+    const double c01 = X[1][2] * X[2][0] - X[1][0] * X[2][2];
+    const double c02 = X[1][0] * X[2][1] - X[1][1] * X[2][0];
+    const double c10 = X[0][2] * X[2][1] - X[0][1] * X[2][2];
+    const double c11 = X[0][0] * X[2][2] - X[0][2] * X[2][0];
+    const double c12 = X[0][1] * X[2][0] - X[0][0] * X[2][1];
+    const double c20 = X[0][1] * X[1][2] - X[0][2] * X[1][1];
+    const double c21 = X[0][2] * X[1][0] - X[0][0] * X[1][2];
+    const double c22 = X[0][0] * X[1][1] - X[0][1] * X[1][0];
+    const double det = X[0][0] * c00 + X[0][1] * c01 + X[0][2] * c02;
+    const double id = 1.0 / det;  // X^-T = cofactor(X) / det
+    const double C[3][3] = {{c00, c01, c02}, {c10, c11, c12}, {c20, c21, c22}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) X[i][j] = 0.5 * (X[i][j] + C[i][j] * id);
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_S) {
+  extern __shared__ double smd[];
+  const int w = blockIdx.x;
+  const WindowDesc W = b.win[w];
+  WindowState* St = b.st + w;
+  if (St->done) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int warp = tid >> 6;
+  const int ncf = W.ncf;
+  const int n = 6 * ncf;
+  const int nb = ncf * (ncf + 1) / 2;
+  const float lam = St->lambda;
+  double* vec = smd;               // 256: right-hand side, then y, then x
+  double* du = smd + 256;          // 256: diag(U)
+  double* panel = smd + 512;       // 256 x kPanel
+  double* red = panel + 256 * kPanel;  // 32: reductions
+  int* flag = reinterpret_cast<int*>(red + 32);
+  double* Lp = W.factor_in_lds ? (red + 48) : (b.A + W.a_off);
+  float* g = b.s_acc + W.s_off;
+
+  // 1. assemble A = S + lambda diag(U) (packed lower) and e; clear the accumulator for the next iteration
+  const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
+  for (int64_t idx = tid; idx < npk; idx += kFactorThreads) {
+    int i = static_cast<int>((sqrt(8.0 * static_cast<double>(idx) + 1.0) - 1.0) * 0.5);
+    while (pk(i + 1, 0) <= idx) ++i;
+    while (pk(i, 0) > idx) --i;
+    const int j = static_cast<int>(idx - pk(i, 0));
+    const int bi = i / 6, ri = i - 6 * bi, bj = j / 6, rj = j - 6 * bj;
+    float* src = g + 36 * upper_block(bj, bi, ncf) + 6 * rj + ri;
+    double v = static_cast<double>(*src);
+    *src = 0.f;
+    if (i == j) v += static_cast<double>(lam) * static_cast<double>(g[36 * nb + 6 * ncf + i]);
+    Lp[idx] = v;
+  }
+  __syncthreads();  // the diag(U) reads above happen before the clears below
+  for (int i = tid; i < n; i += kFactorThreads) {
+    vec[i] = static_cast<double>(g[36 * nb + i]);
+    du[i] = static_cast<double>(g[36 * nb + 6 * ncf + i]);
+    g[36 * nb + i] = 0.f;
+    g[36 * nb + 6 * ncf + i] = 0.f;
+  }
+  if (tid == 0) flag[0] = 0;
+  __syncthreads();
+  if (dbg_S && w == 0) {
+    for (int64_t idx = tid; idx < static_cast<int64_t>(n) * n; idx += kFactorThreads) {
+      const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
+      dbg_S[idx] = i >= j ? Lp[pk(i, j)] : Lp[pk(j, i)];
+    }
+    for (int i = tid; i < n; i += kFactorThreads) dbg_S[static_cast<int64_t>(n) * n + i] = vec[i];
+    __syncthreads();
+  }
+
+  // 2. blocked right-looking Cholesky, forward substitution fused into the panel step
+  for (int k0 = 0; k0 < n; k0 += kPanel) {
+    const int kb = min(kPanel, n - k0);
+    if (warp == 0) {
+      double p[4][kPanel];
+      double rhs[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = k0 + lane + 64 * s;
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) p[s][c] = (row < n && c < kb && row >= k0 + c) ? Lp[pk(row, k0 + c)] : 0.0;
+        rhs[s] = row < n ? vec[row] : 0.0;
+      }
+      bool ok = true;
+#pragma unroll
+      for (int c = 0; c < kPanel; ++c) {
+        if (c < kb) {
+          const double d = readlane_d(p[0][c], c);
+          if (!(d > 0.0)) ok = false;
+          const double l = sqrt(fmax(d, 1e-300));
+          const double il = 1.0 / l;
+          const double y = readlane_d(rhs[0], c) * il;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int row = k0 + lane + 64 * s;
+            if (row > k0 + c) {
+              p[s][c] *= il;
+              rhs[s] -= p[s][c] * y;
+            } else if (row == k0 + c) {
+              p[s][c] = l;
+              rhs[s] = y;
+            }
+          }
+#pragma unroll
+          for (int c2 = c + 1; c2 < kPanel; ++c2) {
+            if (c2 < kb) {
+              const double l2 = readlane_d(p[0][c], c2);
+#pragma unroll
+              for (int s = 0; s < 4; ++s) {
+                const int row = k0 + lane + 64 * s;
+                if (row >= k0 + c2) p[s][c2] -= p[s][c] * l2;
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = k0 + lane + 64 * s;
+        if (row < n) {
+#pragma unroll
+          for (int c = 0; c < kPanel; ++c) {
+            if (c < kb && row >= k0 + c) Lp[pk(row, k0 + c)] = p[s][c];
+            panel[(row - k0) * kPanel + c] = c < kb ? p[s][c] : 0.0;
+          }
+          vec[row] = rhs[s];
+        }
+      }
+      if (!ok && lane == 0) flag[0] = 1;
+    }
+    __syncthreads();
+    if (flag[0]) break;
+    // trailing update of rows/columns >= k0 + kb in 4x4 tiles of the lower triangle
+    const int t0 = k0 + kb;
+    const int tn = n - t0;
+    if (tn > 0) {
+      const int T4 = (tn + 3) >> 2;
+      const int ntiles = T4 * (T4 + 1) / 2;
+      for (int tile = tid; tile < ntiles; tile += kFactorThreads) {
+        int I = static_cast<int>((sqrtf(8.f * tile + 1.f) - 1.f) * 0.5f);
+        while ((I + 1) * (I + 2) / 2 <= tile) ++I;
+        while (I * (I + 1) / 2 > tile) --I;
+        const int J = tile - I * (I + 1) / 2;
+        double li[4][kPanel], lj[4][kPanel];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int ri = t0 + 4 * I + a - k0, rj = t0 + 4 * J + a - k0;
+#pragma unroll
+          for (int c = 0; c < kPanel; ++c) {
+            li[a][c] = (t0 + 4 * I + a < n) ? panel[ri * kPanel + c] : 0.0;
+            lj[a][c] = (t0 + 4 * J + a < n) ? panel[rj * kPanel + c] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = t0 + 4 * I + a;
+          if (i >= n) continue;
+#pragma unroll
+          for (int c2 = 0; c2 < 4; ++c2) {
+            const int j = t0 + 4 * J + c2;
+            if (j > i) continue;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) s += li[a][c] * lj[c2][c];
+            Lp[pk(i, j)] -= s;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const bool ok = flag[0] == 0;
+
+  if (ok) {
+    // 3. backward substitution L^T x = y, blocks of kPanel rows from the bottom
+    const int nblk = (n + kPanel - 1) / kPanel;
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+      const int k0 = blk * kPanel;
+      const int kb = min(kPanel, n - k0);
+      if (tid == 0) {
+        for (int c = kb - 1; c >= 0; --c) {
+          const int row = k0 + c;
+          double s = vec[row];
+          for (int c2 = c + 1; c2 < kb; ++c2) s -= Lp[pk(k0 + c2, row)] * vec[k0 + c2];
+          vec[row] = s / Lp[pk(row, row)];
+        }
+      }
+      __syncthreads();
+      for (int i = tid; i < k0; i += kFactorThreads) {
+        double s = 0.0;
+        for (int c = 0; c < kb; ++c) s += Lp[pk(k0 + c, i)] * vec[k0 + c];
+        vec[i] -= s;
+      }
+      __syncthreads();
+    }
+    // 4. the pose step, its max-norm and dc^T diag(U) dc
+    double sc = 0.0;
+    float mx = 0.f;
+    for (int i = tid; i < n; i += kFactorThreads) {
+      const double x = vec[i];
+      b.dc[W.dc_off + i] = static_cast<float>(x);
+      b.dc64[W.dc_off + i] = x;
+      sc += x * x * du[i];
+      mx = fmaxf(mx, fabsf(static_cast<float>(x)));
+    }
+    sc = wave_sum(sc);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) {
+      red[warp] = sc;
+      red[16 + warp] = static_cast<double>(mx);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0, m = 0.0;
+      for (int k = 0; k < kFactorThreads / 64; ++k) {
+        s += red[k];
+        m = fmax(m, red[16 + k]);
+      }
+      St->scal_pose = s;
+      St->maxdc = static_cast<float>(m);
+    }
+    // 5. trial key frames: Exp(dc_j) * T_j, rotation re-orthonormalized, rounded to FP32
+    const int cur = St->cur;
+    const float4* src = (cur ? b.poses1 : b.poses0) + 3 * W.pose_off;
+    float4* dst = (cur ? b.poses0 : b.poses1) + 3 * W.pose_off;
+    for (int j = tid; j < W.nc; j += kFactorThreads) {
+      const float4 r0 = src[3 * j], r1 = src[3 * j + 1], r2 = src[3 * j + 2];
+      if (j >= ncf) {
+        dst[3 * j] = r0;
+        dst[3 * j + 1] = r1;
+        dst[3 * j + 2] = r2;
+        continue;
+      }
+      double E[3][3], et[3];
+      twist_exp64(vec + 6 * j, E, et);
+      const double R[3][3] = {{r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, {r2.x, r2.y, r2.z}};
+      const double t[3] = {r0.w, r1.w, r2.w};
+      double X[3][3], tn[3];
+      for (int a = 0; a < 3; ++a) {
+        for (int c = 0; c < 3; ++c) X[a][c] = E[a][0] * R[0][c] + E[a][1] * R[1][c] + E[a][2] * R[2][c];
+        tn[a] = E[a][0] * t[0] + E[a][1] * t[1] + E[a][2] * t[2] + et[a];
+      }
+      polar3(X);
+      dst[3 * j] = make_float4(static_cast<float>(X[0][0]), static_cast<float>(X[0][1]), static_cast<float>(X[0][2]), static_cast<float>(tn[0]));
+      dst[3 * j + 1] = make_float4(static_cast<float>(X[1][0]), static_cast<float>(X[1][1]), static_cast<float>(X[1][2]), static_cast<float>(tn[1]));
+      dst[3 * j + 2] = make_float4(static_cast<float>(X[2][0]), static_cast<float>(X[2][1]), static_cast<float>(X[2][2]), static_cast<float>(tn[2]));
+    }
+  }
+  if (tid == 0) St->chol_ok = ok ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Back-substitution, trial cost (FP64) and predicted-reduction terms. INITIAL: cost of the current state.
+// ---------------------------------------------------------------------------------------------------------
+template <bool INITIAL>
+__global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
+  __shared__ float sPose[12 * kMaxPoses];
+  __shared__ float sTrial[12 * kMaxPoses];
+  __shared__ float sCam[12 * kMaxCameras];
+  __shared__ float sAcc[4 * kTilePoints];  // sum_j W_ij^T dc_j, then dp
+  __shared__ int sPP[kTilePairs];
+  __shared__ float sDc[6 * kMaxFreePoses];
+  __shared__ double sRed[3 * (kBlock / 64)];
+  __shared__ int sRedI[kBlock / 64];
+  __shared__ float sRedM[kBlock / 64];
+  const Chunk ch = b.chunks[blockIdx.x];
+  const int w = b.tiles[ch.tile_begin].window;
+  const WindowDesc W = b.win[w];
+  WindowState* St = b.st + w;
+  if (St->done) return;
+  if (!INITIAL && !St->chol_ok) return;
+  const int cur = St->cur;
+  const int tid = threadIdx.x;
+  const int ncf = W.ncf;
+  const float4* pts = cur ? b.pts1 : b.pts0;
+  float4* pts_trial = cur ? b.pts0 : b.pts1;
+  load_pose_lds(sPose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+  if (!INITIAL) {
+    load_pose_lds(sTrial, (cur ? b.poses0 : b.poses1) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+    for (int k = tid; k < 6 * ncf; k += kBlock) sDc[k] = b.dc[W.dc_off + k];
+  }
+  load_pose_lds(sCam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
+  const int kind = b.loss;
+  const float delta = W.robust;
+  const double delta64 = static_cast<double>(W.robust);
+  const int pf_end = W.point_off + W.npf;
+  double cost = 0.0, hess = 0.0, scal = 0.0;
+  int skipped = 0;
+  float maxdp = 0.f;
+
+  for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
+    const Tile T = b.tiles[t];
+    const int p0 = b.point_pair[T.point_begin];
+    const int npair = b.point_pair[T.point_end] - p0;
+    const int npts = T.point_end - T.point_begin;
+    __syncthreads();
+    if (tid < npts) {
+      const int i = T.point_begin + tid;
+      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
+      for (int k = a; k < e; ++k) sPP[k] = tid;
+      sAcc[4 * tid] = sAcc[4 * tid + 1] = sAcc[4 * tid + 2] = sAcc[4 * tid + 3] = 0.f;
+    }
+    __syncthreads();
+    if (!INITIAL) {
+      // (a) sum_j W_ij^T dc_j = sum_obs w J_p^T Lambda J_c dc_j
+      if (tid < npair) {
+        const int pair = p0 + tid;
+        const int tp = sPP[tid];
+        const int i = T.point_begin + tp;
+        const int j = b.pair_pose[pair];
+        if (i < pf_end && j < ncf) {
+          const float4 p = pts[i];
+          const float* d = sDc + 6 * j;
+          float acc[3] = {0.f, 0.f, 0.f};
+          for (int o = b.pair_obs[pair]; o < b.pair_obs[pair + 1]; ++o) {
+            const float2 xy = b.obs_xy[o];
+            const float4 inf = b.obs_info[o];
+            Lin L;
+            if (!linearize(sPose + 12 * j, sCam + 12 * __float_as_int(inf.w), p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y,
+                           inf.z, kind, delta, L))
+              continue;
+            float u0 = 0.f, u1 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+              u0 += L.jc[0][k] * d[k];
+              u1 += L.jc[1][k] * d[k];
+            }
+            const float lu0 = L.w * (inf.x * u0 + inf.y * u1), lu1 = L.w * (inf.y * u0 + inf.z * u1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += L.jp[0][k] * lu0 + L.jp[1][k] * lu1;
+          }
+          lds_add(sAcc + 4 * tp, acc[0]);
+          lds_add(sAcc + 4 * tp + 1, acc[1]);
+          lds_add(sAcc + 4 * tp + 2, acc[2]);
+        }
+      }
+      __syncthreads();
+      // (b) one thread per point: dp = V*^-1 e_b - V*^-1 (sum W^T dc); trial point
+      if (tid < npts) {
+        const int i = T.point_begin + tid;
+        const float4 p = pts[i];
+        float dp[3] = {0.f, 0.f, 0.f};
+        if (i < pf_end) {
+          const float4 a0 = b.paux[3 * i], a1 = b.paux[3 * i + 1], a2 = b.paux[3 * i + 2];
+          const float Vi[3][3] = {{a0.x, a0.y, a0.z}, {a0.y, a0.w, a1.x}, {a0.z, a1.x, a1.y}};
+          const float s[3] = {sAcc[4 * tid], sAcc[4 * tid + 1], sAcc[4 * tid + 2]};
+          const float gb[3] = {a1.z, a1.w, a2.x};
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dp[k] = gb[k] - (Vi[k][0] * s[0] + Vi[k][1] * s[1] + Vi[k][2] * s[2]);
+          const double d0 = dp[0], d1 = dp[1], d2 = dp[2];
+          scal += d0 * d0 * a2.y + d1 * d1 * a2.z + d2 * d2 * a2.w;
+          maxdp = fmaxf(maxdp, fmaxf(fabsf(dp[0]), fmaxf(fabsf(dp[1]), fabsf(dp[2]))));
+        }
+        sAcc[4 * tid] = dp[0];
+        sAcc[4 * tid + 1] = dp[1];
+        sAcc[4 * tid + 2] = dp[2];
+        pts_trial[i] = make_float4(p.x + dp[0], p.y + dp[1], p.z + dp[2], 0.f);
+        if (b.dbg_dp && w == 0) b.dbg_dp[i - W.point_off] = make_float4(dp[0], dp[1], dp[2], 0.f);
+      }
+      __syncthreads();
+    }
+    // (c) one thread per pair: trial cost in FP64, and w |J_c dc + J_p dp|^2_Lambda
+    if (tid < npair) {
+      const int pair = p0 + tid;
+      const int tp = sPP[tid];
+      const int i = T.point_begin + tp;
+      const int j = b.pair_pose[pair];
+      const float4 p = pts[i];
+      float dp[3] = {0.f, 0.f, 0.f}, dcj[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (!INITIAL) {
+        dp[0] = sAcc[4 * tp];
+        dp[1] = sAcc[4 * tp + 1];
+        dp[2] = sAcc[4 * tp + 2];
+        if (j < ncf)
+#pragma unroll
+          for (int k = 0; k < 6; ++k) dcj[k] = sDc[6 * j + k];
+      }
+      const float* Pt = INITIAL ? (sPose + 12 * j) : (sTrial + 12 * j);
+      const double qx = static_cast<double>(p.x + dp[0]), qy = static_cast<double>(p.y + dp[1]),
+                   qz = static_cast<double>(p.z + dp[2]);
+      const double prx = Pt[0] * qx + Pt[1] * qy + Pt[2] * qz + Pt[3];
+      const double pry = Pt[4] * qx + Pt[5] * qy + Pt[6] * qz + Pt[7];
+      const double prz = Pt[8] * qx + Pt[9] * qy + Pt[10] * qz + Pt[11];
+      for (int o = b.pair_obs[pair]; o < b.pair_obs[pair + 1]; ++o) {
+        const float2 xy = b.obs_xy[o];
+        const float4 inf = b.obs_info[o];
+        const float* C = sCam + 12 * __float_as_int(inf.w);
+        const double pcx = C[0] * prx + C[1] * pry + C[2] * prz + C[3];
+        const double pcy = C[4] * prx + C[5] * pry + C[6] * prz + C[7];
+        const double pcz = C[8] * prx + C[9] * pry + C[10] * prz + C[11];
+        if (pcz > static_cast<double>(kHither)) {
+          const double r0 = static_cast<double>(xy.x) - pcx / pcz, r1 = static_cast<double>(xy.y) - pcy / pcz;
+          const double s = r0 * (inf.x * r0 + inf.y * r1) + r1 * (inf.y * r0 + inf.z * r1);
+          cost += rho64(kind, s, delta64);
+        } else {
+          ++skipped;
+        }
+        if (!INITIAL) {
+          Lin L;
+          if (linearize(sPose + 12 * j, C, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L)) {
+            float g0 = 0.f, g1 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+              g0 += L.jc[0][k] * dcj[k];
+              g1 += L.jc[1][k] * dcj[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              g0 += L.jp[0][k] * dp[k];
+              g1 += L.jp[1][k] * dp[k];
+            }
+            hess += static_cast<double>(L.w * (g0 * (inf.x * g0 + inf.y * g1) + g1 * (inf.y * g0 + inf.z * g1)));
+          }
+        }
+      }
+    }
+  }
+  // workgroup reduction, then one set of atomics per workgroup
+  cost = wave_sum(cost);
+  hess = wave_sum(hess);
+  scal = wave_sum(scal);
+  skipped = wave_sum(skipped);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) maxdp = fmaxf(maxdp, __shfl_xor(maxdp, o));
+  const int warp = tid >> 6;
+  if ((tid & 63) == 0) {
+    sRed[3 * warp] = cost;
+    sRed[3 * warp + 1] = hess;
+    sRed[3 * warp + 2] = scal;
+    sRedI[warp] = skipped;
+    sRedM[warp] = maxdp;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double c = 0.0, h = 0.0, s = 0.0;
+    int sk = 0;
+    float m = 0.f;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      c += sRed[3 * k];
+      h += sRed[3 * k + 1];
+      s += sRed[3 * k + 2];
+      sk += sRedI[k];
+      m = fmaxf(m, sRedM[k]);
+    }
+    unsafeAtomicAdd(&St->acc_cost, c);
+    if (!INITIAL) {
+      unsafeAtomicAdd(&St->acc_hess, h);
+      unsafeAtomicAdd(&St->acc_scal_pt, s);
+      atomicMax(&St->acc_maxdp, __float_as_uint(m));
+    }
+    atomicAdd(&St->acc_skipped, sk);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// LM decision per window (schur_complement_bundler_gpu.cpp:147-253). INITIAL: the setup checks (:149-173).
+// ---------------------------------------------------------------------------------------------------------
+template <bool INITIAL>
+__global__ void k_decide(Batch b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= b.n_windows) return;
+  const WindowDesc W = b.win[w];
+  WindowState& S = b.st[w];
+  if (S.done) return;
+  const double cost = S.acc_skipped == W.no ? static_cast<double>(INFINITY) : S.acc_cost;
+  const double hess = S.acc_hess, scal_pt = S.acc_scal_pt;
+  const float maxdp = __uint_as_float(S.acc_maxdp);
+  S.acc_cost = 0.0;
+  S.acc_hess = 0.0;
+  S.acc_scal_pt = 0.0;
+  S.acc_skipped = 0;
+  S.acc_maxdp = 0u;
+  if (INITIAL) {
+    S.initial_cost = S.current_cost = S.last_cost = cost;
+    S.lambda = 0.001f;
+    if (cost < static_cast<double>(kFltEps)) {
+      S.done = 1;
+      S.status = 0;
+    } else if (isinf(cost)) {
+      S.done = 1;
+      S.status = 2;  // SBA_E_INFEASIBLE
+    } else if (W.max_iterations <= 0) {
+      S.done = 1;
+    }
+    return;
+  }
+  S.iterations += 1;
+  const bool last = S.iterations >= W.max_iterations;
+  if (!S.chol_ok) {
+    S.lambda *= 5.f;
+    if (last) S.done = 1;
+    return;
+  }
+  if (S.current_cost < S.initial_cost * static_cast<double>(kFltEps)) {
+    S.done = 1;
+    return;
+  }
+  const double lam = static_cast<double>(S.lambda);
+  const double pred = (hess + 2.0 * lam * (S.scal_pose + scal_pt)) / S.current_cost;
+  if (!b.fixed_iterations && pred < static_cast<double>(kSqrtFltEps) && S.maxdc < kSqrtFltEps && maxdp < kSqrtFltEps) {
+    S.done = 1;
+    return;
+  }
+  const double rho = (1.0 - cost / S.current_cost) / pred;
+  if (rho > 0.25) {
+    if (rho > 0.75 && S.lambda * 0.125f > 0.f) S.lambda *= 0.125f;
+    S.current_cost = cost;
+    S.last_cost = cost;
+    S.cur ^= 1;
+    S.accepted += 1;
+  } else {
+    S.lambda *= 5.f;
+  }
+  if (last) S.done = 1;
+}
+
+// Restores the uploaded initial state and clears every accumulator (the first node of a solve).
+__global__ void k_reset(Batch b) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = gid; i < b.n_points; i += stride) b.pts0[i] = b.pts_init[i];
+  for (int64_t i = gid; i < 3 * static_cast<int64_t>(b.n_poses); i += stride) b.poses0[i] = b.poses_init[i];
+  for (int64_t i = gid; i < b.s_acc_size; i += stride) b.s_acc[i] = 0.f;
+  for (int64_t i = gid; i < b.n_windows; i += stride) {
+    WindowState s = {};
+    s.lambda = 0.001f;
+    b.st[i] = s;
+  }
+}
+
+// Copies each window's final state (whichever buffer is current) to the output streams.
+__global__ void k_gather(Batch b) {
+  const int w = blockIdx.y;
+  const WindowDesc W = b.win[w];
+  const int cur = b.st[w].cur;
+  const float4* pts = cur ? b.pts1 : b.pts0;
+  const float4* poses = cur ? b.poses1 : b.poses0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < W.np; i += gridDim.x * blockDim.x)
+    b.out_pts[W.point_off + i] = pts[W.point_off + i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * W.nc; i += gridDim.x * blockDim.x)
+    b.out_poses[3 * W.pose_off + i] = poses[3 * W.pose_off + i];
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Launch wrappers (host side)
+// ---------------------------------------------------------------------------------------------------------
+namespace sba_dev {
+
+hipError_t launch_reset(const Batch& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset, dim3(1024), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_schur(const Batch& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_schur, dim3(b.n_chunks), dim3(kBlock), static_cast<size_t>(b.lds_words) * 4, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s) {
+  hipLaunchKernelGGL(k_factor, dim3(b.n_windows), dim3(kFactorThreads), static_cast<size_t>(b.factor_lds_words) * 4, s,
+                     b, dbg_S);
+  return hipGetLastError();
+}
+
+hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s) {
+  if (initial)
+    hipLaunchKernelGGL(k_backsub<true>, dim3(b.n_chunks), dim3(kBlock), 0, s, b);
+  else
+    hipLaunchKernelGGL(k_backsub<false>, dim3(b.n_chunks), dim3(kBlock), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s) {
+  const int blocks = (b.n_windows + 63) / 64;
+  if (initial)
+    hipLaunchKernelGGL(k_decide<true>, dim3(blocks), dim3(64), 0, s, b);
+  else
+    hipLaunchKernelGGL(k_decide<false>, dim3(blocks), dim3(64), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const Batch& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(16, b.n_windows), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+// Dynamic shared memory up to the 160 KiB of a gfx950 CU needs no attribute on ROCm; this only clears the
+// runtime's last-error slot so that a stale error is not reported by the first launch of a handle.
+hipError_t set_kernel_attributes() {
+  (void)hipGetLastError();
+  return hipSuccess;
+}
+
+}  // namespace sba_dev

@@ -1,0 +1,699 @@
+// Host runtime of the MI355X SBA: the C ABI of include/cuvslam_sba.h.
+//
+// sba_batch_create packs windows into the structure-of-arrays streams of sba_device.h (a counting sort of
+// each window's observations by point, then by key frame and camera -- the reference does the same sort on
+// the host, libs/cuda_modules/sba.cpp:451-535), plans tiles and chunks, uploads once, and records the
+// whole solve (initial cost, max_iterations LM iterations, final gather) as one hipGraph on the handle's
+// stream. sba_batch_run replays it; nothing returns to the host between iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/cuvslam_sba.h"
+#include "sba_device.h"
+
+namespace sba_dev {
+hipError_t launch_reset(const Batch& b, hipStream_t s);
+hipError_t launch_schur(const Batch& b, hipStream_t s);
+hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s);
+hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
+hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
+hipError_t launch_gather(const Batch& b, hipStream_t s);
+hipError_t set_kernel_attributes();
+}  // namespace sba_dev
+
+using namespace sba_dev;
+
+struct sba_handle {
+  sba_options opt{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool attrs_set = false;
+};
+
+struct sba_batch {
+  int n = 0;
+  int max_iter = 0;
+  std::vector<WindowDesc> win;
+  std::vector<int32_t> status_pre;  // host-detected per-window status (empty problems)
+  std::vector<void*> allocs;
+  Batch dev{};
+  int64_t total_obs = 0, total_pairs = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double* dbg_S = nullptr;  // debug buffer of the reduced system of window 0
+  hipEvent_t ev[8] = {};
+  float class_ms[4] = {0, 0, 0, 0};
+  int class_launches[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(sba_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return fail(h, SBA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct HostPack {
+  std::vector<float2> obs_xy;
+  std::vector<float4> obs_info;
+  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree;
+  std::vector<float4> pts, poses, cams;
+  std::vector<Tile> tiles;
+};
+
+// Validates one window and appends it to the packed streams.
+int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, WindowDesc& W) {
+  const int np = p.num_points, nc = p.num_poses, no = p.num_observations, ncam = p.num_cameras;
+  if (np < 0 || nc < 0 || no < 0 || ncam < 0) return fail(h, SBA_E_INVALID, "negative size in window " + std::to_string(w));
+  if (p.num_fixed_points < 0 || p.num_fixed_points > np || p.num_fixed_key_frames < 0 || p.num_fixed_key_frames > nc)
+    return fail(h, SBA_E_INVALID, "fixed counts out of range in window " + std::to_string(w));
+  if ((np && !p.points) || (nc && !p.rig_from_world) || (ncam && !p.camera_from_rig) ||
+      (no && (!p.observation_xys || !p.observation_infos || !p.point_ids || !p.pose_ids || !p.camera_ids)))
+    return fail(h, SBA_E_INVALID, "null buffer in window " + std::to_string(w));
+  const int ncf = nc - p.num_fixed_key_frames;
+  if (ncf > kMaxFreePoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 40 free key frames");
+  if (nc > kMaxPoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 256 key frames");
+  if (ncam > kMaxCameras) return fail(h, SBA_E_CAPACITY, "more than 32 cameras");
+  for (int o = 0; o < no; ++o) {
+    if (p.point_ids[o] < 0 || p.point_ids[o] >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 ||
+        p.camera_ids[o] >= ncam)
+      return fail(h, SBA_E_INVALID, "observation index out of range in window " + std::to_string(w));
+  }
+  W = WindowDesc{};
+  W.np = np;
+  W.npf = np - p.num_fixed_points;
+  W.nc = nc;
+  W.ncf = ncf;
+  W.ncam = ncam;
+  W.no = no;
+  W.point_off = static_cast<int32_t>(H.pts.size());
+  W.pair_off = static_cast<int32_t>(H.pair_pose.size());
+  W.obs_off = static_cast<int32_t>(H.obs_xy.size());
+  W.pose_off = static_cast<int32_t>(H.poses.size() / 3);
+  W.cam_off = static_cast<int32_t>(H.cams.size() / 3);
+  W.max_iterations = p.max_iterations;
+  W.robust = p.robustifier_scale;
+
+  // counting sort by point, then (key frame, camera) inside each point
+  std::vector<int32_t> start(np + 1, 0);
+  for (int o = 0; o < no; ++o) ++start[p.point_ids[o] + 1];
+  for (int i = 0; i < np; ++i) start[i + 1] += start[i];
+  std::vector<int32_t> order(no), fill(start.begin(), start.end() - 1);
+  for (int o = 0; o < no; ++o) order[fill[p.point_ids[o]]++] = o;
+  for (int i = 0; i < np; ++i)
+    std::sort(order.begin() + start[i], order.begin() + start[i + 1], [&](int a, int b) {
+      if (p.pose_ids[a] != p.pose_ids[b]) return p.pose_ids[a] < p.pose_ids[b];
+      if (p.camera_ids[a] != p.camera_ids[b]) return p.camera_ids[a] < p.camera_ids[b];
+      return a < b;
+    });
+  const int obs_base = static_cast<int>(H.obs_xy.size());
+  for (int k = 0; k < no; ++k) {
+    const int o = order[k];
+    H.obs_xy.push_back(make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]));
+    const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
+    float4 inf;
+    inf.x = L[0];
+    inf.y = 0.5f * (L[1] + L[2]);
+    inf.z = L[3];
+    int cam = p.camera_ids[o];
+    std::memcpy(&inf.w, &cam, 4);
+    H.obs_info.push_back(inf);
+  }
+  for (int i = 0; i < np; ++i) {
+    H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
+    int nfree = 0;
+    int last_pose = -1;
+    for (int k = start[i]; k < start[i + 1]; ++k) {
+      const int pose = p.pose_ids[order[k]];
+      if (pose != last_pose) {
+        H.pair_obs.push_back(obs_base + k);
+        H.pair_pose.push_back(pose);
+        if (pose < ncf) ++nfree;
+        last_pose = pose;
+      }
+    }
+    H.point_nfree.push_back(nfree);
+    H.pts.push_back(make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f));
+  }
+  auto push_pose = [](std::vector<float4>& dst, const float* m) {
+    for (int r = 0; r < 3; ++r) dst.push_back(make_float4(m[r], m[4 + r], m[8 + r], m[12 + r]));
+  };
+  for (int j = 0; j < nc; ++j) push_pose(H.poses, p.rig_from_world + 16 * j);
+  for (int c = 0; c < ncam; ++c) push_pose(H.cams, p.camera_from_rig + 16 * c);
+  // tiles: consecutive points with at most kTilePairs pairs and kTilePoints points
+  int i = 0;
+  while (i < np) {
+    const int begin = i;
+    int pairs = 0;
+    while (i < np && i - begin < kTilePoints) {
+      const int pi = static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
+                                      H.point_pair[W.point_off + i]);
+      if (pairs + pi > kTilePairs && i > begin) break;
+      pairs += pi;
+      ++i;
+    }
+    H.tiles.push_back(Tile{w, W.point_off + begin, W.point_off + i, pairs});
+  }
+  return SBA_OK;
+}
+
+template <class T>
+int upload(sba_handle* h, sba_batch* b, const std::vector<T>& v, T** out, size_t min_count = 1) {
+  const size_t count = std::max(v.size(), min_count);
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+  b->allocs.push_back(p);
+  if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<T*>(p);
+  return SBA_OK;
+}
+
+template <class T>
+int alloc(sba_handle* h, sba_batch* b, size_t count, T** out) {
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  b->allocs.push_back(p);
+  HIP_TRY(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(T)));
+  *out = static_cast<T*>(p);
+  return SBA_OK;
+}
+
+int free_batch(sba_batch* b) {
+  if (!b) return SBA_OK;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (void* p : b->allocs) (void)hipFree(p);
+  delete b;
+  return SBA_OK;
+}
+
+// The launch sequence of one solve. With events, each kernel class is timed (eager mode only).
+int enqueue_solve(sba_handle* h, sba_batch* b, bool timed) {
+  const Batch& d = b->dev;
+  hipStream_t s = h->stream;
+  HIP_TRY(launch_reset(d, s));
+  HIP_TRY(launch_backsub(d, true, s));
+  HIP_TRY(launch_decide(d, true, s));
+  for (int it = 0; it < b->max_iter; ++it) {
+    if (timed) HIP_TRY(hipEventRecord(b->ev[0], s));
+    HIP_TRY(launch_schur(d, s));
+    if (timed) HIP_TRY(hipEventRecord(b->ev[1], s));
+    HIP_TRY(launch_factor(d, nullptr, s));
+    if (timed) HIP_TRY(hipEventRecord(b->ev[2], s));
+    HIP_TRY(launch_backsub(d, false, s));
+    if (timed) HIP_TRY(hipEventRecord(b->ev[3], s));
+    HIP_TRY(launch_decide(d, false, s));
+    if (timed) {
+      HIP_TRY(hipEventRecord(b->ev[4], s));
+      HIP_TRY(hipEventSynchronize(b->ev[4]));
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
+        b->class_ms[k] += ms;
+        b->class_launches[k] += 1;
+      }
+    }
+  }
+  HIP_TRY(launch_gather(d, s));
+  return SBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sba_abi_version(void) { return SBA_ABI_VERSION; }
+
+void sba_default_options(sba_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->device = 0;
+  o->loss = SBA_LOSS_STUDENT_T;
+  o->fixed_iterations = 0;
+  o->use_graph = 1;
+  o->chunk_pairs = 0;
+}
+
+int sba_create(const sba_options* opt, sba_handle** out) {
+  if (!out) return SBA_E_INVALID;
+  *out = nullptr;
+  sba_handle* h = new (std::nothrow) sba_handle;
+  if (!h) return SBA_E_INVALID;
+  if (opt)
+    h->opt = *opt;
+  else
+    sba_default_options(&h->opt);
+  if (h->opt.loss != SBA_LOSS_STUDENT_T && h->opt.loss != SBA_LOSS_HUBER) {
+    delete h;
+    return SBA_E_INVALID;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= h->opt.device || h->opt.device < 0) {
+    delete h;
+    return SBA_E_DEVICE;
+  }
+  if (hipSetDevice(h->opt.device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return SBA_E_DEVICE;
+  }
+  (void)set_kernel_attributes();  // raises the dynamic shared-memory limit where the runtime needs it
+  *out = h;
+  return SBA_OK;
+}
+
+int sba_destroy(sba_handle* h) {
+  if (!h) return SBA_OK;
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SBA_OK;
+}
+
+const char* sba_last_error(const sba_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_batch** out) {
+  if (!h || !problems || n <= 0 || !out) return fail(h, SBA_E_INVALID, "bad arguments to sba_batch_create");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(h->opt.device));
+  HostPack H;
+  sba_batch* b = new (std::nothrow) sba_batch;
+  if (!b) return fail(h, SBA_E_INVALID, "out of host memory");
+  b->n = n;
+  b->win.resize(n);
+  b->status_pre.assign(n, SBA_OK);
+  int64_t s_off = 0, a_off = 0, dc_off = 0;
+  int max_nc = 1, max_ncam = 1, max_nb = 1, max_ncf = 1, max_lds_packed = 0;
+  for (int w = 0; w < n; ++w) {
+    const sba_problem_view& p = problems[w];
+    const int rc = pack_window(h, p, w, H, b->win[w]);
+    if (rc != SBA_OK) {
+      free_batch(b);
+      return rc;
+    }
+    WindowDesc& W = b->win[w];
+    if (W.np == 0 || W.nc == 0 || W.no == 0) b->status_pre[w] = SBA_E_EMPTY;
+    const int nb = W.ncf * (W.ncf + 1) / 2;
+    W.s_off = static_cast<int32_t>(s_off);
+    s_off += 36 * nb + 12 * W.ncf;
+    W.dc_off = static_cast<int32_t>(dc_off);
+    dc_off += 6 * W.ncf;
+    const int nn = 6 * W.ncf;
+    const int64_t packed = static_cast<int64_t>(nn) * (nn + 1) / 2;
+    const int64_t fixed_doubles = 512 + 256 * kPanel + 48;
+    W.factor_in_lds = (fixed_doubles + packed) * 8 <= 160 * 1024 ? 1 : 0;
+    if (W.factor_in_lds) {
+      max_lds_packed = std::max<int>(max_lds_packed, static_cast<int>(packed));
+      W.a_off = 0;
+    } else {
+      W.a_off = a_off;
+      a_off += packed;
+    }
+    max_nc = std::max(max_nc, W.nc);
+    max_ncam = std::max(max_ncam, W.ncam);
+    max_nb = std::max(max_nb, nb);
+    max_ncf = std::max(max_ncf, W.ncf);
+    b->max_iter = std::max(b->max_iter, p.max_iterations);
+    // an empty window never runs: the initial decision sees zero observations as infeasible, so mark it here
+    if (b->status_pre[w] != SBA_OK) W.max_iterations = 0;
+  }
+  b->total_obs = static_cast<int64_t>(H.obs_xy.size());
+  b->total_pairs = static_cast<int64_t>(H.pair_pose.size());
+  H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
+  H.pair_obs.push_back(static_cast<int32_t>(H.obs_xy.size()));
+
+  // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each
+  int64_t target = h->opt.chunk_pairs;
+  if (target <= 0) target = std::min<int64_t>(16384, std::max<int64_t>(1024, b->total_pairs / 1024));
+  std::vector<Chunk> chunks;
+  {
+    size_t t = 0;
+    while (t < H.tiles.size()) {
+      const int w = H.tiles[t].window;
+      const size_t begin = t;
+      int64_t pairs = 0;
+      while (t < H.tiles.size() && H.tiles[t].window == w && (t == begin || pairs < target)) {
+        pairs += H.tiles[t].pad;
+        ++t;
+      }
+      chunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t)});
+    }
+  }
+  for (auto& T : H.tiles) T.pad = 0;
+
+  Batch& d = b->dev;
+  d.n_windows = n;
+  d.n_chunks = static_cast<int32_t>(chunks.size());
+  d.n_points = static_cast<int32_t>(H.pts.size());
+  d.n_poses = static_cast<int32_t>(H.poses.size() / 3);
+  d.n_cams = static_cast<int32_t>(H.cams.size() / 3);
+  d.loss = h->opt.loss;
+  d.fixed_iterations = h->opt.fixed_iterations;
+  d.s_acc_size = s_off;
+  int rc = SBA_OK;
+#define UP(vec, field)                                                                   \
+  if ((rc = upload(h, b, vec, const_cast<std::remove_const_t<std::remove_pointer_t<decltype(d.field)>>**>(&d.field))) != SBA_OK) { \
+    free_batch(b);                                                                       \
+    return rc;                                                                           \
+  }
+  UP(b->win, win);
+  UP(H.tiles, tiles);
+  UP(chunks, chunks);
+  UP(H.obs_xy, obs_xy);
+  UP(H.obs_info, obs_info);
+  UP(H.pair_obs, pair_obs);
+  UP(H.pair_pose, pair_pose);
+  UP(H.point_pair, point_pair);
+  UP(H.point_nfree, point_nfree);
+  UP(H.pts, pts_init);
+  UP(H.poses, poses_init);
+  UP(H.cams, cams);
+#undef UP
+#define AL(count, field)                                \
+  if ((rc = alloc(h, b, count, &d.field)) != SBA_OK) { \
+    free_batch(b);                                      \
+    return rc;                                          \
+  }
+  AL(static_cast<size_t>(n), st);
+  AL(H.pts.size(), pts0);
+  AL(H.pts.size(), pts1);
+  AL(3 * H.pts.size(), paux);
+  AL(H.poses.size(), poses0);
+  AL(H.poses.size(), poses1);
+  AL(static_cast<size_t>(s_off), s_acc);
+  AL(static_cast<size_t>(a_off), A);
+  AL(static_cast<size_t>(dc_off), dc);
+  AL(static_cast<size_t>(dc_off), dc64);
+  AL(H.pts.size(), out_pts);
+  AL(H.poses.size(), out_poses);
+#undef AL
+  d.dbg_dp = nullptr;
+  // shared-memory layout of the Schur kernel, sized for the batch's largest window
+  int off = 0;
+  d.lds_pose = off;
+  off += 12 * max_nc;
+  d.lds_cam = off;
+  off += 12 * max_ncam;
+  d.lds_S = off;
+  off += 36 * max_nb;
+  d.lds_e = off;
+  off += 6 * max_ncf;
+  d.lds_du = off;
+  off += 6 * max_ncf;
+  d.lds_acc = off;
+  off += 12 * kTilePoints;
+  d.lds_Z = off;
+  off += 18 * kTilePairs;
+  d.lds_pp = off;
+  off += kTilePairs;
+  d.lds_pre = off;
+  off += kTilePoints + 1;
+  d.lds_words = off;
+  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + max_lds_packed);
+  if (static_cast<int64_t>(d.lds_words) * 4 + 64 > 160 * 1024)
+    return free_batch(b), fail(h, SBA_E_CAPACITY, "Schur partial does not fit in shared memory");
+  for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipMalloc(&b->dbg_S, sizeof(double) * (240 * 240 + 240)));
+  b->allocs.push_back(b->dbg_S);
+  if (h->opt.use_graph) {
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int erc = enqueue_solve(h, b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (erc != SBA_OK || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      free_batch(b);
+      return fail(h, SBA_E_DEVICE, "graph capture failed");
+    }
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  }
+  *out = b;
+  return SBA_OK;
+}
+
+int sba_batch_run(sba_handle* h, sba_batch* b) {
+  if (!h || !b) return SBA_E_INVALID;
+  HIP_TRY(hipSetDevice(h->opt.device));
+  if (b->gexec) {
+    HIP_TRY(hipGraphLaunch(b->gexec, h->stream));
+    return SBA_OK;
+  }
+  return enqueue_solve(h, b, false);
+}
+
+int sba_batch_sync(sba_handle* h) {
+  if (!h) return SBA_E_INVALID;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return SBA_OK;
+}
+
+int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results) {
+  if (!h || !b || !results) return SBA_E_INVALID;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<WindowState> st(b->n);
+  HIP_TRY(hipMemcpy(st.data(), b->dev.st, sizeof(WindowState) * b->n, hipMemcpyDeviceToHost));
+  std::vector<float4> pts(b->dev.n_points), poses(3 * static_cast<size_t>(b->dev.n_poses));
+  HIP_TRY(hipMemcpy(pts.data(), b->dev.out_pts, sizeof(float4) * pts.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(poses.data(), b->dev.out_poses, sizeof(float4) * poses.size(), hipMemcpyDeviceToHost));
+  for (int w = 0; w < b->n; ++w) {
+    const WindowDesc& W = b->win[w];
+    sba_result& r = results[w];
+    const WindowState& s = st[w];
+    r.iterations = s.iterations;
+    r.initial_cost = static_cast<float>(s.initial_cost);
+    r.last_cost = static_cast<float>(s.last_cost);
+    r.initial_cost_f64 = s.initial_cost;
+    r.last_cost_f64 = s.last_cost;
+    r.lambda = s.lambda;
+    r.accepted = s.accepted;
+    r.status = b->status_pre[w] != SBA_OK ? b->status_pre[w] : s.status;
+    if (r.status == SBA_E_EMPTY) {
+      r.iterations = 0;
+      r.initial_cost = r.last_cost = 0.f;
+      r.initial_cost_f64 = r.last_cost_f64 = 0.0;
+    }
+    // state is written back only for a solve that ran (the reference leaves the problem untouched on failure)
+    if (r.status != SBA_OK) continue;
+    if (r.points)
+      for (int i = 0; i < W.np; ++i) {
+        const float4 q = pts[W.point_off + i];
+        r.points[3 * i] = q.x;
+        r.points[3 * i + 1] = q.y;
+        r.points[3 * i + 2] = q.z;
+      }
+    if (r.rig_from_world)
+      for (int j = 0; j < W.nc; ++j) {
+        float* m = r.rig_from_world + 16 * j;
+        const float4* rows = &poses[3 * (W.pose_off + j)];
+        const float R[3][4] = {{rows[0].x, rows[0].y, rows[0].z, rows[0].w},
+                               {rows[1].x, rows[1].y, rows[1].z, rows[1].w},
+                               {rows[2].x, rows[2].y, rows[2].z, rows[2].w}};
+        for (int c = 0; c < 4; ++c)
+          for (int rr = 0; rr < 3; ++rr) m[4 * c + rr] = R[rr][c];
+        m[3] = m[7] = m[11] = 0.f;
+        m[15] = 1.f;
+      }
+  }
+  return SBA_OK;
+}
+
+int sba_batch_destroy(sba_handle* h, sba_batch* b) {
+  if (h) (void)hipSetDevice(h->opt.device);
+  return free_batch(b);
+}
+
+int sba_batch_info(sba_handle* h, sba_batch* b, int64_t* num_observations, int64_t* num_pairs, int32_t* num_chunks,
+                   int32_t* num_windows) {
+  if (!h || !b) return SBA_E_INVALID;
+  if (num_observations) *num_observations = b->total_obs;
+  if (num_pairs) *num_pairs = b->total_pairs;
+  if (num_chunks) *num_chunks = b->dev.n_chunks;
+  if (num_windows) *num_windows = b->n;
+  return SBA_OK;
+}
+
+int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int32_t* launches_per_class) {
+  if (!h || !b) return SBA_E_INVALID;
+  for (int k = 0; k < 4; ++k) {
+    b->class_ms[k] = 0.f;
+    b->class_launches[k] = 0;
+  }
+  const int rc = enqueue_solve(h, b, true);
+  if (rc != SBA_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 4; ++k) {
+    if (ms_per_class) ms_per_class[k] = b->class_ms[k];
+    if (launches_per_class) launches_per_class[k] = b->class_launches[k];
+  }
+  return SBA_OK;
+}
+
+int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc, double* dp,
+                         double* scalars) {
+  if (!h || !b) return SBA_E_INVALID;
+  HIP_TRY(hipSetDevice(h->opt.device));
+  Batch d = b->dev;
+  const WindowDesc& W = b->win[0];
+  hipStream_t s = h->stream;
+  float4* dpbuf = nullptr;
+  HIP_TRY(hipMalloc(&dpbuf, sizeof(float4) * std::max(1, W.np)));
+  d.dbg_dp = dpbuf;
+  int rc = SBA_OK;
+  do {
+    if (launch_reset(d, s) != hipSuccess || launch_backsub(d, true, s) != hipSuccess || launch_decide(d, true, s) != hipSuccess) {
+      rc = fail(h, SBA_E_DEVICE, "launch failed");
+      break;
+    }
+    WindowState st0;
+    if (hipMemcpyAsync(&st0, d.st, sizeof(st0), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(h, SBA_E_DEVICE, "state read failed");
+      break;
+    }
+    const double current = st0.current_cost;
+    st0.lambda = static_cast<float>(lambda);
+    st0.done = 0;
+    if (hipMemcpyAsync(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice, s) != hipSuccess ||
+        launch_schur(d, s) != hipSuccess || launch_factor(d, b->dbg_S, s) != hipSuccess ||
+        launch_backsub(d, false, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(h, SBA_E_DEVICE, "step failed");
+      break;
+    }
+    WindowState st1;
+    const int n = 6 * W.ncf;
+    std::vector<double> Sbuf(static_cast<size_t>(n) * n + n), dcv(n);
+    std::vector<float4> dpv(std::max(1, W.np));
+    if (hipMemcpy(&st1, d.st, sizeof(st1), hipMemcpyDeviceToHost) != hipSuccess ||
+        (n && hipMemcpy(Sbuf.data(), b->dbg_S, sizeof(double) * Sbuf.size(), hipMemcpyDeviceToHost) != hipSuccess) ||
+        (n && hipMemcpy(dcv.data(), d.dc64 + W.dc_off, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) ||
+        hipMemcpy(dpv.data(), dpbuf, sizeof(float4) * dpv.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(h, SBA_E_DEVICE, "readback failed");
+      break;
+    }
+    if (S) std::copy(Sbuf.begin(), Sbuf.begin() + static_cast<size_t>(n) * n, S);
+    if (e) std::copy(Sbuf.begin() + static_cast<size_t>(n) * n, Sbuf.end(), e);
+    if (dc) std::copy(dcv.begin(), dcv.end(), dc);
+    if (dp)
+      for (int i = 0; i < W.npf; ++i) {
+        dp[3 * i] = dpv[i].x;
+        dp[3 * i + 1] = dpv[i].y;
+        dp[3 * i + 2] = dpv[i].z;
+      }
+    if (scalars) {
+      scalars[0] = st1.acc_skipped == W.no ? INFINITY : st1.acc_cost;
+      scalars[1] = st1.acc_skipped;
+      scalars[2] = st1.acc_hess;
+      scalars[3] = st1.scal_pose + st1.acc_scal_pt;
+      scalars[4] = __builtin_bit_cast(float, st1.acc_maxdp);
+      scalars[5] = st1.maxdc;
+      scalars[6] = current;
+      scalars[7] = st1.chol_ok;
+    }
+    if (!st1.chol_ok) rc = SBA_E_INFEASIBLE;
+  } while (false);
+  (void)hipFree(dpbuf);
+  return rc;
+}
+
+int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bvec, double* x) {
+  if (!h || !A || !bvec || !x || n <= 0 || n % 6 != 0 || n > 6 * kMaxFreePoses) return fail(h, SBA_E_INVALID, "bad debug_factor size");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  const int ncf = n / 6, nb = ncf * (ncf + 1) / 2;
+  // pack A into the accumulator layout: upper 6x6 blocks (row-major), then e, then diag(U) = 0
+  std::vector<float> acc(36 * nb + 12 * ncf, 0.f);
+  for (int bj = 0; bj < ncf; ++bj)
+    for (int bk = bj; bk < ncf; ++bk) {
+      const int blk = bj * ncf - (bj * (bj - 1)) / 2 + (bk - bj);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+          const int i = 6 * bj + r, j = 6 * bk + c;
+          if (bj == bk && c < r) continue;
+          acc[36 * blk + 6 * r + c] = static_cast<float>(A[static_cast<int64_t>(j) * n + i]);  // lower triangle of A
+        }
+    }
+  for (int i = 0; i < n; ++i) acc[36 * nb + i] = static_cast<float>(bvec[i]);
+  sba_batch* b = new (std::nothrow) sba_batch;
+  if (!b) return fail(h, SBA_E_INVALID, "out of host memory");
+  b->n = 1;
+  b->win.resize(1);
+  WindowDesc& W = b->win[0];
+  W = WindowDesc{};
+  W.nc = W.ncf = ncf;
+  W.ncam = 1;
+  W.no = 1;
+  W.max_iterations = 1;
+  const int64_t packed = static_cast<int64_t>(n) * (n + 1) / 2;
+  W.factor_in_lds = (512 + 256 * kPanel + 48 + packed) * 8 <= 160 * 1024 ? 1 : 0;
+  std::vector<float4> poses(3 * ncf);
+  for (int j = 0; j < ncf; ++j) {
+    poses[3 * j] = make_float4(1, 0, 0, 0);
+    poses[3 * j + 1] = make_float4(0, 1, 0, 0);
+    poses[3 * j + 2] = make_float4(0, 0, 1, 0);
+  }
+  Batch& d = b->dev;
+  d.n_windows = 1;
+  int rc = SBA_OK;
+  WindowState st0{};
+  st0.lambda = 0.f;
+  std::vector<WindowState> stv(1, st0);
+  if ((rc = upload(h, b, b->win, const_cast<WindowDesc**>(&d.win))) != SBA_OK ||
+      (rc = upload(h, b, stv, &d.st)) != SBA_OK || (rc = upload(h, b, acc, &d.s_acc)) != SBA_OK ||
+      (rc = upload(h, b, poses, &d.poses0)) != SBA_OK || (rc = upload(h, b, poses, &d.poses1)) != SBA_OK ||
+      (rc = alloc(h, b, static_cast<size_t>(n), &d.dc)) != SBA_OK || (rc = alloc(h, b, static_cast<size_t>(n), &d.dc64)) != SBA_OK ||
+      (rc = alloc(h, b, static_cast<size_t>(W.factor_in_lds ? 1 : packed), &d.A)) != SBA_OK) {
+    free_batch(b);
+    return rc;
+  }
+  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + (W.factor_in_lds ? static_cast<int>(packed) : 0));
+  if (launch_factor(d, nullptr, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+    free_batch(b);
+    return fail(h, SBA_E_DEVICE, "factor launch failed");
+  }
+  WindowState st1{};
+  if (hipMemcpy(&st1, d.st, sizeof(st1), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(x, d.dc64, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+    free_batch(b);
+    return fail(h, SBA_E_DEVICE, "factor readback failed");
+  }
+  free_batch(b);
+  return st1.chol_ok ? SBA_OK : SBA_E_INFEASIBLE;
+}
+
+int sba_solve_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_result* results) {
+  sba_batch* b = nullptr;
+  int rc = sba_batch_create(h, problems, n, &b);
+  if (rc != SBA_OK) return rc;
+  rc = sba_batch_run(h, b);
+  if (rc == SBA_OK) rc = sba_batch_read(h, b, results);
+  sba_batch_destroy(h, b);
+  return rc;
+}
+
+int sba_solve(sba_handle* h, const sba_problem_view* problem, sba_result* result) {
+  if (!h || !problem || !result) return fail(h, SBA_E_INVALID, "bad arguments to sba_solve");
+  if (problem->num_points == 0 || problem->num_poses == 0 || problem->num_observations == 0) {
+    result->status = SBA_E_EMPTY;
+    result->iterations = 0;
+    return SBA_E_EMPTY;
+  }
+  const int rc = sba_solve_batch(h, problem, 1, result);
+  if (rc != SBA_OK) return rc;
+  return result->status;
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+/*
+ * cuvslam_sba.h -- C ABI of the MI355X sliding-window sparse bundle adjustment (SBA).
+ *
+ * This is the drop-in boundary for cuVSLAM's visual SBA solve:
+ *   cuvslam::sba::SchurComplementBundlerGpu::solve(BundleAdjustmentProblem&)
+ *     (reference: libs/sba/schur_complement_bundler_gpu.h:29-38,
+ *                 libs/sba/schur_complement_bundler_gpu.cpp:115-259)
+ * with the in/out problem of libs/sba/bundle_adjustment_problem.h:34-68.
+ *
+ * Every buffer is a plain pointer with the memory layout of the reference's std::vector<...>::data():
+ *   points            Vector3f   -> float[3]   per point
+ *   observation_xys   Vector2f   -> float[2]   per observation (normalized image coordinates)
+ *   observation_infos Matrix2f   -> float[4]   per observation, column-major 2x2 information matrix
+ *   point_ids/pose_ids int32, camera_ids int8
+ *   rig_from_world    Isometry3f -> float[16]  per key frame, column-major 4x4
+ *   camera_from_rig   Isometry3f -> float[16]  per camera (camera::Rig::camera_from_rig, libs/camera/rig.h:29-43)
+ * The last num_fixed_points points and the last num_fixed_key_frames key frames do not move
+ * (bundle_adjustment_problem.h:35,52-56).
+ *
+ * No function throws. Every function returns an sba_status; sba_last_error() describes the last failure.
+ * One handle is used by one thread at a time (the reference bundler has the same contract).
+ */
+#ifndef CUVSLAM_SBA_H_
+#define CUVSLAM_SBA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBA_ABI_VERSION 1
+
+typedef enum sba_status {
+  SBA_OK = 0,              /* solve ran; the reference bundler returns true */
+  SBA_E_EMPTY = 1,         /* no points, poses or observations (schur_complement_bundler_gpu.cpp:121-123) */
+  SBA_E_INFEASIBLE = 2,    /* every observation is inside the near plane at the start (gpu.cpp:165-168) */
+  SBA_E_DEVICE = 3,        /* a HIP runtime error */
+  SBA_E_CAPACITY = 4,      /* a window exceeds what the kernels support */
+  SBA_E_INVALID = 5        /* a bad argument (null pointer, index out of range, ...) */
+} sba_status;
+
+typedef enum sba_loss {
+  SBA_LOSS_STUDENT_T = 0,  /* the reference's hard-wired loss: rho(s) = (d+1) log(1+s/d)
+                              (schur_complement_bundler_cpu.cpp:39-40, math/robust_cost_function.cpp:89-95) */
+  SBA_LOSS_HUBER = 1       /* rho(s) = s/2 for s < d^2, else d sqrt(s) - d^2/2 (robust_cost_function.cpp:62-87) */
+} sba_loss;
+
+/* The problem of one window. Mirrors cuvslam::sba::BundleAdjustmentProblem field by field. */
+typedef struct sba_problem_view {
+  int32_t num_points;
+  int32_t num_fixed_points;
+  int32_t num_observations;
+  int32_t num_poses;
+  int32_t num_fixed_key_frames;
+  int32_t num_cameras;
+  const float* points;            /* [num_points][3] */
+  const float* observation_xys;   /* [num_observations][2] */
+  const float* observation_infos; /* [num_observations][4], column-major */
+  const int32_t* point_ids;       /* [num_observations] */
+  const int32_t* pose_ids;        /* [num_observations] */
+  const int8_t* camera_ids;       /* [num_observations] */
+  const float* rig_from_world;    /* [num_poses][16], column-major */
+  const float* camera_from_rig;   /* [num_cameras][16], column-major */
+  int32_t max_iterations;         /* BundleAdjustmentProblem::max_iterations (default 20) */
+  float robustifier_scale;        /* BundleAdjustmentProblem::robustifier_scale (default 1) */
+} sba_problem_view;
+
+/* The solver statistics and final state of one window. */
+typedef struct sba_result {
+  float* points;                  /* out [num_points][3]; may alias the problem's points */
+  float* rig_from_world;          /* out [num_poses][16]; may alias the problem's poses */
+  int32_t iterations;             /* BundleAdjustmentProblem::iterations */
+  float initial_cost;             /* BundleAdjustmentProblem::initial_cost */
+  float last_cost;                /* BundleAdjustmentProblem::last_cost */
+  int32_t status;                 /* sba_status of this window */
+  double initial_cost_f64;        /* the same costs as evaluated (in FP64) by the solver */
+  double last_cost_f64;
+  float lambda;                   /* final LM damping */
+  int32_t accepted;               /* number of accepted LM steps */
+} sba_result;
+
+typedef struct sba_options {
+  int32_t device;                 /* HIP device ordinal */
+  int32_t loss;                   /* sba_loss; SBA_LOSS_STUDENT_T matches the reference */
+  int32_t fixed_iterations;       /* 1: run exactly max_iterations LM iterations (the step-size
+                                     convergence exit is off; the zero-cost and infeasible guards stay) */
+  int32_t use_graph;              /* 1: capture the LM iterations of a batch in one hipGraph */
+  int32_t chunk_pairs;            /* work per Schur workgroup in (point, key frame) pairs; 0 = autotuned */
+  int32_t reserved[7];
+} sba_options;
+
+typedef struct sba_handle sba_handle;
+typedef struct sba_batch sba_batch;
+
+/* Fills *opt with the defaults (device 0, Student-t loss, reference early exit, graphs on, autotuned tiles). */
+void sba_default_options(sba_options* opt);
+
+int sba_create(const sba_options* opt, sba_handle** out);
+int sba_destroy(sba_handle* h);
+const char* sba_last_error(const sba_handle* h);
+int sba_abi_version(void);
+
+/* Solves one window in place, like SchurComplementBundlerGpu::solve (schur_complement_bundler_gpu.cpp:93-100).
+ * The return value is the window's status; SBA_OK corresponds to the reference's `true`. */
+int sba_solve(sba_handle* h, const sba_problem_view* problem, sba_result* result);
+
+/* Solves n independent windows of uneven size in one batch: every kernel of an LM iteration covers all
+ * windows, and the iterations run as one hipGraph. No reference counterpart (the reference solves one
+ * window per call). Returns SBA_OK when the batch ran; per-window statuses are in results[i].status. */
+int sba_solve_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_result* results);
+
+/* Device-resident batches, for callers that solve the same packed windows repeatedly (serving, timing):
+ * upload and pack once, then run the LM iterations from the uploaded initial state as often as needed. */
+int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_batch** out);
+int sba_batch_run(sba_handle* h, sba_batch* b);       /* asynchronous on the handle's stream */
+int sba_batch_sync(sba_handle* h);                     /* waits for the handle's stream */
+int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results);   /* synchronous */
+int sba_batch_destroy(sba_handle* h, sba_batch* b);
+/* Number of launches and observations of a batch, and the kernel-event timings of its last run
+ * (milliseconds per kernel class: [linearize+Schur, reduce+factor, back-substitute+cost, decide]). */
+int sba_batch_info(sba_handle* h, sba_batch* b, int64_t* num_observations, int64_t* num_pairs,
+                   int32_t* num_chunks, int32_t* num_windows);
+int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int32_t* launches_per_class);
+
+/* One LM step of window 0 of a batch from its current state with damping lambda, for phase-level parity
+ * tests (the reference tests its phases the same way: libs/cuda_modules/test/sba_test.cpp).
+ * Outputs (any may be null):
+ *   S       [n*n] FP64 damped reduced camera system, n = 6 * free key frames (before factorization)
+ *   e       [n]   reduced right-hand side
+ *   dc      [n]   pose step;  dp [3 * free points] point step
+ *   scalars [6]   trial cost, number skipped, hessian term, scaling term, max|dp|, max|dc|
+ * Returns SBA_OK, or SBA_E_INFEASIBLE when the factorization failed (S not positive definite). */
+int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc,
+                         double* dp, double* scalars);
+
+/* Factor/solve of one symmetric n x n system (n a multiple of 6, n <= 240) with the reduced-camera-system
+ * kernel: A row-major (only its lower triangle is read), b [n] -> x [n]. Returns SBA_OK, or
+ * SBA_E_INFEASIBLE when A is not positive definite (the reference's devInfo != 0,
+ * libs/cuda_modules/sba.cpp:619-642). For the factorization known-answer tests. */
+int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* b, double* x);
+
+/* ---- Synthetic scenes (the benchmark generator; host only, no device needed) -------------------------- */
+typedef struct sba_scene_config {
+  int32_t num_keyframes;          /* key frames */
+  int32_t num_points;             /* 3-D points */
+  int32_t sample_observations;    /* 0: a point is observed wherever it projects in-image;
+                                     1: each point is observed by U{2..num_keyframes} key frames it projects in */
+  int32_t num_cameras;            /* 1 (mono pinhole) */
+  int32_t num_fixed_keyframes;    /* the first key frames are fixed (stored last); 1 leaves the scale free */
+  double fx, fy, cx, cy;
+  int32_t width, height;
+  double keyframe_spacing;        /* metres along +x */
+  double keyframe_rotation_deg;   /* sigma of each key frame's random rotation */
+  double pixel_noise;             /* sigma, pixels */
+  double outlier_rate;            /* fraction of observations replaced by a uniform pixel */
+  double pose_sigma_m, pose_sigma_deg, point_sigma_m;   /* initial-state perturbation */
+  double huber_px;                /* information = diag(fx^2, fy^2) / huber_px^2 */
+  double xy_extent, z_min, z_max; /* points: x, y in [-xy_extent, xy_extent], z in [z_min, z_max] */
+  uint64_t seed;
+} sba_scene_config;
+
+typedef struct sba_scene sba_scene;
+
+void sba_scene_default_config(sba_scene_config* cfg);
+int sba_scene_generate(const sba_scene_config* cfg, sba_scene** out);
+/* Fills a problem view that points into the scene's own storage (valid until sba_scene_free). */
+int sba_scene_view(const sba_scene* s, sba_problem_view* view);
+/* Ground-truth points [num_points][3] and rig_from_world [num_poses][16] in the stored order. */
+int sba_scene_ground_truth(const sba_scene* s, const float** points, const float** rig_from_world);
+void sba_scene_free(sba_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUVSLAM_SBA_H_ */

@@ -1,0 +1,809 @@
+// TEST INFRASTRUCTURE -- the CPU oracle of the sliding-window SBA. Only tests/, the smoke check and the
+// benchmark's CPU-baseline leg load this library, and only as the checker or the timed CPU baseline; the
+// product (cuvslam_amd/) never calls it.
+//
+// A restatement, in plain C++ with no Eigen, of cuVSLAM's Schur-complement Levenberg-Marquardt bundler
+// (libs/sba/schur_complement_bundler_cpu.cpp, with the GPU bundler's control flow of
+// libs/sba/schur_complement_bundler_gpu.cpp:115-259). It is templated on the scalar:
+//   * double: the FP64 oracle of the plain method (Cholesky RCS solve, block-sparse Schur complement);
+//   * float : the reference CPU path restated step for step (dense Y*W product as cpu.cpp:264-277, Jacobi
+//             pseudo-inverse of the RCS with threshold 1e-6 as cpu.cpp:337-346, FP32 everywhere).
+// Each function cites the reference lines it follows. What pins each function (closed forms, finite
+// differences, numpy dense solves, the reference's own known answers) is in tests/test_oracle.py.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+constexpr float kFltEps = std::numeric_limits<float>::epsilon();  // common/types.h:203-206
+const float kSqrtFltEps = std::sqrt(kFltEps);                       // common/types.h:208-211
+constexpr float kHither = 0.1f;                                     // epipolar/near_plane.h:33-38
+
+struct OrcProblem {  // same field order as sba_problem_view (include/cuvslam_sba.h)
+  int32_t num_points, num_fixed_points, num_observations, num_poses, num_fixed_key_frames, num_cameras;
+  const float* points;
+  const float* observation_xys;
+  const float* observation_infos;
+  const int32_t* point_ids;
+  const int32_t* pose_ids;
+  const int8_t* camera_ids;
+  const float* rig_from_world;
+  const float* camera_from_rig;
+  int32_t max_iterations;
+  float robustifier_scale;
+};
+
+struct OrcOptions {
+  int32_t loss;              // 0 Student-t, 1 Huber
+  int32_t rcs_solver;        // 0 Cholesky (GPU bundler), 1 Jacobi pseudo-inverse (CPU bundler)
+  int32_t dense_schur;       // 1: dense temp * point_pose_block product as the CPU bundler does
+  int32_t fixed_iterations;  // 1: no step-size convergence exit
+  int32_t reorthonormalize;  // 1: UpdateState re-orthonormalizes rotations (cpu.cpp:377-380)
+  int32_t pad[3];
+};
+
+struct OrcResult {
+  double* points;  // out [np][3]
+  double* poses;   // out [nc][16] column-major
+  int32_t iterations, status, accepted, pad;
+  double initial_cost, last_cost, lambda;
+  int32_t trace_capacity, pad2;
+  double* trace_lambda;
+  double* trace_cost;
+  double* trace_rho;
+  int32_t* trace_flags;  // bit 0 accepted, bit 1 factorization failed, bit 2 converged exit
+};
+
+template <class T>
+struct Pose {
+  T R[3][3];
+  T t[3];
+};
+
+template <class T>
+Pose<T> compose(const Pose<T>& a, const Pose<T>& b) {  // a * b
+  Pose<T> c;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) c.R[i][j] = a.R[i][0] * b.R[0][j] + a.R[i][1] * b.R[1][j] + a.R[i][2] * b.R[2][j];
+    c.t[i] = a.R[i][0] * b.t[0] + a.R[i][1] * b.t[1] + a.R[i][2] * b.t[2] + a.t[i];
+  }
+  return c;
+}
+
+template <class T>
+void apply(const Pose<T>& a, const T p[3], T q[3]) {
+  for (int i = 0; i < 3; ++i) q[i] = a.R[i][0] * p[0] + a.R[i][1] * p[1] + a.R[i][2] * p[2] + a.t[i];
+}
+
+template <class T>
+Pose<T> load_pose(const float* m) {  // column-major 4x4 (Eigen Isometry3f)
+  Pose<T> p;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) p.R[r][c] = static_cast<T>(m[c * 4 + r]);
+    p.t[r] = static_cast<T>(m[12 + r]);
+  }
+  return p;
+}
+
+template <class T>
+void store_pose(const Pose<T>& p, double* m) {
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) m[c * 4 + r] = static_cast<double>(p.R[r][c]);
+  m[3] = m[7] = m[11] = 0;
+  for (int r = 0; r < 3; ++r) m[12 + r] = static_cast<double>(p.t[r]);
+  m[15] = 1;
+}
+
+// Robust losses, math/robust_cost_function.cpp:62-95 (Student-t with nu = 1, robust_cost_function.h:34-35).
+template <class T>
+T loss_rho(int kind, T s, T d) {
+  if (kind == 0) return (d + T(1)) * std::log(T(1) + s / d);
+  if (d <= T(0)) return T(0.5) * s;
+  if (s < d * d) return T(0.5) * s;
+  return d * std::sqrt(s) - T(0.5) * d * d;
+}
+
+template <class T>
+T loss_drho(int kind, T s, T d) {
+  if (kind == 0) return (d + T(1)) / (d + s);
+  if (d <= T(0)) return T(0.5);
+  if (s < d * d) return T(0.5);
+  return T(0.5) * d / std::sqrt(s);
+}
+
+// Exp of a twist (omega, v), math/twist.cpp:23-41 (theta clamped at FLT_EPSILON).
+template <class T>
+Pose<T> twist_exp(const T tw[6]) {
+  const T* w = tw;
+  const T* v = tw + 3;
+  const T theta = std::max<T>(std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]), T(kFltEps));
+  const T c = std::cos(theta);
+  const T st = std::sin(theta) / theta;
+  const T n[3] = {w[0] / theta, w[1] / theta, w[2] / theta};
+  Pose<T> E;
+  const T ws[3] = {w[0] * st, w[1] * st, w[2] * st};
+  // SkewSymmetric(w * s_theta, c) + (1 - c) n n^T   (common/types.h:189-191)
+  const T K[3][3] = {{c, -ws[2], ws[1]}, {ws[2], c, -ws[0]}, {-ws[1], ws[0], c}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) E.R[i][j] = K[i][j] + (T(1) - c) * n[i] * n[j];
+  const T nxv[3] = {n[1] * v[2] - n[2] * v[1], n[2] * v[0] - n[0] * v[2], n[0] * v[1] - n[1] * v[0]};
+  const T nxnxv[3] = {n[1] * nxv[2] - n[2] * nxv[1], n[2] * nxv[0] - n[0] * nxv[2], n[0] * nxv[1] - n[1] * nxv[0]};
+  for (int i = 0; i < 3; ++i) E.t[i] = v[i] + ((T(1) - c) / theta * nxv[i] + (T(1) - st) * nxnxv[i]);
+  return E;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (Golub & Van Loan, Matrix Computations,
+// Algorithm 8.5.3). a (row-major, destroyed) -> eigenvalues in ev, eigenvectors as columns of q.
+template <class T>
+void sym_jacobi(int n, T* a, T* ev, T* q) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) q[i * n + j] = (i == j);
+  T fro = 0;
+  for (int i = 0; i < n * n; ++i) fro += a[i] * a[i];
+  const T tol = std::numeric_limits<T>::epsilon() * std::sqrt(fro);
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    T off = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a[i * n + j] * a[i * n + j];
+    if (std::sqrt(T(2) * off) <= tol) break;
+    for (int p = 0; p < n; ++p)
+      for (int r = p + 1; r < n; ++r) {
+        const T apr = a[p * n + r];
+        if (apr == T(0)) continue;
+        const T tau = (a[r * n + r] - a[p * n + p]) / (T(2) * apr);
+        const T t = (tau >= 0 ? T(1) : T(-1)) / (std::fabs(tau) + std::sqrt(T(1) + tau * tau));
+        const T c = T(1) / std::sqrt(T(1) + t * t);
+        const T s = t * c;
+        for (int k = 0; k < n; ++k) {  // A <- A J (columns p, r)
+          const T akp = a[k * n + p], akr = a[k * n + r];
+          a[k * n + p] = c * akp - s * akr;
+          a[k * n + r] = s * akp + c * akr;
+        }
+        for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, r)
+          const T apk = a[p * n + k], ark = a[r * n + k];
+          a[p * n + k] = c * apk - s * ark;
+          a[r * n + k] = s * apk + c * ark;
+        }
+        for (int k = 0; k < n; ++k) {
+          const T qkp = q[k * n + p], qkr = q[k * n + r];
+          q[k * n + p] = c * qkp - s * qkr;
+          q[k * n + r] = s * qkp + c * qkr;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) ev[i] = a[i * n + i];
+}
+
+// Pseudo-inverse of a symmetric matrix through its SVD, the way Eigen::JacobiSVD::solve with setThreshold
+// works (cpu.cpp:226-254, 339-346): a singular value |lambda_k| counts when it exceeds thr * max |lambda|.
+// For a symmetric matrix the singular values are |lambda_k| and the pseudo-inverse is sum q_k q_k^T / lambda_k.
+template <class T>
+int sym_pinv(int n, const T* A, T thr, T* out) {
+  std::vector<T> a(A, A + n * n), ev(n), q(n * n);
+  sym_jacobi(n, a.data(), ev.data(), q.data());
+  T smax = 0;
+  for (int k = 0; k < n; ++k) smax = std::max(smax, std::fabs(ev[k]));
+  int rank = 0;
+  std::fill(out, out + n * n, T(0));
+  if (smax <= T(0)) return 0;
+  for (int k = 0; k < n; ++k) {
+    if (!(std::fabs(ev[k]) > thr * smax)) continue;
+    ++rank;
+    const T inv = T(1) / ev[k];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) out[i * n + j] += q[i * n + k] * inv * q[j * n + k];
+  }
+  return rank;
+}
+
+// Point-block inverse (cpu.cpp:218-262; sba_v1.cu:1016-1069): rank 3 -> inverse, rank 2 -> drop the
+// smallest singular direction, rank <= 1 -> zero.
+template <class T>
+int point_pinv(const T M[9], T out[9]) {
+  const int rank = sym_pinv<T>(3, M, T(1e-6), out);
+  if (rank < 2) std::fill(out, out + 9, T(0));
+  return rank;
+}
+
+// Lower Cholesky A = L L^T then L L^T x = b (the GPU bundler's spotrf/spotrs, libs/cuda_modules/sba.cpp:
+// 606-640). Returns false when a pivot is not positive (devInfo != 0).
+template <class T>
+bool chol_solve(int n, const T* A, const T* b, T* x) {
+  std::vector<T> L(A, A + n * n);
+  for (int k = 0; k < n; ++k) {
+    T d = L[k * n + k];
+    for (int t = 0; t < k; ++t) d -= L[k * n + t] * L[k * n + t];
+    if (!(d > T(0))) return false;
+    const T lkk = std::sqrt(d);
+    L[k * n + k] = lkk;
+    for (int i = k + 1; i < n; ++i) {
+      T s = L[i * n + k];
+      for (int t = 0; t < k; ++t) s -= L[i * n + t] * L[k * n + t];
+      L[i * n + k] = s / lkk;
+    }
+  }
+  std::vector<T> y(n);
+  for (int i = 0; i < n; ++i) {
+    T s = b[i];
+    for (int t = 0; t < i; ++t) s -= L[i * n + t] * y[t];
+    y[i] = s / L[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    T s = y[i];
+    for (int t = i + 1; t < n; ++t) s -= L[t * n + i] * x[t];
+    x[i] = s / L[i * n + i];
+  }
+  return true;
+}
+
+// Closest rotation U V^T of a 3x3 matrix with the reflection fix, common/rotation_utils.h:36-50.
+template <class T>
+void nearest_rotation(const T M[3][3], T R[3][3]) {
+  T MtM[9], ev[3], V[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) MtM[i * 3 + j] = M[0][i] * M[0][j] + M[1][i] * M[1][j] + M[2][i] * M[2][j];
+  sym_jacobi<T>(3, MtM, ev, V);
+  int order[3] = {0, 1, 2};  // singular values descending
+  std::sort(order, order + 3, [&](int a, int b) { return ev[a] > ev[b]; });
+  T U[3][3], Vs[3][3];
+  for (int k = 0; k < 3; ++k) {
+    const int c = order[k];
+    const T s = std::sqrt(std::max(ev[c], T(0)));
+    for (int i = 0; i < 3; ++i) Vs[i][k] = V[i * 3 + c];
+    for (int i = 0; i < 3; ++i)
+      U[i][k] = s > T(0) ? (M[i][0] * Vs[0][k] + M[i][1] * Vs[1][k] + M[i][2] * Vs[2][k]) / s : T(0);
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = U[i][0] * Vs[j][0] + U[i][1] * Vs[j][1] + U[i][2] * Vs[j][2];
+  const T det = R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1]) - R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0]) +
+                R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]);
+  if (det < T(0))
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) R[i][j] -= T(2) * U[i][2] * Vs[j][2];
+}
+
+template <class T>
+struct Model {  // ModelFunction, schur_complement_bundler_cpu.h:29-41
+  std::vector<T> r;   // 2 per observation
+  std::vector<T> Jc;  // 2x6 row-major per observation
+  std::vector<T> Jp;  // 2x3 row-major per observation
+  std::vector<T> w;
+};
+
+template <class T>
+struct Full {  // FullSystem, schur_complement_bundler_cpu.h:50-56 (block-sparse point_pose_block)
+  int np, nc;
+  std::vector<T> U;   // [nc][6][6]
+  std::vector<T> ea;  // [6 nc]
+  std::vector<T> V;   // [np][3][3]
+  std::vector<T> eb;  // [3 np]
+  // point_pose_block blocks hpc = w Jp^T L Jc (3x6) per (free point, free pose), listed per point
+  std::vector<std::vector<std::pair<int, std::vector<T>>>> Wt;
+};
+
+template <class T>
+struct Solver {
+  // problem
+  int np, npf, no, nc, ncf, ncam;
+  std::vector<T> pts;
+  std::vector<Pose<T>> poses;
+  std::vector<Pose<T>> cams;
+  const OrcProblem* P;
+  OrcOptions opt;
+  T delta;
+
+  void load(const OrcProblem* p, const OrcOptions* o) {
+    P = p;
+    opt = *o;
+    np = p->num_points;
+    npf = p->num_points - p->num_fixed_points;
+    no = p->num_observations;
+    nc = p->num_poses;
+    ncf = p->num_poses - p->num_fixed_key_frames;
+    ncam = p->num_cameras;
+    pts.resize(3 * static_cast<size_t>(np));
+    for (size_t i = 0; i < pts.size(); ++i) pts[i] = static_cast<T>(p->points[i]);
+    poses.resize(nc);
+    for (int j = 0; j < nc; ++j) poses[j] = load_pose<T>(p->rig_from_world + 16 * j);
+    cams.resize(ncam);
+    for (int c = 0; c < ncam; ++c) cams[c] = load_pose<T>(p->camera_from_rig + 16 * c);
+    delta = static_cast<T>(p->robustifier_scale);
+  }
+
+  T info(int o, int r, int c) const { return static_cast<T>(P->observation_infos[4 * o + c * 2 + r]); }
+
+  // UpdateModel, cpu.cpp:64-131.
+  void update_model(Model<T>& m) const {
+    m.r.assign(2 * no, T(0));
+    m.Jc.assign(12 * no, T(0));
+    m.Jp.assign(6 * no, T(0));
+    m.w.assign(no, T(0));
+    for (int o = 0; o < no; ++o) {
+      const int pi = P->point_ids[o], pj = P->pose_ids[o], cam = P->camera_ids[o];
+      const Pose<T>& Tcr = cams[cam];
+      const Pose<T> Tcw = compose(Tcr, poses[pj]);
+      T pc[3];
+      apply(Tcw, &pts[3 * pi], pc);
+      if (!(pc[2] > T(kHither))) continue;
+      const T pred[2] = {pc[0] / pc[2], pc[1] / pc[2]};
+      const T r[2] = {static_cast<T>(P->observation_xys[2 * o]) - pred[0],
+                      static_cast<T>(P->observation_xys[2 * o + 1]) - pred[1]};
+      const T iz = T(1) / pc[2];
+      const T dproj[2][3] = {{iz, T(0), -pred[0] * iz}, {T(0), iz, -pred[1] * iz}};
+      T* Jp = &m.Jp[6 * o];
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 3; ++b)
+          Jp[a * 3 + b] = dproj[a][0] * Tcw.R[0][b] + dproj[a][1] * Tcw.R[1][b] + dproj[a][2] * Tcw.R[2][b];
+      // camera twist Jacobian [dproj * skew(-pc), dproj], then rig twist via Adjoint(camera_from_rig)
+      const T S[3][3] = {{T(0), pc[2], -pc[1]}, {-pc[2], T(0), pc[0]}, {pc[1], -pc[0], T(0)}};  // skew(-pc)
+      T Jcam[2][6];
+      for (int a = 0; a < 2; ++a) {
+        for (int b = 0; b < 3; ++b) Jcam[a][b] = dproj[a][0] * S[0][b] + dproj[a][1] * S[1][b] + dproj[a][2] * S[2][b];
+        for (int b = 0; b < 3; ++b) Jcam[a][3 + b] = dproj[a][b];
+      }
+      T Ad[6][6] = {};
+      const T tx[3][3] = {{T(0), -Tcr.t[2], Tcr.t[1]}, {Tcr.t[2], T(0), -Tcr.t[0]}, {-Tcr.t[1], Tcr.t[0], T(0)}};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          Ad[a][b] = Tcr.R[a][b];
+          Ad[3 + a][3 + b] = Tcr.R[a][b];
+          Ad[3 + a][b] = tx[a][0] * Tcr.R[0][b] + tx[a][1] * Tcr.R[1][b] + tx[a][2] * Tcr.R[2][b];
+        }
+      T* Jc = &m.Jc[12 * o];
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 6; ++b) {
+          T s = 0;
+          for (int k = 0; k < 6; ++k) s += Jcam[a][k] * Ad[k][b];
+          Jc[a * 6 + b] = s;
+        }
+      m.r[2 * o] = r[0];
+      m.r[2 * o + 1] = r[1];
+      const T sq = r[0] * (info(o, 0, 0) * r[0] + info(o, 0, 1) * r[1]) + r[1] * (info(o, 1, 0) * r[0] + info(o, 1, 1) * r[1]);
+      m.w[o] = loss_drho<T>(opt.loss, sq, delta);
+    }
+  }
+
+  // BuildFullSystem, cpu.cpp:136-209.
+  void build_full(Full<T>& F, const Model<T>& m) const {
+    F.np = npf;
+    F.nc = ncf;
+    F.U.assign(36 * static_cast<size_t>(ncf), T(0));
+    F.ea.assign(6 * static_cast<size_t>(ncf), T(0));
+    F.V.assign(9 * static_cast<size_t>(npf), T(0));
+    F.eb.assign(3 * static_cast<size_t>(npf), T(0));
+    F.Wt.assign(npf, {});
+    for (int o = 0; o < no; ++o) {
+      const T w = m.w[o];
+      const T* Jc = &m.Jc[12 * o];
+      const T* Jp = &m.Jp[6 * o];
+      T L[2][2] = {{info(o, 0, 0), info(o, 0, 1)}, {info(o, 1, 0), info(o, 1, 1)}};
+      T LJc[2][6], LJp[2][3];
+      for (int a = 0; a < 2; ++a) {
+        for (int b = 0; b < 6; ++b) LJc[a][b] = L[a][0] * Jc[b] + L[a][1] * Jc[6 + b];
+        for (int b = 0; b < 3; ++b) LJp[a][b] = L[a][0] * Jp[b] + L[a][1] * Jp[3 + b];
+      }
+      const T wr[2] = {w * m.r[2 * o], w * m.r[2 * o + 1]};
+      const T Lwr[2] = {L[0][0] * wr[0] + L[0][1] * wr[1], L[1][0] * wr[0] + L[1][1] * wr[1]};
+      const int c = P->pose_ids[o], p = P->point_ids[o];
+      if (c < ncf && p < npf) {
+        auto& lst = F.Wt[p];
+        std::vector<T>* blk = nullptr;
+        for (auto& e : lst)
+          if (e.first == c) blk = &e.second;
+        if (!blk) {
+          lst.emplace_back(c, std::vector<T>(18, T(0)));
+          blk = &lst.back().second;
+        }
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 6; ++b) (*blk)[a * 6 + b] += (Jp[a] * LJc[0][b] + Jp[3 + a] * LJc[1][b]) * w;
+      }
+      if (c < ncf) {
+        T* U = &F.U[36 * c];
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) U[a * 6 + b] += (Jc[a] * LJc[0][b] + Jc[6 + a] * LJc[1][b]) * w;
+        for (int a = 0; a < 6; ++a) F.ea[6 * c + a] += Jc[a] * Lwr[0] + Jc[6 + a] * Lwr[1];
+      }
+      if (p < npf) {
+        T* V = &F.V[9 * p];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) V[a * 3 + b] += (Jp[a] * LJp[0][b] + Jp[3 + a] * LJp[1][b]) * w;
+        for (int a = 0; a < 3; ++a) F.eb[3 * p + a] += Jp[a] * Lwr[0] + Jp[3 + a] * Lwr[1];
+      }
+    }
+  }
+
+  struct Reduced {  // ReducedSystem, cpu.h:58-64 (Y kept block-sparse per point)
+    std::vector<T> S, e, inv, gb;
+    std::vector<std::vector<std::pair<int, std::vector<T>>>> Y;  // per point: (pose, 6x3 block)
+  };
+
+  // BuildReducedSystem, cpu.cpp:211-287.
+  void build_reduced(Reduced& R, const Full<T>& F, T lambda) const {
+    const int n = 6 * ncf;
+    R.inv.assign(9 * static_cast<size_t>(npf), T(0));
+    R.gb.assign(3 * static_cast<size_t>(npf), T(0));
+    for (int i = 0; i < npf; ++i) {
+      T M[9];
+      for (int k = 0; k < 9; ++k) M[k] = F.V[9 * i + k];
+      for (int d = 0; d < 3; ++d) M[4 * d] += F.V[9 * i + 4 * d] * lambda;
+      point_pinv<T>(M, &R.inv[9 * i]);
+      for (int a = 0; a < 3; ++a)
+        R.gb[3 * i + a] = R.inv[9 * i + 3 * a] * F.eb[3 * i] + R.inv[9 * i + 3 * a + 1] * F.eb[3 * i + 1] +
+                          R.inv[9 * i + 3 * a + 2] * F.eb[3 * i + 2];
+    }
+    R.S.assign(static_cast<size_t>(n) * n, T(0));
+    for (int c = 0; c < ncf; ++c)
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) R.S[(6 * c + a) * n + 6 * c + b] = F.U[36 * c + 6 * a + b];
+    R.e.assign(F.ea.begin(), F.ea.end());
+    R.Y.assign(npf, {});
+    if (opt.dense_schur) {
+      // temp = point_pose_block^T (n x 3np), column blocks *= inv, S -= temp * point_pose_block (cpu.cpp:264-277)
+      const size_t K = 3 * static_cast<size_t>(npf);
+      std::vector<T> ppb(K * n, T(0));  // point_pose_block, row-major (3np x n)
+      for (int i = 0; i < npf; ++i)
+        for (auto& e : F.Wt[i])
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 6; ++b) ppb[(3 * i + a) * n + 6 * e.first + b] = e.second[a * 6 + b];
+      std::vector<T> temp(static_cast<size_t>(n) * K);  // row-major (n x 3np)
+      for (size_t k = 0; k < K; ++k)
+        for (int a = 0; a < n; ++a) temp[a * K + k] = ppb[k * n + a];
+      for (int i = 0; i < npf; ++i)
+        for (int a = 0; a < n; ++a) {
+          T row[3] = {temp[a * K + 3 * i], temp[a * K + 3 * i + 1], temp[a * K + 3 * i + 2]};
+          for (int c = 0; c < 3; ++c)
+            temp[a * K + 3 * i + c] = row[0] * R.inv[9 * i + c] + row[1] * R.inv[9 * i + 3 + c] + row[2] * R.inv[9 * i + 6 + c];
+        }
+      for (int a = 0; a < n; ++a) {
+        T* Srow = &R.S[static_cast<size_t>(a) * n];
+        const T* trow = &temp[a * K];
+        for (size_t k = 0; k < K; ++k) {
+          const T t = trow[k];
+          const T* prow = &ppb[k * n];
+          for (int b = 0; b < n; ++b) Srow[b] -= t * prow[b];
+        }
+        T s = 0;
+        for (size_t k = 0; k < K; ++k) s += trow[k] * F.eb[k];
+        R.e[a] -= s;
+      }
+      for (int i = 0; i < npf; ++i)
+        for (auto& e : F.Wt[i]) {
+          std::vector<T> y(18);
+          for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 3; ++c) y[a * 3 + c] = temp[(6 * e.first + a) * K + 3 * i + c];
+          R.Y[i].emplace_back(e.first, y);
+        }
+    } else {
+      for (int i = 0; i < npf; ++i) {
+        const T* inv = &R.inv[9 * i];
+        for (auto& e : F.Wt[i]) {
+          std::vector<T> y(18);  // Y = hpc^T * inv (6x3)
+          for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 3; ++c)
+              y[a * 3 + c] = e.second[a] * inv[c] + e.second[6 + a] * inv[3 + c] + e.second[12 + a] * inv[6 + c];
+          R.Y[i].emplace_back(e.first, y);
+        }
+        for (auto& ya : R.Y[i]) {
+          const int ca = ya.first;
+          for (auto& wb : F.Wt[i]) {
+            const int cb = wb.first;
+            for (int a = 0; a < 6; ++a)
+              for (int b = 0; b < 6; ++b)
+                R.S[(6 * ca + a) * n + 6 * cb + b] -= ya.second[a * 3] * wb.second[b] + ya.second[a * 3 + 1] * wb.second[6 + b] +
+                                                      ya.second[a * 3 + 2] * wb.second[12 + b];
+          }
+          for (int a = 0; a < 6; ++a)
+            R.e[6 * ca + a] -= ya.second[a * 3] * F.eb[3 * i] + ya.second[a * 3 + 1] * F.eb[3 * i + 1] +
+                               ya.second[a * 3 + 2] * F.eb[3 * i + 2];
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) R.S[static_cast<size_t>(i) * n + i] += lambda * F.U[36 * (i / 6) + 7 * (i % 6)];
+  }
+
+  struct Update {
+    std::vector<T> dc, dp;  // pose_step (6 ncf), point_step (3 npf)
+    std::vector<Pose<T>> pose;
+    bool ok = true;
+  };
+
+  // ComputeUpdate, cpu.cpp:337-365 (Jacobi pseudo-inverse) or the GPU bundler's Cholesky (sba.cpp:606-640).
+  void compute_update(Update& u, const Reduced& R) const {
+    const int n = 6 * ncf;
+    u.dc.assign(n, T(0));
+    u.ok = true;
+    if (n > 0) {
+      if (opt.rcs_solver == 0) {
+        u.ok = chol_solve<T>(n, R.S.data(), R.e.data(), u.dc.data());
+      } else {
+        std::vector<T> pinv(static_cast<size_t>(n) * n);
+        sym_pinv<T>(n, R.S.data(), T(1e-6), pinv.data());
+        for (int a = 0; a < n; ++a) {
+          T s = 0;
+          for (int b = 0; b < n; ++b) s += pinv[static_cast<size_t>(a) * n + b] * R.e[b];
+          u.dc[a] = s;
+        }
+      }
+    }
+    u.dp.assign(3 * static_cast<size_t>(npf), T(0));
+    for (int i = 0; i < npf; ++i) {
+      T d[3] = {R.gb[3 * i], R.gb[3 * i + 1], R.gb[3 * i + 2]};
+      for (auto& y : R.Y[i])
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < 6; ++a) d[c] -= y.second[a * 3 + c] * u.dc[6 * y.first + a];
+      for (int c = 0; c < 3; ++c) u.dp[3 * i + c] = d[c];
+    }
+    u.pose.resize(ncf);
+    for (int j = 0; j < ncf; ++j) u.pose[j] = twist_exp<T>(&u.dc[6 * j]);
+  }
+
+  // EvaluateCost, cpu.cpp:289-335. Returns +inf when every observation is inside the near plane.
+  T evaluate_cost(const Update* u, int* skipped_out) const {
+    T cost = 0;
+    int skipped = 0;
+    for (int o = 0; o < no; ++o) {
+      const int pi = P->point_ids[o], pj = P->pose_ids[o], cam = P->camera_ids[o];
+      T pw[3] = {pts[3 * pi], pts[3 * pi + 1], pts[3 * pi + 2]};
+      if (u && pi < npf)
+        for (int d = 0; d < 3; ++d) pw[d] += u->dp[3 * pi + d];
+      T pr0[3], pr[3], pc[3];
+      apply(poses[pj], pw, pr0);
+      if (u && pj < ncf)
+        apply(u->pose[pj], pr0, pr);
+      else
+        std::copy(pr0, pr0 + 3, pr);
+      apply(cams[cam], pr, pc);
+      if (pc[2] > T(kHither)) {
+        const T r[2] = {static_cast<T>(P->observation_xys[2 * o]) - pc[0] / pc[2],
+                        static_cast<T>(P->observation_xys[2 * o + 1]) - pc[1] / pc[2]};
+        const T sq = r[0] * (info(o, 0, 0) * r[0] + info(o, 0, 1) * r[1]) + r[1] * (info(o, 1, 0) * r[0] + info(o, 1, 1) * r[1]);
+        cost += loss_rho<T>(opt.loss, sq, delta);
+      } else {
+        ++skipped;
+      }
+    }
+    if (skipped_out) *skipped_out = skipped;
+    if (skipped == no) return std::numeric_limits<T>::infinity();
+    return cost;
+  }
+
+  // UpdateState, cpu.cpp:367-383.
+  void update_state(const Update& u) {
+    for (int i = 0; i < npf; ++i)
+      for (int d = 0; d < 3; ++d) pts[3 * i + d] += u.dp[3 * i + d];
+    for (int j = 0; j < ncf; ++j) {
+      Pose<T> m = compose(u.pose[j], poses[j]);
+      if (opt.reorthonormalize) {
+        T R[3][3];
+        nearest_rotation<T>(m.R, R);
+        std::memcpy(m.R, R, sizeof(R));
+      }
+      poses[j] = m;
+    }
+  }
+
+  // ComputePredictedRelativeReduction, cpu.cpp:385-433; returns the numerators (hessian, scaling) too.
+  T predicted(T current, T lambda, const Update& u, const Full<T>& F, T* hess_out, T* scal_out) const {
+    T pose_h = 0, cross = 0, point_h = 0, pose_s = 0, point_s = 0;
+    for (int c = 0; c < ncf; ++c)
+      for (int a = 0; a < 6; ++a) {
+        T s = 0;
+        for (int b = 0; b < 6; ++b) s += F.U[36 * c + 6 * a + b] * u.dc[6 * c + b];
+        pose_h += u.dc[6 * c + a] * s;
+        pose_s += u.dc[6 * c + a] * F.U[36 * c + 7 * a] * u.dc[6 * c + a];
+      }
+    for (int i = 0; i < npf; ++i) {
+      for (auto& e : F.Wt[i])
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 6; ++b) cross += u.dp[3 * i + a] * e.second[a * 6 + b] * u.dc[6 * e.first + b];
+      for (int a = 0; a < 3; ++a) {
+        T s = 0;
+        for (int b = 0; b < 3; ++b) s += F.V[9 * i + 3 * a + b] * u.dp[3 * i + b];
+        point_h += u.dp[3 * i + a] * s;
+        point_s += u.dp[3 * i + a] * F.V[9 * i + 4 * a] * u.dp[3 * i + a];
+      }
+    }
+    const T hess = pose_h + T(2) * cross + point_h;
+    const T scal = pose_s + point_s;
+    if (hess_out) *hess_out = hess;
+    if (scal_out) *scal_out = scal;
+    return (pose_h + cross) / current + (cross + point_h) / current + ((pose_s + point_s) / current) * T(2) * lambda;
+  }
+
+  static T inf_norm(const std::vector<T>& v) {
+    T m = 0;
+    for (T x : v) m = std::max(m, std::fabs(x));
+    return m;
+  }
+
+  // SchurComplementBundlerCpu::solve (cpu.cpp:435-529) with the GPU bundler's handling of a failed
+  // factorization (gpu.cpp:204-209).
+  int solve(OrcResult* res) {
+    res->iterations = 0;
+    res->accepted = 0;
+    if (np == 0 || nc == 0 || no == 0) return 1;  // gpu.cpp:121-123
+    Update u;
+    u.dc.assign(6 * static_cast<size_t>(ncf), T(0));
+    u.dp.assign(3 * static_cast<size_t>(npf), T(0));
+    Model<T> model;
+    update_model(model);
+    const T initial = evaluate_cost(nullptr, nullptr);
+    T current = initial;
+    res->initial_cost = static_cast<double>(initial);
+    res->last_cost = static_cast<double>(initial);
+    res->lambda = 1e-3;
+    if (initial < T(kFltEps)) return 0;
+    if (std::isinf(initial)) return 2;
+    Full<T> F;
+    build_full(F, model);
+    T lambda = T(0.001f);
+    Reduced R;
+    int it = 0;
+    while (it < P->max_iterations) {
+      ++it;
+      res->iterations = it;
+      int flags = 0;
+      build_reduced(R, F, lambda);
+      compute_update(u, R);
+      const T lambda_used = lambda;
+      T cost = 0, rho = 0;
+      bool stop = false;
+      if (!u.ok) {
+        lambda *= T(5);
+        flags |= 2;
+      } else {
+        cost = evaluate_cost(&u, nullptr);
+        if (current < initial * T(kFltEps)) {
+          stop = true;
+          flags |= 4;
+        } else {
+          const T pred = predicted(current, lambda, u, F, nullptr, nullptr);
+          if (!opt.fixed_iterations && pred < T(kSqrtFltEps) && inf_norm(u.dc) < T(kSqrtFltEps) &&
+              inf_norm(u.dp) < T(kSqrtFltEps)) {
+            stop = true;
+            flags |= 4;
+          } else {
+            rho = (T(1) - cost / current) / pred;
+            if (rho > T(0.25f)) {
+              if (rho > T(0.75f) && lambda * T(0.125f) > T(0)) lambda *= T(0.125f);
+              current = cost;
+              res->last_cost = static_cast<double>(cost);
+              res->accepted += 1;
+              flags |= 1;
+              update_state(u);
+              update_model(model);
+              build_full(F, model);
+            } else {
+              lambda *= T(5);
+            }
+          }
+        }
+      }
+      if (res->trace_capacity >= it) {
+        if (res->trace_lambda) res->trace_lambda[it - 1] = static_cast<double>(lambda_used);
+        if (res->trace_cost) res->trace_cost[it - 1] = static_cast<double>(cost);
+        if (res->trace_rho) res->trace_rho[it - 1] = static_cast<double>(rho);
+        if (res->trace_flags) res->trace_flags[it - 1] = flags;
+      }
+      if (stop) break;
+    }
+    res->lambda = static_cast<double>(lambda);
+    return 0;
+  }
+
+  void export_state(OrcResult* res) const {
+    if (res->points)
+      for (int i = 0; i < 3 * np; ++i) res->points[i] = static_cast<double>(pts[i]);
+    if (res->poses)
+      for (int j = 0; j < nc; ++j) store_pose(poses[j], res->poses + 16 * j);
+  }
+};
+
+template <class T>
+int solve_impl(const OrcProblem* p, const OrcOptions* o, OrcResult* r) {
+  if (!p || !o || !r) return 5;
+  Solver<T> s;
+  s.load(p, o);
+  const int status = s.solve(r);
+  r->status = status;
+  s.export_state(r);
+  return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_solve_f64(const OrcProblem* p, const OrcOptions* o, OrcResult* r) { return solve_impl<double>(p, o, r); }
+int orc_solve_f32(const OrcProblem* p, const OrcOptions* o, OrcResult* r) { return solve_impl<float>(p, o, r); }
+
+// Per-observation residual (2), pose Jacobian (2x6 row-major), point Jacobian (2x3), weight, at the input state.
+int orc_linearize_f64(const OrcProblem* p, const OrcOptions* o, double* r, double* Jc, double* Jp, double* w) {
+  Solver<double> s;
+  s.load(p, o);
+  Model<double> m;
+  s.update_model(m);
+  std::copy(m.r.begin(), m.r.end(), r);
+  std::copy(m.Jc.begin(), m.Jc.end(), Jc);
+  std::copy(m.Jp.begin(), m.Jp.end(), Jp);
+  std::copy(m.w.begin(), m.w.end(), w);
+  return 0;
+}
+
+// One LM step at the input state with damping lambda (the phases of cpu.cpp:479-494):
+// S [n*n] damped reduced system, e [n], dc [n], dp [3 npf];
+// scalars: trial cost, skipped, delta^T H delta, delta^T diag(H) delta, max|dp|, max|dc|, current cost, ok.
+int orc_step_f64(const OrcProblem* p, const OrcOptions* o, double lambda, double* S, double* e, double* dc,
+                 double* dp, double* scalars) {
+  Solver<double> s;
+  s.load(p, o);
+  Model<double> m;
+  s.update_model(m);
+  Full<double> F;
+  s.build_full(F, m);
+  Solver<double>::Reduced R;
+  s.build_reduced(R, F, lambda);
+  if (S) std::copy(R.S.begin(), R.S.end(), S);
+  if (e) std::copy(R.e.begin(), R.e.end(), e);
+  Solver<double>::Update u;
+  s.compute_update(u, R);
+  if (dc) std::copy(u.dc.begin(), u.dc.end(), dc);
+  if (dp) std::copy(u.dp.begin(), u.dp.end(), dp);
+  int skipped = 0;
+  const double current = s.evaluate_cost(nullptr, nullptr);
+  const double trial = s.evaluate_cost(&u, &skipped);
+  double hess = 0, scal = 0;
+  s.predicted(current > 0 ? current : 1.0, lambda, u, F, &hess, &scal);
+  if (scalars) {
+    scalars[0] = trial;
+    scalars[1] = skipped;
+    scalars[2] = hess;
+    scalars[3] = scal;
+    scalars[4] = Solver<double>::inf_norm(u.dp);
+    scalars[5] = Solver<double>::inf_norm(u.dc);
+    scalars[6] = current;
+    scalars[7] = u.ok ? 1.0 : 0.0;
+  }
+  return u.ok ? 0 : 2;
+}
+
+// Cost at the input state moved by (dc, dp) (either may be null for a zero step).
+double orc_cost_f64(const OrcProblem* p, const OrcOptions* o, const double* dc, const double* dp, int32_t* skipped) {
+  Solver<double> s;
+  s.load(p, o);
+  Solver<double>::Update u;
+  u.dc.assign(6 * static_cast<size_t>(s.ncf), 0.0);
+  u.dp.assign(3 * static_cast<size_t>(s.npf), 0.0);
+  if (dc) std::copy(dc, dc + u.dc.size(), u.dc.begin());
+  if (dp) std::copy(dp, dp + u.dp.size(), u.dp.begin());
+  u.pose.resize(s.ncf);
+  for (int j = 0; j < s.ncf; ++j) u.pose[j] = twist_exp<double>(&u.dc[6 * j]);
+  int sk = 0;
+  const double c = s.evaluate_cost(&u, &sk);
+  if (skipped) *skipped = sk;
+  return c;
+}
+
+int orc_chol_solve_f64(int32_t n, const double* A, const double* b, double* x) { return chol_solve<double>(n, A, b, x) ? 0 : 1; }
+int orc_chol_solve_f32(int32_t n, const float* A, const float* b, float* x) { return chol_solve<float>(n, A, b, x) ? 0 : 1; }
+int orc_sym_pinv_f64(int32_t n, const double* A, double thr, double* out) { return sym_pinv<double>(n, A, thr, out); }
+int orc_point_pinv_f64(const double* A, double* out) { return point_pinv<double>(A, out); }
+
+void orc_exp_f64(const double* tw, double* m16) { store_pose(twist_exp<double>(tw), m16); }
+
+void orc_nearest_rotation_f64(const double* M9, double* R9) {
+  double M[3][3], R[3][3];
+  for (int i = 0; i < 9; ++i) M[i / 3][i % 3] = M9[i];
+  nearest_rotation<double>(M, R);
+  for (int i = 0; i < 9; ++i) R9[i] = R[i / 3][i % 3];
+}
+
+double orc_loss_f64(int32_t kind, double s, double d, int32_t derivative) {
+  return derivative ? loss_drho<double>(kind, s, d) : loss_rho<double>(kind, s, d);
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""TEST INFRASTRUCTURE -- ctypes wrapper of the CPU oracle (oracle/sba_oracle.cpp).
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg use this module, as the checker or as the
+timed CPU baseline. ``solve_f64`` is the FP64 plain method (Cholesky RCS solve, the GPU bundler's control
+flow); ``solve_f32`` restates the reference CPU path (schur_complement_bundler_cpu.cpp: FP32, dense Schur
+product, Jacobi pseudo-inverse of the RCS with threshold 1e-6).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_sba.so")
+
+
+class OrcOptions(C.Structure):
+    _fields_ = [("loss", C.c_int32), ("rcs_solver", C.c_int32), ("dense_schur", C.c_int32),
+                ("fixed_iterations", C.c_int32), ("reorthonormalize", C.c_int32), ("pad", C.c_int32 * 3)]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("points", C.POINTER(C.c_double)), ("poses", C.POINTER(C.c_double)), ("iterations", C.c_int32),
+                ("status", C.c_int32), ("accepted", C.c_int32), ("pad", C.c_int32), ("initial_cost", C.c_double),
+                ("last_cost", C.c_double), ("lambda_", C.c_double), ("trace_capacity", C.c_int32),
+                ("pad2", C.c_int32), ("trace_lambda", C.POINTER(C.c_double)), ("trace_cost", C.POINTER(C.c_double)),
+                ("trace_rho", C.POINTER(C.c_double)), ("trace_flags", C.POINTER(C.c_int32))]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        from cuvslam_amd._lib import ProblemView
+        PV = C.POINTER(ProblemView)
+        dp = C.POINTER(C.c_double)
+        for name in ("orc_solve_f64", "orc_solve_f32"):
+            getattr(L, name).argtypes = [PV, C.POINTER(OrcOptions), C.POINTER(OrcResult)]
+            getattr(L, name).restype = C.c_int
+        L.orc_linearize_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp]
+        L.orc_step_f64.argtypes = [PV, C.POINTER(OrcOptions), C.c_double, dp, dp, dp, dp, dp]
+        L.orc_step_f64.restype = C.c_int
+        L.orc_cost_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, C.POINTER(C.c_int32)]
+        L.orc_cost_f64.restype = C.c_double
+        L.orc_chol_solve_f64.argtypes = [C.c_int32, dp, dp, dp]
+        L.orc_sym_pinv_f64.argtypes = [C.c_int32, dp, C.c_double, dp]
+        L.orc_point_pinv_f64.argtypes = [dp, dp]
+        L.orc_exp_f64.argtypes = [dp, dp]
+        L.orc_exp_f64.restype = None
+        L.orc_nearest_rotation_f64.argtypes = [dp, dp]
+        L.orc_nearest_rotation_f64.restype = None
+        L.orc_loss_f64.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32]
+        L.orc_loss_f64.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def options(loss=0, rcs_solver=0, dense_schur=0, fixed_iterations=0, reorthonormalize=1):
+    o = OrcOptions()
+    o.loss, o.rcs_solver, o.dense_schur = loss, rcs_solver, dense_schur
+    o.fixed_iterations, o.reorthonormalize = fixed_iterations, reorthonormalize
+    return o
+
+
+def solve(problem, precision="f64", trace=64, **opt):
+    """Runs the oracle; returns a dict with final points (N,3), poses (N,4,4) [row, col], statistics and trace."""
+    L = lib()
+    v = problem.view()
+    o = options(**opt)
+    n_p, n_c = problem.points.shape[0], problem.rig_from_world.shape[0]
+    pts = np.zeros((n_p, 3))
+    poses = np.zeros((n_c, 16))
+    tl, tc, tr = np.zeros(trace), np.zeros(trace), np.zeros(trace)
+    tf = np.zeros(trace, np.int32)
+    r = OrcResult()
+    r.points, r.poses = _d(pts), _d(poses)
+    r.trace_capacity = trace
+    r.trace_lambda, r.trace_cost, r.trace_rho = _d(tl), _d(tc), _d(tr)
+    r.trace_flags = tf.ctypes.data_as(C.POINTER(C.c_int32))
+    fn = L.orc_solve_f64 if precision == "f64" else L.orc_solve_f32
+    rc = fn(C.byref(v), C.byref(o), C.byref(r))
+    it = r.iterations
+    return dict(status=rc, points=pts, poses=poses.reshape(n_c, 4, 4).transpose(0, 2, 1), iterations=it,
+                accepted=r.accepted, initial_cost=r.initial_cost, last_cost=r.last_cost, lam=r.lambda_,
+                trace_lambda=tl[:it], trace_cost=tc[:it], trace_rho=tr[:it], trace_flags=tf[:it])
+
+
+def linearize(problem, **opt):
+    L = lib()
+    v = problem.view()
+    n = problem.num_observations
+    r, Jc, Jp, w = np.zeros((n, 2)), np.zeros((n, 2, 6)), np.zeros((n, 2, 3)), np.zeros(n)
+    L.orc_linearize_f64(C.byref(v), C.byref(options(**opt)), _d(r), _d(Jc), _d(Jp), _d(w))
+    return r, Jc, Jp, w
+
+
+def step(problem, lam, **opt):
+    """One LM step at the problem's state: (S, e, dc, dp, scalars, rc)."""
+    L = lib()
+    v = problem.view()
+    ncf = problem.rig_from_world.shape[0] - problem.num_fixed_key_frames
+    npf = problem.points.shape[0] - problem.num_fixed_points
+    n = 6 * ncf
+    S, e, dc, dp, sc = np.zeros((n, n)), np.zeros(n), np.zeros(n), np.zeros(3 * npf), np.zeros(8)
+    rc = L.orc_step_f64(C.byref(v), C.byref(options(**opt)), lam, _d(S), _d(e), _d(dc), _d(dp), _d(sc))
+    return S, e, dc, dp, sc, rc
+
+
+def cost(problem, dc=None, dp=None, **opt):
+    L = lib()
+    v = problem.view()
+    sk = C.c_int32()
+    dcp = _d(np.ascontiguousarray(dc, dtype=np.float64)) if dc is not None else None
+    dpp = _d(np.ascontiguousarray(dp, dtype=np.float64)) if dp is not None else None
+    c = L.orc_cost_f64(C.byref(v), C.byref(options(**opt)), dcp, dpp, C.byref(sk))
+    return c, sk.value
+
+
+def chol_solve(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    rc = lib().orc_chol_solve_f64(A.shape[0], _d(A), _d(b), _d(x))
+    return x, rc
+
+
+def sym_pinv(A, thr=1e-6):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros_like(A)
+    rank = lib().orc_sym_pinv_f64(A.shape[0], _d(A), thr, _d(out))
+    return out, rank
+
+
+def point_pinv(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros((3, 3))
+    rank = lib().orc_point_pinv_f64(_d(A), _d(out))
+    return out, rank
+
+
+def exp(tw):
+    tw = np.ascontiguousarray(tw, dtype=np.float64)
+    m = np.zeros(16)
+    lib().orc_exp_f64(_d(tw), _d(m))
+    return m.reshape(4, 4).T.copy()
+
+
+def nearest_rotation(M):
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    R = np.zeros((3, 3))
+    lib().orc_nearest_rotation_f64(_d(M), _d(R))
+    return R
+
+
+def loss(kind, s, d, derivative=False):
+    return lib().orc_loss_f64(kind, s, d, int(derivative))
