@@ -1,0 +1,192 @@
+"""Benchmark: ms per SBA solve (10 LM iterations) on one MI355X, relative cost error vs the FP64 oracle.
+
+A step is one whole solve of the window from its initial state, with the inputs resident in HBM: the
+hipGraph of one solve (initial cost, 10 LM iterations, gather) replayed on the solver's stream.
+
+Default workload: 30 key frames (2 fixed), 20,000 points, pinhole 450 px, N(0, 0.5 px) noise, 5% outliers,
+Huber 1 px, exactly 10 LM iterations. `--workload c3` runs the batched form (512 uneven windows per step).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--chunk-pairs", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SBA batch is sharded across GPUs by window with no exchange, "
+              "and multi-GPU runs are not part of this build (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import cuvslam_amd as ca
+    from cuvslam_amd import roofline as RF
+
+    t0 = time.time()
+    if args.workload == "c2":
+        problems = [ca.scene.generate("c2", seed=2, num_fixed_keyframes=2, max_iterations=10)[0]]
+        workload = "30 key frames (2 fixed), 20000 points, pinhole 450 px, 0.5 px noise, 5% outliers, Huber 1 px"
+    else:
+        problems = ca.scene.batched_windows(512, seed=3, max_iterations=10)
+        workload = "512 windows: key frames U[5,40], points U[500,8000], outliers U[0,10]%, Huber 1 px"
+    pristine = [p.copy() for p in problems]
+    n_obs = sum(p.num_observations for p in problems)
+    log(f"[bench] generated {len(problems)} window(s), {n_obs} observations in {time.time() - t0:.1f}s")
+
+    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=args.chunk_pairs)
+    t0 = time.time()
+    batch = solver.batch(problems)
+    info = batch.info()
+    log(f"[bench] packed and uploaded in {time.time() - t0:.1f}s: {info}")
+
+    for _ in range(args.warmup):
+        batch.run()
+    batch.sync()
+    log("[bench] warmup done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    batch.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    solves_per_s = len(problems) * args.steps / elapsed
+    log(f"[bench] {args.steps} steps in {elapsed:.3f}s: {ms_per_step:.3f} ms/step, {solves_per_s:.1f} solves/s")
+
+    # per-kernel durations: the same solve launched eagerly with HIP events around every kernel
+    ms, launches = batch.kernel_times()
+    names = ["k_schur", "k_reduce", "k_factor", "k_backsub", "k_decide"]
+    per_launch = {n: (m / max(l, 1)) for n, m, l in zip(names, ms, launches)}
+    log(f"[bench] per-launch ms: {per_launch}")
+
+    work = RF.work(problems)
+    # the dominant kernel and the roofline that bounds it
+    dom = max(["k_schur", "k_factor", "k_backsub"], key=lambda n: per_launch[n])
+    key = {"k_schur": "schur", "k_factor": "factor", "k_backsub": "backsub"}[dom]
+    bytes_, f32, f64 = work[key]
+    t_launch = per_launch[dom] * 1e-3
+    if f64 > f32:
+        ach, peak, unit, bound = f64 / t_launch / 1e12, RF.FP64_PEAK_TFLOPS, "TFLOP/s", "mfma"
+    elif f32 / max(bytes_, 1) > RF.FP32_PEAK_TFLOPS * 1e3 / RF.HBM_PEAK_GBS:
+        ach, peak, unit, bound = f32 / t_launch / 1e12, RF.FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
+    else:
+        ach, peak, unit, bound = bytes_ / t_launch / 1e9, RF.HBM_PEAK_GBS, "GB/s", "hbm"
+    roof = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": unit, "frac": round(ach / peak, 6),
+            "traffic": None, "kernel": dom, "launch_ms": round(per_launch[dom], 5),
+            "algorithmic": {"bytes": bytes_, "flops_f32": f32, "flops_f64": f64}}
+
+    # accuracy against the FP64 oracle (window 0)
+    res = batch.read()
+    extra = {}
+    p0 = problems[0]
+    if not args.no_oracle:
+        from oracle import sba_oracle as O
+        t0 = time.time()
+        ref = O.solve(pristine[0], "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
+        extra["rel_cost_err_vs_fp64_oracle"] = abs(p0.last_cost_f64 - ref["last_cost"]) / ref["last_cost"]
+        q = pristine[0].copy()
+        q.points, q.rig_from_world = p0.points, p0.rig_from_world
+        c_eval, _ = O.cost(q, loss=ca.LOSS_HUBER)
+        extra["rel_cost_err_state_reevaluated"] = abs(c_eval - ref["last_cost"]) / ref["last_cost"]
+        nf = p0.rig_from_world.shape[0] - p0.num_fixed_key_frames
+        dt = np.abs(p0.rig_from_world[:nf, :3, 3] - ref["poses"][:nf, :3, 3]).max()
+        dR = p0.rig_from_world[:nf, :3, :3] @ np.transpose(ref["poses"][:nf, :3, :3], (0, 2, 1))
+        ang = np.degrees(np.arccos(np.clip((np.trace(dR, axis1=1, axis2=2) - 1) / 2, -1, 1))).max()
+        extra["pose_err_m"] = float(dt)
+        extra["pose_err_deg"] = float(ang)
+        extra["point_rms_vs_oracle_m"] = float(np.sqrt(((p0.points - ref["points"]) ** 2).mean()))
+        extra["oracle_iterations"] = ref["iterations"]
+        log(f"[bench] oracle FP64 solve {time.time() - t0:.1f}s: {extra}")
+    extra["iterations"] = int(p0.iterations)
+    extra["accepted"] = int(p0.accepted)
+    extra["final_cost"] = p0.last_cost_f64
+
+    if args.workload == "c3" and not args.no_plain:
+        # the plain form: the same windows solved one window per launch sequence, one after another
+        plain = [solver.batch([p]) for p in pristine]
+        for pb in plain[:4]:
+            pb.run()
+        plain[0].sync()
+        t0 = time.perf_counter()
+        for pb in plain:
+            pb.run()
+            pb.sync()
+        t_plain = time.perf_counter() - t0
+        extra["plain_solves_per_s"] = len(plain) / t_plain
+        extra["batched_over_plain"] = solves_per_s / extra["plain_solves_per_s"]
+        log(f"[bench] plain one-window-per-launch: {t_plain:.3f}s for {len(plain)} windows, "
+            f"{extra['plain_solves_per_s']:.1f} solves/s; batched/plain = {extra['batched_over_plain']:.1f}x")
+        for pb in plain:
+            pb.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import sba_oracle as O
+        sample = pristine[0]
+        t0 = time.perf_counter()
+        O.solve(sample, "f32", loss=ca.LOSS_HUBER, rcs_solver=1, dense_schur=1, fixed_iterations=1)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / t_cpu, 6), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"one full solve (10 LM iterations) of window 0 ({sample.num_observations} observations) "
+                         f"with the FP32 restatement of the reference CPU bundler, single thread, {cpu_model()}",
+               "seconds": round(t_cpu, 3)}
+        log(f"[bench] CPU baseline {t_cpu:.2f}s per solve")
+
+    line = {
+        "metric": "ms per BA solve (10 LM iters) + batched solves/s; rel. cost err vs FP64 oracle",
+        "value": round(solves_per_s, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "windows": len(problems), "observations": n_obs,
+                   "pairs": info["pairs"], "lm_iterations": 10, "chunks": info["chunks"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_ms_per_launch": {k: round(v, 5) for k, v in per_launch.items()},
+        "accuracy": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

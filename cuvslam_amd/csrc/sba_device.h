@@ -16,8 +16,10 @@
 //                            diag(V) (3), written by the Schur kernel for the back-substitution
 //   key frames      poses[2] 3 x float4 rows of [R | t] of rig_from_world, double-buffered
 //   cameras         cams     3 x float4 rows of [R | t] of camera_from_rig
-//   reduced system  s_acc    FP32 accumulator of the reduced camera system S (upper 6x6 blocks, packed
-//                            block-row-major), then e (6 per free key frame), then diag(U)
+//   reduced system  part     one record per chunk: its Schur products sum_i Z_i Z_i^T (Z = W G, G G^T the
+//                            damped point-block pseudo-inverse) as upper 16x16 tiles in matrix-core lane
+//                            order, then U (21 upper entries per free key frame), e (6), diag U (6)
+//                   sred     the window's records summed in chunk order (deterministic)
 //                   A        FP64 Cholesky workspace (packed lower triangle) for windows whose factor
 //                            does not fit in LDS
 //                   dc       FP32 pose step (6 per free key frame)
@@ -30,17 +32,24 @@ namespace sba_dev {
 
 constexpr int kBlock = 256;        // threads of the per-tile kernels
 constexpr int kTilePairs = 256;    // max (point, key frame) pairs per tile (one per thread)
-constexpr int kTilePoints = 256;   // max points per tile (one per thread)
-constexpr int kMaxFreePoses = 40;  // free key frames per window the shared-memory Schur partial supports
+constexpr int kTilePoints = 16;    // max points per tile: 48 Schur columns, 12 matrix-core k-steps
+constexpr int kMaxFreePoses = 40;  // free key frames per window (reduced system order <= 240)
 constexpr int kMaxPoses = 256;
 constexpr int kMaxCameras = 32;    // camera::Rig::kMaxCameras (libs/camera/rig.h:30)
+constexpr int kNT = 15;            // 16x16 tiles per side of the largest reduced system (240 / 16)
+constexpr int kZcLd = 240;         // leading dimension of the staged Z columns (bank-conflict free)
+constexpr int kKt = 48;            // Schur columns per tile (3 per point)
 constexpr int kFactorThreads = 512;
 constexpr int kPanel = 8;          // Cholesky panel width
 
 struct WindowDesc {
   int32_t np, npf, nc, ncf, ncam, no;
   int32_t point_off, pair_off, obs_off, pose_off, cam_off;
-  int32_t s_off;     // floats into s_acc
+  int32_t nt;        // 16x16 tiles per side of the reduced system, ceil(6 ncf / 16)
+  int32_t rec;       // floats of one Schur record: upper tiles (256 each), U (21 ncf), e (6 ncf), diag U (6 ncf)
+  int32_t nchunks;   // Schur records (chunks) of the window
+  int64_t part_off;  // floats into part: the window's first chunk record
+  int64_t sred_off;  // floats into sred: the window's reduced record
   int64_t a_off;     // doubles into A (only when the factor lives in global memory)
   int32_t dc_off;    // floats into dc
   int32_t max_iterations;
@@ -64,16 +73,19 @@ struct Tile {
 
 struct Chunk {
   int32_t tile_begin, tile_end;
+  int64_t part_off;  // floats into part: this chunk's record
 };
 
 struct Batch {
   int32_t n_windows, n_chunks, n_points, n_poses, n_cams;
-  int32_t loss, fixed_iterations, pad0;
-  int64_t s_acc_size;
+  int32_t loss, fixed_iterations, max_rec;
   const WindowDesc* win;
   WindowState* st;
-  const Tile* tiles;
+  const Tile* tiles;     // Schur tiles (<= kTilePairs pairs, <= kTilePoints points)
   const Chunk* chunks;
+  const Tile* btiles;    // back-substitution tiles (<= kTilePairs pairs, <= kBlock points)
+  const Chunk* bchunks;
+  int32_t n_bchunks, pad1;
   const float2* obs_xy;
   const float4* obs_info;
   const int32_t* pair_obs;
@@ -88,7 +100,8 @@ struct Batch {
   float4* poses1;
   const float4* poses_init;
   const float4* cams;
-  float* s_acc;
+  float* part;       // per-chunk Schur records
+  float* sred;       // per-window reduced records
   double* A;
   float* dc;
   double* dc64;
@@ -96,7 +109,7 @@ struct Batch {
   float4* out_poses;
   float4* dbg_dp;    // optional: the point step of the last back-substitution (phase-level tests)
   // shared-memory layout of the Schur kernel (offsets in 4-byte words)
-  int32_t lds_pose, lds_cam, lds_S, lds_e, lds_du, lds_acc, lds_Z, lds_pp, lds_pre, lds_words;
+  int32_t lds_pose, lds_cam, lds_u, lds_acc, lds_w, lds_pp, lds_zc, lds_words;
   int32_t factor_lds_words;  // LDS words of the factor kernel
 };
 

@@ -48,7 +48,6 @@ __device__ __forceinline__ double rho64(int kind, double s, double d) {
   return d * sqrt(s) - 0.5 * d * d;
 }
 
-__device__ __forceinline__ float4 ld_pt(const float4* p, int i) { return p[i]; }
 
 // A pose in shared memory: 12 floats, rows of [R | t].
 struct Lin {
@@ -159,24 +158,10 @@ __device__ __forceinline__ void point_factor(const float V[6], float lambda, flo
     for (int k = 0; k < 3; ++k) G[i][k] = v[i][k] * sc[k];
 }
 
-__device__ __forceinline__ int upper_block(int j, int k, int ncf) {  // j <= k < ncf
-  return j * ncf - (j * (j - 1)) / 2 + (k - j);
-}
-
 template <class T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
-
-__device__ __forceinline__ int wave_incl_scan(int x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
   return x;
 }
 
@@ -196,69 +181,118 @@ __device__ __forceinline__ void load_pose_lds(float* dst, const float4* src, int
 
 // ---------------------------------------------------------------------------------------------------------
 // Linearize + Schur complement. One workgroup per chunk (consecutive tiles of one window's points).
+//
+// The Schur products S -= sum_i W_i V_i*^-1 W_i^T are written as S -= sum_i Z_i Z_i^T with Z_i = W_i G_i and
+// G_i G_i^T the damped point-block pseudo-inverse: a symmetric rank-3 update per point. Over the points of a
+// tile the Z blocks are staged in shared memory as dense columns (6 rows per free key frame, 3 columns per
+// point, zero where a key frame does not see the point), and the chunk's sum Z Z^T accumulates in
+// registers on the FP32 matrix cores (v_mfma_f32_16x16x4_f32) over the upper 16x16 tiles of S. Each wave owns
+// a fixed quarter of the tiles. Nothing is atomically added in global memory: a chunk writes its record once
+// and k_reduce sums the records of a window in chunk order.
 // ---------------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
-  extern __shared__ float sm[];
-  const Chunk ch = b.chunks[blockIdx.x];
-  const int w = b.tiles[ch.tile_begin].window;
-  const WindowDesc W = b.win[w];
-  const WindowState* St = b.st + w;
-  if (St->done) return;
-  const int cur = St->cur;
-  const float lam = St->lambda;
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTilesPerWave = (kNT * (kNT + 1) / 2 + 3) / 4;  // 30
+
+// Upper tile T (row-major over the kNT x kNT upper triangle) -> (I, J).
+constexpr int tile_i(int T) {
+  int I = 0;
+  while (T >= kNT - I) {
+    T -= kNT - I;
+    ++I;
+  }
+  return I;
+}
+constexpr int tile_j(int T) {
+  int I = 0;
+  while (T >= kNT - I) {
+    T -= kNT - I;
+    ++I;
+  }
+  return I + T;
+}
+
+__device__ __forceinline__ int upper_tile(int I, int J, int nt) { return I * nt - (I * (I - 1)) / 2 + (J - I); }
+
+// The matrix-core part of one tile: acc(I, J) += Zc[16 I.., k] Zc[16 J.., k]^T over its k-steps, for the
+// tiles of wave WAVE (T = WAVE, WAVE + 4, ...). Lane layout of v_mfma_f32_16x16x4_f32: A lane l holds
+// A[l % 16][l / 16], B lane l holds B[l / 16][l % 16], D lane l holds D[4 (l / 16) + v][l % 16].
+template <int WAVE>
+__device__ __forceinline__ void mfma_tile(const float* sZc, int nslab, int nt, int lane, floatx4 (&acc)[kTilesPerWave]) {
+  for (int s = 0; s < nslab; ++s) {
+    const float* col = sZc + (4 * s + (lane >> 4)) * kZcLd + (lane & 15);
+    float f[kNT];
+#pragma unroll
+    for (int I = 0; I < kNT; ++I) f[I] = I < nt ? col[16 * I] : 0.f;
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+      const int T = WAVE + 4 * t;
+      if (T < kNT * (kNT + 1) / 2) {
+        const int I = tile_i(T), J = tile_j(T);
+        if (J < nt) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[I], f[J], acc[t], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int WAVE>
+__device__ __forceinline__ void store_tiles(float* rec, int nt, int lane, const floatx4 (&acc)[kTilesPerWave]) {
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const int T = WAVE + 4 * t;
+    if (T < kNT * (kNT + 1) / 2) {
+      const int I = tile_i(T), J = tile_j(T);
+      if (J < nt) {
+        float* dst = rec + 256 * upper_tile(I, J, nt);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dst[64 * v + lane] = acc[t][v];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, const WindowDesc& W, float lam, int cur,
+                                            float* sm) {
   const float4* __restrict__ pts = cur ? b.pts1 : b.pts0;
-  const float4* __restrict__ poses = cur ? b.poses1 : b.poses0;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int ncf = W.ncf;
-  const int nb = ncf * (ncf + 1) / 2;
+  const int nt = W.nt;
   float* sPose = sm + b.lds_pose;
   float* sCam = sm + b.lds_cam;
-  float* sS = sm + b.lds_S;
-  float* sE = sm + b.lds_e;
-  float* sDu = sm + b.lds_du;
-  float* sAcc = sm + b.lds_acc;  // 12 per tile point: V (6), e_b (3) then G (9) and G^T e_b (3)
-  float* sZ = sm + b.lds_Z;      // 18 per tile pair: W_ij (6x3) then Z_ij = W_ij G_i
+  float* sU = sm + b.lds_u;  // U (21 per free key frame), e (6), diag U (6)
+  float* sE = sU + 21 * ncf;
+  float* sDu = sE + 6 * ncf;
+  float* sAcc = sm + b.lds_acc;  // 12 per tile point: V (6), e_b (3), then G (9) and G^T e_b (3)
+  float* sW = sm + b.lds_w;      // 18 per tile pair: W_ij (6x3)
   int* sPP = reinterpret_cast<int*>(sm + b.lds_pp);
-  int* sPre = reinterpret_cast<int*>(sm + b.lds_pre);
-  __shared__ int sWave[kBlock / 64];
-
-  load_pose_lds(sPose, poses + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
-  load_pose_lds(sCam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
-  for (int k = tid; k < 36 * nb; k += kBlock) sS[k] = 0.f;
-  for (int k = tid; k < 6 * ncf; k += kBlock) {
-    sE[k] = 0.f;
-    sDu[k] = 0.f;
-  }
+  float* sZc = sm + b.lds_zc;    // Z columns: [k][row], k = 3 * tile point + c, row = 6 * key frame + r
   const int kind = b.loss;
   const float delta = W.robust;
-  const int pf_end = W.point_off + W.npf;  // global index one past the last free point
+  const int pf_end = W.point_off + W.npf;
+
+  floatx4 acc[kTilesPerWave];
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
     const Tile T = b.tiles[t];
     const int p0 = b.point_pair[T.point_begin];
     const int npair = b.point_pair[T.point_end] - p0;
     const int npts = T.point_end - T.point_begin;
-    __syncthreads();  // previous tile's shared arrays are free; pose/S init visible
-    // (a) pair -> tile point map, zero the point accumulators, pair-pair counts
-    int q = 0;
+    const int kpad = (3 * npts + 3) & ~3;
+    __syncthreads();  // the previous tile's shared arrays are free
+    // (a) pair -> tile point map, zero the point accumulators and the staged columns
     if (tid < npts) {
       const int i = T.point_begin + tid;
       const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
       for (int k = a; k < e; ++k) sPP[k] = tid;
 #pragma unroll
       for (int k = 0; k < 12; ++k) sAcc[12 * tid + k] = 0.f;
-      q = (i < pf_end) ? b.point_nfree[i] : 0;
     }
-    {
-      const int cnt = q * (q + 1) / 2;
-      const int incl = wave_incl_scan(cnt);
-      if ((tid & 63) == 63) sWave[tid >> 6] = incl;
-      __syncthreads();
-      int base = 0;
-      for (int k = 0; k < (tid >> 6); ++k) base += sWave[k];
-      if (tid < npts) sPre[tid + 1] = base + incl;
-      if (tid == 0) sPre[0] = 0;
-    }
+    for (int k = tid; k < kpad * kZcLd; k += kBlock) sZc[k] = 0.f;
     __syncthreads();
     // (b) one thread per pair: linearize its observations (one per camera) and accumulate
     if (tid < npair) {
@@ -268,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
       const int j = b.pair_pose[pair];
       const bool point_free = i < pf_end;
       const bool pose_free = j < ncf;
-      const float4 p = ld_pt(pts, i);
+      const float4 p = pts[i];
       float Wm[6][3] = {}, U[21] = {}, ea[6] = {}, V[6] = {}, eb[3] = {};
       const int o_end = b.pair_obs[pair + 1];
       for (int o = b.pair_obs[pair]; o < o_end; ++o) {
@@ -278,7 +312,6 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
         Lin L;
         if (!linearize(sPose + 12 * j, sCam + 12 * cam, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L))
           continue;
-        // Lambda * J and Lambda * r, weighted
         float LJc[2][6], LJp[2][3];
         const float wr0 = L.w * (inf.x * L.r0 + inf.y * L.r1);
         const float wr1 = L.w * (inf.y * L.r0 + inf.z * L.r1);
@@ -319,19 +352,15 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
         }
       }
       if (point_free) {
-        float* acc = sAcc + 12 * tp;
+        float* a = sAcc + 12 * tp;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) lds_add(acc + k, V[k]);
+        for (int k = 0; k < 6; ++k) lds_add(a + k, V[k]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) lds_add(acc + 6 + k, eb[k]);
+        for (int k = 0; k < 3; ++k) lds_add(a + 6 + k, eb[k]);
       }
       if (pose_free) {
-        float* blk = sS + 36 * upper_block(j, j, ncf);
-        int u = 0;
 #pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = r; c < 6; ++c) lds_add(blk + 6 * r + c, U[u++]);
+        for (int u = 0; u < 21; ++u) lds_add(sU + 21 * j + u, U[u]);
         int d = 0;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
           d += 6 - r;
         }
         if (point_free) {
-          float* z = sZ + 18 * tid;
+          float* z = sW + 18 * tid;
 #pragma unroll
           for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -353,9 +382,9 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
     if (tid < npts) {
       const int i = T.point_begin + tid;
       if (i < pf_end) {
-        float* acc = sAcc + 12 * tid;
-        const float V[6] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]};
-        const float eb[3] = {acc[6], acc[7], acc[8]};
+        float* a = sAcc + 12 * tid;
+        const float V[6] = {a[0], a[1], a[2], a[3], a[4], a[5]};
+        const float eb[3] = {a[6], a[7], a[8]};
         float G[3][3];
         point_factor(V, lam, G);
         float tv[3], Vi[6], gb[3];
@@ -376,90 +405,93 @@ __global__ __launch_bounds__(kBlock) void k_schur(Batch b) {
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-          for (int c = 0; c < 3; ++c) acc[3 * r + c] = G[r][c];
-        acc[9] = tv[0];
-        acc[10] = tv[1];
-        acc[11] = tv[2];
+          for (int c = 0; c < 3; ++c) a[3 * r + c] = G[r][c];
+        a[9] = tv[0];
+        a[10] = tv[1];
+        a[11] = tv[2];
       }
     }
     __syncthreads();
-    // (d) one thread per (free point, free key frame) pair: Z = W G, e_j -= Z G^T e_b
+    // (d) one thread per (free point, free key frame) pair: Z = W G into the staged columns, e_j -= Z G^T e_b
     if (tid < npair) {
       const int pair = p0 + tid;
       const int tp = sPP[tid];
       const int i = T.point_begin + tp;
       const int j = b.pair_pose[pair];
       if (i < pf_end && j < ncf) {
-        const float* acc = sAcc + 12 * tp;
-        float* z = sZ + 18 * tid;
-        float Wm[6][3];
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) Wm[r][c] = z[3 * r + c];
+        const float* a = sAcc + 12 * tp;
+        const float* z = sW + 18 * tid;
+        float* zc = sZc + 3 * tp * kZcLd + 6 * j;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           float zr[3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) zr[c] = Wm[r][0] * acc[c] + Wm[r][1] * acc[3 + c] + Wm[r][2] * acc[6 + c];
+          for (int c = 0; c < 3; ++c) zr[c] = z[3 * r] * a[c] + z[3 * r + 1] * a[3 + c] + z[3 * r + 2] * a[6 + c];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) z[3 * r + c] = zr[c];
-          lds_add(sE + 6 * j + r, -(zr[0] * acc[9] + zr[1] * acc[10] + zr[2] * acc[11]));
+          for (int c = 0; c < 3; ++c) zc[c * kZcLd + r] = zr[c];
+          lds_add(sE + 6 * j + r, -(zr[0] * a[9] + zr[1] * a[10] + zr[2] * a[11]));
         }
       }
     }
     __syncthreads();
-    // (e) Schur pair products S_{j_a j_b} -= Z_a Z_b^T over each point's free pairs a <= b
-    const int total = sPre[npts];
-    for (int uidx = tid; uidx < total; uidx += kBlock) {
-      int lo = 0, hi = npts;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (sPre[mid] <= uidx) lo = mid; else hi = mid;
-      }
-      const int l = uidx - sPre[lo];
-      int bb = static_cast<int>((sqrtf(8.f * l + 1.f) - 1.f) * 0.5f);
-      while ((bb + 1) * (bb + 2) / 2 <= l) ++bb;
-      while (bb * (bb + 1) / 2 > l) --bb;
-      const int aa = l - bb * (bb + 1) / 2;
-      const int pbase = b.point_pair[T.point_begin + lo] - p0;
-      const int ka = pbase + aa, kb = pbase + bb;
-      const int ja = b.pair_pose[p0 + ka], jb = b.pair_pose[p0 + kb];
-      const float* za = sZ + 18 * ka;
-      const float* zb = sZ + 18 * kb;
-      float A[6][3], B[6][3];
-#pragma unroll
-      for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          A[r][c] = za[3 * r + c];
-          B[r][c] = zb[3 * r + c];
-        }
-      float* blk = sS + 36 * upper_block(ja, jb, ncf);
-      if (aa == bb) {
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = r; c < 6; ++c) lds_add(blk + 6 * r + c, -(A[r][0] * B[c][0] + A[r][1] * B[c][1] + A[r][2] * B[c][2]));
-      } else {
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = 0; c < 6; ++c) lds_add(blk + 6 * r + c, -(A[r][0] * B[c][0] + A[r][1] * B[c][1] + A[r][2] * B[c][2]));
-      }
+    // (e) matrix cores: acc += Zc Zc^T over the tile's k-steps (only this loop is specialised per wave)
+    switch (tid >> 6) {
+      case 0: mfma_tile<0>(sZc, kpad / 4, nt, lane, acc); break;
+      case 1: mfma_tile<1>(sZc, kpad / 4, nt, lane, acc); break;
+      case 2: mfma_tile<2>(sZc, kpad / 4, nt, lane, acc); break;
+      default: mfma_tile<3>(sZc, kpad / 4, nt, lane, acc); break;
     }
   }
-  __syncthreads();
-  // flush the chunk's partial reduced system into the window's accumulator
-  float* g = b.s_acc + W.s_off;
-  for (int k = tid; k < 36 * nb; k += kBlock) {
-    const float v = sS[k];
-    if (v != 0.f) unsafeAtomicAdd(g + k, v);
+  // the chunk's record: Schur tiles (lane order), then U, e, diag U
+  float* rec = b.part + ch.part_off;
+  switch (tid >> 6) {
+    case 0: store_tiles<0>(rec, nt, lane, acc); break;
+    case 1: store_tiles<1>(rec, nt, lane, acc); break;
+    case 2: store_tiles<2>(rec, nt, lane, acc); break;
+    default: store_tiles<3>(rec, nt, lane, acc); break;
   }
-  for (int k = tid; k < 6 * ncf; k += kBlock) {
-    const float ve = sE[k], vd = sDu[k];
-    if (ve != 0.f) unsafeAtomicAdd(g + 36 * nb + k, ve);
-    if (vd != 0.f) unsafeAtomicAdd(g + 36 * nb + 6 * ncf + k, vd);
+  __syncthreads();
+  const int ntu = nt * (nt + 1) / 2;
+  for (int k = tid; k < 33 * ncf; k += kBlock) rec[256 * ntu + k] = sU[k];
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
+  extern __shared__ float sm[];
+  const Chunk ch = b.chunks[blockIdx.x];
+  const int w = b.tiles[ch.tile_begin].window;
+  const WindowDesc W = b.win[w];
+  const WindowState* St = b.st + w;
+  if (St->done) return;
+  const int cur = St->cur;
+  const float lam = St->lambda;
+  const int tid = threadIdx.x;
+  load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+  load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
+  for (int k = tid; k < 33 * W.ncf; k += kBlock) sm[b.lds_u + k] = 0.f;
+  schur_chunk(b, ch, W, lam, cur, sm);
+}
+
+// Sums the chunk records of each window in chunk order (deterministic; no atomics).
+__global__ __launch_bounds__(64) void k_reduce(Batch b) {
+  const int w = blockIdx.y;
+  const WindowDesc W = b.win[w];
+  if (b.st[w].done) return;
+  const float* src = b.part + W.part_off;
+  float* dst = b.sred + W.sred_off;
+  const int64_t rec = W.rec;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W.rec; e += gridDim.x * blockDim.x) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int c = 0;
+    for (; c + 4 <= W.nchunks; c += 4) {
+      s0 += src[(c + 0) * rec + e];
+      s1 += src[(c + 1) * rec + e];
+      s2 += src[(c + 2) * rec + e];
+      s3 += src[(c + 3) * rec + e];
+    }
+    for (; c < W.nchunks; ++c) s0 += src[c * rec + e];
+    dst[e] = (s0 + s1) + (s2 + s3);
   }
 }
 
@@ -534,28 +566,33 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   double* red = panel + 256 * kPanel;  // 32: reductions
   int* flag = reinterpret_cast<int*>(red + 32);
   double* Lp = W.factor_in_lds ? (red + 48) : (b.A + W.a_off);
-  float* g = b.s_acc + W.s_off;
+  const int nt = W.nt;
+  const float* R = b.sred + W.sred_off;  // sum Z Z^T tiles, then U, e, diag U
+  const float* RU = R + 256 * (nt * (nt + 1) / 2);
+  const float* RE = RU + 21 * ncf;
+  const float* RD = RE + 6 * ncf;
+  (void)nb;
 
-  // 1. assemble A = S + lambda diag(U) (packed lower) and e; clear the accumulator for the next iteration
+  // 1. assemble A = U - sum Z Z^T + lambda diag(U) (packed lower) and e
   const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
   for (int64_t idx = tid; idx < npk; idx += kFactorThreads) {
     int i = static_cast<int>((sqrt(8.0 * static_cast<double>(idx) + 1.0) - 1.0) * 0.5);
     while (pk(i + 1, 0) <= idx) ++i;
     while (pk(i, 0) > idx) --i;
     const int j = static_cast<int>(idx - pk(i, 0));
-    const int bi = i / 6, ri = i - 6 * bi, bj = j / 6, rj = j - 6 * bj;
-    float* src = g + 36 * upper_block(bj, bi, ncf) + 6 * rj + ri;
-    double v = static_cast<double>(*src);
-    *src = 0.f;
-    if (i == j) v += static_cast<double>(lam) * static_cast<double>(g[36 * nb + 6 * ncf + i]);
+    // S is symmetric: entry (j, i), j <= i, of upper tile (j / 16, i / 16), in matrix-core lane order
+    const int r = j & 15, c = i & 15;
+    double v = -static_cast<double>(R[256 * upper_tile(j >> 4, i >> 4, nt) + 64 * (r & 3) + 16 * (r >> 2) + c]);
+    if (i / 6 == j / 6) {
+      const int pose = i / 6, rr = j - 6 * pose, cc = i - 6 * pose;
+      v += static_cast<double>(RU[21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)]);
+    }
+    if (i == j) v += static_cast<double>(lam) * static_cast<double>(RD[i]);
     Lp[idx] = v;
   }
-  __syncthreads();  // the diag(U) reads above happen before the clears below
   for (int i = tid; i < n; i += kFactorThreads) {
-    vec[i] = static_cast<double>(g[36 * nb + i]);
-    du[i] = static_cast<double>(g[36 * nb + 6 * ncf + i]);
-    g[36 * nb + i] = 0.f;
-    g[36 * nb + 6 * ncf + i] = 0.f;
+    vec[i] = static_cast<double>(RE[i]);
+    du[i] = static_cast<double>(RD[i]);
   }
   if (tid == 0) flag[0] = 0;
   __syncthreads();
@@ -621,7 +658,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
 #pragma unroll
           for (int c = 0; c < kPanel; ++c) {
             if (c < kb && row >= k0 + c) Lp[pk(row, k0 + c)] = p[s][c];
-            panel[(row - k0) * kPanel + c] = c < kb ? p[s][c] : 0.0;
+            panel[c * 256 + (row - k0)] = c < kb ? p[s][c] : 0.0;
           }
           vec[row] = rhs[s];
         }
@@ -630,40 +667,29 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
     }
     __syncthreads();
     if (flag[0]) break;
-    // trailing update of rows/columns >= k0 + kb in 4x4 tiles of the lower triangle
+    // trailing update of rows/columns >= k0 + kb: a wave takes a 64-row strip and 4 columns; the strip's
+    // panel values are read lane-consecutive, the 4 columns' values are broadcasts (no bank conflicts)
     const int t0 = k0 + kb;
     const int tn = n - t0;
     if (tn > 0) {
-      const int T4 = (tn + 3) >> 2;
-      const int ntiles = T4 * (T4 + 1) / 2;
-      for (int tile = tid; tile < ntiles; tile += kFactorThreads) {
-        int I = static_cast<int>((sqrtf(8.f * tile + 1.f) - 1.f) * 0.5f);
-        while ((I + 1) * (I + 2) / 2 <= tile) ++I;
-        while (I * (I + 1) / 2 > tile) --I;
-        const int J = tile - I * (I + 1) / 2;
-        double li[4][kPanel], lj[4][kPanel];
+      const int nstrip = (tn + 63) >> 6;
+      const int ngroup = (tn + 3) >> 2;
+      for (int item = warp; item < nstrip * ngroup; item += kFactorThreads / 64) {
+        const int strip = item / ngroup, grp = item - strip * ngroup;
+        const int i = t0 + 64 * strip + lane;
+        const int j0 = t0 + 4 * grp;
+        if (j0 > t0 + 64 * strip + 63) continue;  // the whole 64x4 block is above the diagonal
+        double li[kPanel];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int ri = t0 + 4 * I + a - k0, rj = t0 + 4 * J + a - k0;
+        for (int c = 0; c < kPanel; ++c) li[c] = i < n ? panel[c * 256 + (i - k0)] : 0.0;
 #pragma unroll
-          for (int c = 0; c < kPanel; ++c) {
-            li[a][c] = (t0 + 4 * I + a < n) ? panel[ri * kPanel + c] : 0.0;
-            lj[a][c] = (t0 + 4 * J + a < n) ? panel[rj * kPanel + c] : 0.0;
-          }
-        }
+        for (int jj = 0; jj < 4; ++jj) {
+          const int j = j0 + jj;
+          if (j >= n) break;
+          double s2 = 0.0;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = t0 + 4 * I + a;
-          if (i >= n) continue;
-#pragma unroll
-          for (int c2 = 0; c2 < 4; ++c2) {
-            const int j = t0 + 4 * J + c2;
-            if (j > i) continue;
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < kPanel; ++c) s += li[a][c] * lj[c2][c];
-            Lp[pk(i, j)] -= s;
-          }
+          for (int c = 0; c < kPanel; ++c) s2 += li[c] * panel[c * 256 + (j - k0)];
+          if (i < n && j <= i) Lp[pk(i, j)] -= s2;
         }
       }
     }
@@ -764,8 +790,8 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];
   __shared__ float sRedM[kBlock / 64];
-  const Chunk ch = b.chunks[blockIdx.x];
-  const int w = b.tiles[ch.tile_begin].window;
+  const Chunk ch = b.bchunks[blockIdx.x];
+  const int w = b.btiles[ch.tile_begin].window;
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done) return;
@@ -790,7 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   float maxdp = 0.f;
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
-    const Tile T = b.tiles[t];
+    const Tile T = b.btiles[t];
     const int p0 = b.point_pair[T.point_begin];
     const int npair = b.point_pair[T.point_end] - p0;
     const int npts = T.point_end - T.point_begin;
@@ -1021,7 +1047,6 @@ __global__ void k_reset(Batch b) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = gid; i < b.n_points; i += stride) b.pts0[i] = b.pts_init[i];
   for (int64_t i = gid; i < 3 * static_cast<int64_t>(b.n_poses); i += stride) b.poses0[i] = b.poses_init[i];
-  for (int64_t i = gid; i < b.s_acc_size; i += stride) b.s_acc[i] = 0.f;
   for (int64_t i = gid; i < b.n_windows; i += stride) {
     WindowState s = {};
     s.lambda = 0.001f;
@@ -1042,6 +1067,24 @@ __global__ void k_gather(Batch b) {
     b.out_poses[3 * W.pose_off + i] = poses[3 * W.pose_off + i];
 }
 
+// One v_mfma_f32_16x16x4_f32 on fixed inputs, to check the lane layout the Schur kernel relies on:
+// A (16x4) lane l holds A[l % 16][l / 16]; B (4x16) lane l holds B[l / 16][l % 16];
+// D (16x16) lane l holds D[4 (l / 16) + v][l % 16] in its v-th value.
+__global__ void k_mfma_layout(const float* A, const float* B, float* D) {
+  const int l = threadIdx.x;
+  const float a = A[(l % 16) * 4 + l / 16];
+  const float b = B[(l / 16) * 16 + l % 16];
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) D[(4 * (l / 16) + v) * 16 + l % 16] = acc[v];
+}
+
+// A plain streaming copy of a known size, the calibration of HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_calibrate(float4* __restrict__ dst, const float4* __restrict__ src, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Launch wrappers (host side)
 // ---------------------------------------------------------------------------------------------------------
@@ -1057,6 +1100,12 @@ hipError_t launch_schur(const Batch& b, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_reduce(const Batch& b, hipStream_t s) {
+  const int gx = (b.max_rec + 63) / 64;
+  hipLaunchKernelGGL(k_reduce, dim3(gx > 0 ? gx : 1, b.n_windows), dim3(64), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s) {
   hipLaunchKernelGGL(k_factor, dim3(b.n_windows), dim3(kFactorThreads), static_cast<size_t>(b.factor_lds_words) * 4, s,
                      b, dbg_S);
@@ -1065,9 +1114,9 @@ hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s) {
 
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s) {
   if (initial)
-    hipLaunchKernelGGL(k_backsub<true>, dim3(b.n_chunks), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_backsub<true>, dim3(b.n_bchunks), dim3(kBlock), 0, s, b);
   else
-    hipLaunchKernelGGL(k_backsub<false>, dim3(b.n_chunks), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_backsub<false>, dim3(b.n_bchunks), dim3(kBlock), 0, s, b);
   return hipGetLastError();
 }
 
@@ -1087,6 +1136,16 @@ hipError_t launch_gather(const Batch& b, hipStream_t s) {
 
 // Dynamic shared memory up to the 160 KiB of a gfx950 CU needs no attribute on ROCm; this only clears the
 // runtime's last-error slot so that a stale error is not reported by the first launch of a handle.
+hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_layout(const float* A, const float* B, float* D, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_layout, dim3(1), dim3(64), 0, s, A, B, D);
+  return hipGetLastError();
+}
+
 hipError_t set_kernel_attributes() {
   (void)hipGetLastError();
   return hipSuccess;

@@ -22,10 +22,13 @@
 namespace sba_dev {
 hipError_t launch_reset(const Batch& b, hipStream_t s);
 hipError_t launch_schur(const Batch& b, hipStream_t s);
+hipError_t launch_reduce(const Batch& b, hipStream_t s);
 hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s);
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_gather(const Batch& b, hipStream_t s);
+hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s);
+hipError_t launch_mfma_layout(const float* A, const float* B, float* D, hipStream_t s);
 hipError_t set_kernel_attributes();
 }  // namespace sba_dev
 
@@ -50,8 +53,8 @@ struct sba_batch {
   hipGraphExec_t gexec = nullptr;
   double* dbg_S = nullptr;  // debug buffer of the reduced system of window 0
   hipEvent_t ev[8] = {};
-  float class_ms[4] = {0, 0, 0, 0};
-  int class_launches[4] = {0, 0, 0, 0};
+  float class_ms[5] = {0, 0, 0, 0, 0};
+  int class_launches[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -72,7 +75,7 @@ struct HostPack {
   std::vector<float4> obs_info;
   std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree;
   std::vector<float4> pts, poses, cams;
-  std::vector<Tile> tiles;
+  std::vector<Tile> tiles, btiles;
 };
 
 // Validates one window and appends it to the packed streams.
@@ -154,20 +157,24 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   };
   for (int j = 0; j < nc; ++j) push_pose(H.poses, p.rig_from_world + 16 * j);
   for (int c = 0; c < ncam; ++c) push_pose(H.cams, p.camera_from_rig + 16 * c);
-  // tiles: consecutive points with at most kTilePairs pairs and kTilePoints points
-  int i = 0;
-  while (i < np) {
-    const int begin = i;
-    int pairs = 0;
-    while (i < np && i - begin < kTilePoints) {
-      const int pi = static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
-                                      H.point_pair[W.point_off + i]);
-      if (pairs + pi > kTilePairs && i > begin) break;
-      pairs += pi;
-      ++i;
+  // tiles: consecutive points with at most kTilePairs pairs and max_points points
+  auto make_tiles = [&](std::vector<Tile>& out, int max_points) {
+    int i = 0;
+    while (i < np) {
+      const int begin = i;
+      int pairs = 0;
+      while (i < np && i - begin < max_points) {
+        const int pi = static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
+                                        H.point_pair[W.point_off + i]);
+        if (pairs + pi > kTilePairs && i > begin) break;
+        pairs += pi;
+        ++i;
+      }
+      out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pairs});
     }
-    H.tiles.push_back(Tile{w, W.point_off + begin, W.point_off + i, pairs});
-  }
+  };
+  make_tiles(H.tiles, kTilePoints);
+  make_tiles(H.btiles, kBlock);
   return SBA_OK;
 }
 
@@ -214,15 +221,17 @@ int enqueue_solve(sba_handle* h, sba_batch* b, bool timed) {
     if (timed) HIP_TRY(hipEventRecord(b->ev[0], s));
     HIP_TRY(launch_schur(d, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[1], s));
-    HIP_TRY(launch_factor(d, nullptr, s));
+    HIP_TRY(launch_reduce(d, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[2], s));
-    HIP_TRY(launch_backsub(d, false, s));
+    HIP_TRY(launch_factor(d, nullptr, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[3], s));
+    HIP_TRY(launch_backsub(d, false, s));
+    if (timed) HIP_TRY(hipEventRecord(b->ev[4], s));
     HIP_TRY(launch_decide(d, false, s));
     if (timed) {
-      HIP_TRY(hipEventRecord(b->ev[4], s));
-      HIP_TRY(hipEventSynchronize(b->ev[4]));
-      for (int k = 0; k < 4; ++k) {
+      HIP_TRY(hipEventRecord(b->ev[5], s));
+      HIP_TRY(hipEventSynchronize(b->ev[5]));
+      for (int k = 0; k < 5; ++k) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
         b->class_ms[k] += ms;
@@ -297,8 +306,8 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   b->n = n;
   b->win.resize(n);
   b->status_pre.assign(n, SBA_OK);
-  int64_t s_off = 0, a_off = 0, dc_off = 0;
-  int max_nc = 1, max_ncam = 1, max_nb = 1, max_ncf = 1, max_lds_packed = 0;
+  int64_t sred_off = 0, a_off = 0, dc_off = 0;
+  int max_nc = 1, max_ncam = 1, max_ncf = 1, max_lds_packed = 0, max_rec = 0;
   for (int w = 0; w < n; ++w) {
     const sba_problem_view& p = problems[w];
     const int rc = pack_window(h, p, w, H, b->win[w]);
@@ -308,9 +317,11 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
     WindowDesc& W = b->win[w];
     if (W.np == 0 || W.nc == 0 || W.no == 0) b->status_pre[w] = SBA_E_EMPTY;
-    const int nb = W.ncf * (W.ncf + 1) / 2;
-    W.s_off = static_cast<int32_t>(s_off);
-    s_off += 36 * nb + 12 * W.ncf;
+    W.nt = (6 * W.ncf + 15) / 16;
+    W.rec = 256 * (W.nt * (W.nt + 1) / 2) + 33 * W.ncf;
+    W.sred_off = sred_off;
+    sred_off += W.rec;
+    max_rec = std::max(max_rec, W.rec);
     W.dc_off = static_cast<int32_t>(dc_off);
     dc_off += 6 * W.ncf;
     const int nn = 6 * W.ncf;
@@ -326,7 +337,6 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
     max_nc = std::max(max_nc, W.nc);
     max_ncam = std::max(max_ncam, W.ncam);
-    max_nb = std::max(max_nb, nb);
     max_ncf = std::max(max_ncf, W.ncf);
     b->max_iter = std::max(b->max_iter, p.max_iterations);
     // an empty window never runs: the initial decision sees zero observations as infeasible, so mark it here
@@ -337,34 +347,61 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
   H.pair_obs.push_back(static_cast<int32_t>(H.obs_xy.size()));
 
-  // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each
+  // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each; each owns one record
   int64_t target = h->opt.chunk_pairs;
-  if (target <= 0) target = std::min<int64_t>(16384, std::max<int64_t>(1024, b->total_pairs / 1024));
+  if (target <= 0) target = std::max<int64_t>(1024, b->total_pairs / 512);
   std::vector<Chunk> chunks;
+  int64_t part_off = 0;
   {
     size_t t = 0;
+    int last_w = -1;
     while (t < H.tiles.size()) {
       const int w = H.tiles[t].window;
+      WindowDesc& W = b->win[w];
+      if (w != last_w) {
+        W.part_off = part_off;
+        W.nchunks = 0;
+        last_w = w;
+      }
       const size_t begin = t;
       int64_t pairs = 0;
       while (t < H.tiles.size() && H.tiles[t].window == w && (t == begin || pairs < target)) {
         pairs += H.tiles[t].pad;
         ++t;
       }
-      chunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t)});
+      chunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), part_off});
+      part_off += W.rec;
+      W.nchunks += 1;
+    }
+  }
+  // back-substitution chunks: the same grouping over the larger back-substitution tiles
+  std::vector<Chunk> bchunks;
+  {
+    size_t t = 0;
+    while (t < H.btiles.size()) {
+      const int w = H.btiles[t].window;
+      const size_t begin = t;
+      int64_t pairs = 0;
+      while (t < H.btiles.size() && H.btiles[t].window == w && (t == begin || pairs < target)) {
+        pairs += H.btiles[t].pad;
+        ++t;
+      }
+      bchunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), 0});
     }
   }
   for (auto& T : H.tiles) T.pad = 0;
+  for (auto& T : H.btiles) T.pad = 0;
 
   Batch& d = b->dev;
   d.n_windows = n;
   d.n_chunks = static_cast<int32_t>(chunks.size());
+  d.n_bchunks = static_cast<int32_t>(bchunks.size());
   d.n_points = static_cast<int32_t>(H.pts.size());
   d.n_poses = static_cast<int32_t>(H.poses.size() / 3);
   d.n_cams = static_cast<int32_t>(H.cams.size() / 3);
   d.loss = h->opt.loss;
   d.fixed_iterations = h->opt.fixed_iterations;
-  d.s_acc_size = s_off;
+  d.max_rec = max_rec;
   int rc = SBA_OK;
 #define UP(vec, field)                                                                   \
   if ((rc = upload(h, b, vec, const_cast<std::remove_const_t<std::remove_pointer_t<decltype(d.field)>>**>(&d.field))) != SBA_OK) { \
@@ -374,6 +411,8 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(b->win, win);
   UP(H.tiles, tiles);
   UP(chunks, chunks);
+  UP(H.btiles, btiles);
+  UP(bchunks, bchunks);
   UP(H.obs_xy, obs_xy);
   UP(H.obs_info, obs_info);
   UP(H.pair_obs, pair_obs);
@@ -395,7 +434,8 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   AL(3 * H.pts.size(), paux);
   AL(H.poses.size(), poses0);
   AL(H.poses.size(), poses1);
-  AL(static_cast<size_t>(s_off), s_acc);
+  AL(static_cast<size_t>(part_off), part);
+  AL(static_cast<size_t>(sred_off), sred);
   AL(static_cast<size_t>(a_off), A);
   AL(static_cast<size_t>(dc_off), dc);
   AL(static_cast<size_t>(dc_off), dc64);
@@ -409,20 +449,17 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   off += 12 * max_nc;
   d.lds_cam = off;
   off += 12 * max_ncam;
-  d.lds_S = off;
-  off += 36 * max_nb;
-  d.lds_e = off;
-  off += 6 * max_ncf;
-  d.lds_du = off;
-  off += 6 * max_ncf;
+  d.lds_u = off;
+  off += 33 * max_ncf;
   d.lds_acc = off;
   off += 12 * kTilePoints;
-  d.lds_Z = off;
+  d.lds_w = off;
   off += 18 * kTilePairs;
   d.lds_pp = off;
   off += kTilePairs;
-  d.lds_pre = off;
-  off += kTilePoints + 1;
+  off = (off + 63) & ~63;
+  d.lds_zc = off;
+  off += kKt * kZcLd;
   d.lds_words = off;
   d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + max_lds_packed);
   if (static_cast<int64_t>(d.lds_words) * 4 + 64 > 160 * 1024)
@@ -530,14 +567,14 @@ int sba_batch_info(sba_handle* h, sba_batch* b, int64_t* num_observations, int64
 
 int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int32_t* launches_per_class) {
   if (!h || !b) return SBA_E_INVALID;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     b->class_ms[k] = 0.f;
     b->class_launches[k] = 0;
   }
   const int rc = enqueue_solve(h, b, true);
   if (rc != SBA_OK) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     if (ms_per_class) ms_per_class[k] = b->class_ms[k];
     if (launches_per_class) launches_per_class[k] = b->class_launches[k];
   }
@@ -569,7 +606,7 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
     st0.lambda = static_cast<float>(lambda);
     st0.done = 0;
     if (hipMemcpyAsync(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice, s) != hipSuccess ||
-        launch_schur(d, s) != hipSuccess || launch_factor(d, b->dbg_S, s) != hipSuccess ||
+        launch_schur(d, s) != hipSuccess || launch_reduce(d, s) != hipSuccess || launch_factor(d, b->dbg_S, s) != hipSuccess ||
         launch_backsub(d, false, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
       rc = fail(h, SBA_E_DEVICE, "step failed");
       break;
@@ -613,20 +650,22 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
 int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bvec, double* x) {
   if (!h || !A || !bvec || !x || n <= 0 || n % 6 != 0 || n > 6 * kMaxFreePoses) return fail(h, SBA_E_INVALID, "bad debug_factor size");
   HIP_TRY(hipSetDevice(h->opt.device));
-  const int ncf = n / 6, nb = ncf * (ncf + 1) / 2;
-  // pack A into the accumulator layout: upper 6x6 blocks (row-major), then e, then diag(U) = 0
-  std::vector<float> acc(36 * nb + 12 * ncf, 0.f);
-  for (int bj = 0; bj < ncf; ++bj)
-    for (int bk = bj; bk < ncf; ++bk) {
-      const int blk = bj * ncf - (bj * (bj - 1)) / 2 + (bk - bj);
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) {
-          const int i = 6 * bj + r, j = 6 * bk + c;
-          if (bj == bk && c < r) continue;
-          acc[36 * blk + 6 * r + c] = static_cast<float>(A[static_cast<int64_t>(j) * n + i]);  // lower triangle of A
+  const int ncf = n / 6;
+  const int nt = (n + 15) / 16, ntu = nt * (nt + 1) / 2;
+  // a reduced record holding -A as the Schur tiles (so that U - sum Z Z^T = A), U = 0, e = b, diag U = 0
+  std::vector<float> rec(256 * ntu + 33 * ncf, 0.f);
+  for (int I = 0; I < nt; ++I)
+    for (int J = I; J < nt; ++J) {
+      const int tile = I * nt - (I * (I - 1)) / 2 + (J - I);
+      for (int r = 0; r < 16; ++r)
+        for (int c = 0; c < 16; ++c) {
+          const int i = 16 * I + r, j = 16 * J + c;
+          if (i >= n || j >= n) continue;
+          const double v = i >= j ? A[static_cast<int64_t>(i) * n + j] : A[static_cast<int64_t>(j) * n + i];
+          rec[256 * tile + 64 * (r & 3) + 16 * (r >> 2) + c] = static_cast<float>(-v);
         }
     }
-  for (int i = 0; i < n; ++i) acc[36 * nb + i] = static_cast<float>(bvec[i]);
+  for (int i = 0; i < n; ++i) rec[256 * ntu + 21 * ncf + i] = static_cast<float>(bvec[i]);
   sba_batch* b = new (std::nothrow) sba_batch;
   if (!b) return fail(h, SBA_E_INVALID, "out of host memory");
   b->n = 1;
@@ -634,6 +673,8 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
   WindowDesc& W = b->win[0];
   W = WindowDesc{};
   W.nc = W.ncf = ncf;
+  W.nt = nt;
+  W.rec = static_cast<int32_t>(rec.size());
   W.ncam = 1;
   W.no = 1;
   W.max_iterations = 1;
@@ -652,7 +693,7 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
   st0.lambda = 0.f;
   std::vector<WindowState> stv(1, st0);
   if ((rc = upload(h, b, b->win, const_cast<WindowDesc**>(&d.win))) != SBA_OK ||
-      (rc = upload(h, b, stv, &d.st)) != SBA_OK || (rc = upload(h, b, acc, &d.s_acc)) != SBA_OK ||
+      (rc = upload(h, b, stv, &d.st)) != SBA_OK || (rc = upload(h, b, rec, &d.sred)) != SBA_OK ||
       (rc = upload(h, b, poses, &d.poses0)) != SBA_OK || (rc = upload(h, b, poses, &d.poses1)) != SBA_OK ||
       (rc = alloc(h, b, static_cast<size_t>(n), &d.dc)) != SBA_OK || (rc = alloc(h, b, static_cast<size_t>(n), &d.dc64)) != SBA_OK ||
       (rc = alloc(h, b, static_cast<size_t>(W.factor_in_lds ? 1 : packed), &d.A)) != SBA_OK) {
@@ -672,6 +713,49 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
   }
   free_batch(b);
   return st1.chol_ok ? SBA_OK : SBA_E_INFEASIBLE;
+}
+
+int sba_debug_mfma_layout(sba_handle* h, const float* A, const float* B, float* D) {
+  if (!h || !A || !B || !D) return fail(h, SBA_E_INVALID, "bad debug_mfma_layout arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  float *a = nullptr, *b = nullptr, *d = nullptr;
+  HIP_TRY(hipMalloc(&a, 64 * 4));
+  HIP_TRY(hipMalloc(&b, 64 * 4));
+  HIP_TRY(hipMalloc(&d, 256 * 4));
+  HIP_TRY(hipMemcpy(a, A, 64 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b, B, 64 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(launch_mfma_layout(a, b, d, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(D, d, 256 * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(d);
+  return SBA_OK;
+}
+
+int sba_debug_copy(sba_handle* h, int64_t bytes, int32_t repeats, float* ms) {
+  if (!h || bytes <= 0 || repeats <= 0) return fail(h, SBA_E_INVALID, "bad debug_copy arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  const int64_t n = bytes / 16;
+  void *a = nullptr, *c = nullptr;
+  HIP_TRY(hipMalloc(&a, n * 16));
+  HIP_TRY(hipMalloc(&c, n * 16));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipMemsetAsync(a, 0, n * 16, h->stream));
+  HIP_TRY(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < repeats; ++r) HIP_TRY(launch_copy_calibrate(static_cast<float4*>(c), static_cast<const float4*>(a), n, h->stream));
+  HIP_TRY(hipEventRecord(e1, h->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t / repeats;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(c);
+  return SBA_OK;
 }
 
 int sba_solve_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_result* results) {

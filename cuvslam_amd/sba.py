@@ -224,9 +224,9 @@ class DeviceBatch:
 
     def kernel_times(self):
         """One eager solve with HIP events around every launch: total ms and launches per kernel class
-        (linearize+Schur, factor, back-substitute+cost, decide)."""
-        ms = (C.c_float * 4)()
-        n = (C.c_int32 * 4)()
+        (linearize+Schur, record reduction, factor, back-substitute+cost, decide)."""
+        ms = (C.c_float * 5)()
+        n = (C.c_int32 * 5)()
         rc = self.solver._L.sba_batch_kernel_times(self.solver._h, self._b, ms, n)
         if rc != SBA_OK:
             raise SbaError(rc, self.solver.last_error())

@@ -72,7 +72,7 @@ def generate(preset: str | None = None, seed: int = 1, max_iterations: int = 10,
         L.sba_scene_free(sc)
 
 
-def batched_windows(n_windows: int = 512, seed: int = 3, max_iterations: int = 10, kf_range=(5, 40),
+def batched_windows(n_windows: int = 512, seed: int = 3, max_iterations: int = 10, kf_range=(5, 40), num_fixed_keyframes: int = 2,
                     pt_range=(500, 8000), outlier_range=(0.0, 0.10)):
     """The uneven batch: per window, key frames U{kf_range}, points U{pt_range}, outlier rate U(outlier_range),
     each point observed by U{2..key frames} of the key frames it projects in. Window w uses seed
@@ -84,6 +84,6 @@ def batched_windows(n_windows: int = 512, seed: int = 3, max_iterations: int = 1
         npt = int(rng.integers(pt_range[0], pt_range[1] + 1))
         orate = float(rng.uniform(*outlier_range))
         p, _, _ = generate(seed=seed * 10**6 + w, max_iterations=max_iterations, num_keyframes=nk, num_points=npt,
-                           outlier_rate=orate, sample_observations=1)
+                           outlier_rate=orate, sample_observations=1, num_fixed_keyframes=num_fixed_keyframes)
         out.append(p)
     return out

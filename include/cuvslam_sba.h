@@ -118,8 +118,9 @@ int sba_batch_run(sba_handle* h, sba_batch* b);       /* asynchronous on the han
 int sba_batch_sync(sba_handle* h);                     /* waits for the handle's stream */
 int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results);   /* synchronous */
 int sba_batch_destroy(sba_handle* h, sba_batch* b);
-/* Number of launches and observations of a batch, and the kernel-event timings of its last run
- * (milliseconds per kernel class: [linearize+Schur, reduce+factor, back-substitute+cost, decide]). */
+/* Number of observations, pairs, chunks and windows of a batch; and one eager solve timed with HIP events
+ * around every launch (total milliseconds and launches per kernel class, 5 classes:
+ * [linearize+Schur, record reduction, factor/solve, back-substitute+cost, decide]). */
 int sba_batch_info(sba_handle* h, sba_batch* b, int64_t* num_observations, int64_t* num_pairs,
                    int32_t* num_chunks, int32_t* num_windows);
 int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int32_t* launches_per_class);
@@ -140,6 +141,14 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
  * SBA_E_INFEASIBLE when A is not positive definite (the reference's devInfo != 0,
  * libs/cuda_modules/sba.cpp:619-642). For the factorization known-answer tests. */
 int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* b, double* x);
+
+/* Copies `bytes` between two device buffers `repeats` times with a plain streaming kernel (k_copy_calibrate)
+ * and returns the mean ms per copy: the known-size reference that calibrates HBM byte counters. */
+int sba_debug_copy(sba_handle* h, int64_t bytes, int32_t repeats, float* ms);
+
+/* One FP32 16x16x4 matrix-core product D = A B (A 16x4, B 4x16, D 16x16, row-major) on one wavefront:
+ * checks the lane layout the Schur kernel relies on. */
+int sba_debug_mfma_layout(sba_handle* h, const float* A, const float* B, float* D);
 
 /* ---- Synthetic scenes (the benchmark generator; host only, no device needed) -------------------------- */
 typedef struct sba_scene_config {

@@ -15,6 +15,18 @@ def small_scene(seed=1, n_points=300, n_kf=6, nf=2, **kw):
     return p, g, q
 
 
+def test_mfma_lane_layout(solver_huber):
+    # the v_mfma_f32_16x16x4_f32 lane layout the Schur kernel relies on, checked against a plain product
+    import ctypes as C
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 4)).astype(np.float32)
+    B = rng.standard_normal((4, 16)).astype(np.float32)
+    D = np.zeros((16, 16), np.float32)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    assert solver_huber._L.sba_debug_mfma_layout(solver_huber._h, f(A), f(B), f(D)) == 0
+    np.testing.assert_allclose(D, A.astype(np.float64) @ B.astype(np.float64), rtol=1e-5, atol=1e-5)
+
+
 def test_factor_known_answer(solver_huber):
     # The reference's Cholesky known answer (libs/cuda_modules/test/sba_test.cpp:700-719): A x = b with
     # A = [[4,1,0],[1,3,1],[0,1,2]], b = (6,10,8) -> x = (1,2,3); embedded in a 6x6 block-diagonal system.

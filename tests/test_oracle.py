@@ -1,0 +1,274 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each function of oracle/sba_oracle.cpp is checked against something it does not compute itself:
+  * golden fixtures from tests/golden/make_fixtures.py (numpy/scipy: finite-difference Jacobians and a dense
+    solve of the damped normal equations) -- linearization, full system, Schur complement, point-block
+    inverse, Cholesky, back-substitution, damping, trial cost, predicted-reduction terms;
+  * the reference's own known answers (libs/cuda_modules/test/sba_test.cpp: the 3x3 Cholesky system, the
+    near-plane bands, convergence on exact observations);
+  * closed forms and library routines (scipy.linalg.expm, numpy SVD / pinv / cholesky).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import cuvslam_amd as ca
+from oracle import sba_oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def fixture_problems():
+    d = json.load(open(os.path.join(HERE, "golden", "lm_step_fixtures.json")))
+    out = []
+    for c in d["cases"]:
+        p = ca.BundleAdjustmentProblem(
+            np.array(c["points"], np.float32), np.array(c["xy"], np.float32), np.array(c["info"], np.float32),
+            np.array(c["point_ids"], np.int32), np.array(c["pose_ids"], np.int32), np.array(c["camera_ids"], np.int8),
+            np.array(c["rig_from_world"], np.float32), np.array(c["camera_from_rig"], np.float32),
+            num_fixed_points=c["num_fixed_points"], num_fixed_key_frames=c["num_fixed_key_frames"],
+            max_iterations=10, robustifier_scale=c["delta"])
+        out.append((p, c))
+    return out
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_step_matches_dense_fixture(case):
+    p, c = fixture_problems()[case]
+    for st in c["steps"]:
+        S, e, dc, dp, sc, rc = O.step(p, st["lam"], loss=c["loss"])
+        assert rc == 0
+        # limited by the finite-difference Jacobians of the fixture (central differences, h = 1e-6)
+        assert rel(S, st["S"]) < 1e-6
+        assert rel(e, st["e"]) < 1e-6
+        assert rel(dc, st["dc"]) < 1e-5
+        assert rel(dp, st["dp"]) < 1e-5
+        assert abs(sc[0] - st["trial_cost"]) / st["trial_cost"] < 1e-7
+        assert abs(sc[2] - st["hess"]) / st["hess"] < 1e-6
+        assert abs(sc[3] - st["scal"]) / st["scal"] < 1e-5
+        assert abs(sc[6] - c["initial_cost"]) / c["initial_cost"] < 1e-12
+
+
+def test_dense_schur_equals_block_sparse():
+    # the reference CPU path's dense temp * point_pose_block product (cpu.cpp:264-277) against the block form
+    p, c = fixture_problems()[1]
+    a = O.step(p, 1e-2, loss=c["loss"], dense_schur=1)
+    b = O.step(p, 1e-2, loss=c["loss"], dense_schur=0)
+    for x, y in zip(a[:4], b[:4]):
+        assert rel(x, y) < 1e-12
+
+
+def test_pseudo_inverse_solver_matches_cholesky_on_pd_system():
+    # cpu.cpp:337-346 (Jacobi pseudo-inverse, threshold 1e-6) and the GPU bundler's Cholesky agree on a
+    # well-conditioned reduced system
+    p, c = fixture_problems()[0]
+    a = O.step(p, 1e-2, loss=c["loss"], rcs_solver=0)
+    b = O.step(p, 1e-2, loss=c["loss"], rcs_solver=1)
+    assert rel(a[2], b[2]) < 1e-9
+
+
+def test_linearize_jacobians_vs_central_differences():
+    p, _ = fixture_problems()[0]
+    r, Jc, Jp, w = O.linearize(p, loss=1)
+    pts = p.points.astype(np.float64)
+    poses = p.rig_from_world.astype(np.float64)
+    cams = p.camera_from_rig.astype(np.float64)
+
+    def hat(tw):
+        M = np.zeros((4, 4))
+        M[:3, :3] = [[0, -tw[2], tw[1]], [tw[2], 0, -tw[0]], [-tw[1], tw[0], 0]]
+        M[:3, 3] = tw[3:]
+        return M
+
+    for o in range(0, p.num_observations, 7):
+        i, j, c = p.point_ids[o], p.pose_ids[o], p.camera_ids[o]
+
+        def proj(T, X):
+            q = cams[c] @ T @ np.append(X, 1.0)
+            return q[:2] / q[2]
+
+        h = 1e-6
+        num_c = np.zeros((2, 6))
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = h
+            num_c[:, k] = (proj(expm(hat(e)) @ poses[j], pts[i]) - proj(expm(hat(-e)) @ poses[j], pts[i])) / (2 * h)
+        num_p = np.zeros((2, 3))
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            num_p[:, k] = (proj(poses[j], pts[i] + e) - proj(poses[j], pts[i] - e)) / (2 * h)
+        np.testing.assert_allclose(Jc[o], num_c, rtol=1e-6, atol=1e-8)
+        np.testing.assert_allclose(Jp[o], num_p, rtol=1e-6, atol=1e-8)
+        np.testing.assert_allclose(r[o], p.observation_xys[o].astype(np.float64) - proj(poses[j], pts[i]), atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_robust_loss_closed_forms(kind):
+    d = 0.7
+    for s in [0.0, 0.1, 0.48, 0.49, 0.5, 2.0, 30.0]:
+        rho = O.loss(kind, s, d)
+        if kind == 0:
+            assert rho == pytest.approx((d + 1) * np.log(1 + s / d), rel=1e-14)
+            assert O.loss(kind, s, d, True) == pytest.approx((d + 1) / (d + s), rel=1e-14)
+        else:
+            want = 0.5 * s if s < d * d else d * np.sqrt(s) - 0.5 * d * d
+            assert rho == pytest.approx(want, rel=1e-14, abs=1e-300)
+        if s > 0.05 and abs(s - d * d) > 1e-3:  # rho' is the derivative of rho (away from the Huber knee)
+            h = 1e-6
+            fd = (O.loss(kind, s + h, d) - O.loss(kind, s - h, d)) / (2 * h)
+            assert O.loss(kind, s, d, True) == pytest.approx(fd, rel=1e-7)
+    # Huber is continuous at the knee s = d^2 (robust_cost_function.cpp:62-75)
+    assert O.loss(1, d * d * (1 - 1e-12), d) == pytest.approx(O.loss(1, d * d, d), rel=1e-9)
+
+
+def test_exp_matches_matrix_exponential():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        tw = np.concatenate([rng.normal(0, 0.8, 3), rng.normal(0, 2, 3)])
+        M = np.zeros((4, 4))
+        M[:3, :3] = [[0, -tw[2], tw[1]], [tw[2], 0, -tw[0]], [-tw[1], tw[0], 0]]
+        M[:3, 3] = tw[3:]
+        np.testing.assert_allclose(O.exp(tw), expm(M), atol=1e-12)
+    np.testing.assert_allclose(O.exp(np.zeros(6)), np.eye(4), atol=1e-15)
+
+
+def test_nearest_rotation_is_svd_polar_factor():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        R0 = expm(np.array([[0, -0.3, 0.2], [0.3, 0, -0.1], [-0.2, 0.1, 0]]) * rng.normal())
+        M = R0 + rng.normal(0, 1e-3, (3, 3))
+        U, _, Vt = np.linalg.svd(M)
+        want = U @ Vt
+        if np.linalg.det(want) < 0:
+            want = want - 2 * np.outer(U[:, 2], Vt[2])
+        R = O.nearest_rotation(M)
+        np.testing.assert_allclose(R, want, atol=1e-12)
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_cholesky_known_answer_and_failure():
+    # libs/cuda_modules/test/sba_test.cpp:700-719
+    A = np.array([[4, 1, 0], [1, 3, 1], [0, 1, 2]], float)
+    x, rc = O.chol_solve(A, np.array([6, 10, 8], float))
+    assert rc == 0
+    np.testing.assert_allclose(x, [1, 2, 3], atol=1e-14)
+    _, rc = O.chol_solve(np.array([[1, 2, 0], [2, 1, 0], [0, 0, 1]], float), np.array([6, 10, 8], float))
+    assert rc == 1
+    rng = np.random.default_rng(2)
+    M = rng.standard_normal((30, 30))
+    A = M @ M.T + 30 * np.eye(30)
+    b = rng.standard_normal(30)
+    x, rc = O.chol_solve(A, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-12)
+
+
+def test_point_pseudo_inverse_rank_rule():
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    full = Q @ np.diag([5.0, 2.0, 0.5]) @ Q.T
+    inv, rank = O.point_pinv(full)
+    assert rank == 3
+    np.testing.assert_allclose(inv, np.linalg.inv(full), rtol=1e-12)
+    # rank 2: the smallest singular value is below 1e-6 of the largest -> its direction is dropped
+    two = Q @ np.diag([5.0, 2.0, 1e-9]) @ Q.T
+    inv, rank = O.point_pinv(two)
+    assert rank == 2
+    np.testing.assert_allclose(inv, np.linalg.pinv(two, rcond=1e-6), rtol=1e-9, atol=1e-12)
+    # rank 1 -> zero (schur_complement_bundler_cpu.cpp:242-254)
+    one = Q @ np.diag([5.0, 1e-9, 1e-10]) @ Q.T
+    inv, rank = O.point_pinv(one)
+    assert rank == 1
+    assert np.all(inv == 0)
+
+
+def test_cost_zero_at_exact_observations_and_infinite_inside_near_plane():
+    p, _, _ = ca.scene.generate(seed=4, num_points=100, num_keyframes=4, pixel_noise=0.0, outlier_rate=0.0,
+                                pose_sigma_m=0.0, pose_sigma_deg=0.0, point_sigma_m=0.0)
+    c, sk = O.cost(p, loss=1)
+    assert sk == 0 and c < 1e-6 * p.num_observations  # FP32 storage of the exact projections
+    # sba_test.cpp:1402-1462: every point between 0.02 and 0.08 m in front of the cameras -> all skipped, +inf
+    q = p.copy()
+    q.rig_from_world[:] = np.eye(4, dtype=np.float32)
+    q.points[:, 2] = np.linspace(0.02, 0.08, q.points.shape[0], dtype=np.float32)
+    q.points[:, :2] *= 0.001
+    c, sk = O.cost(q, loss=1)
+    assert sk == q.num_observations and np.isinf(c)
+    r = O.solve(q, "f64", loss=1)
+    assert r["status"] == 2
+    # 0.3-0.8 m is in front of the near plane: kept (sba_test.cpp:1361-1396)
+    q.points[:, 2] = np.linspace(0.3, 0.8, q.points.shape[0], dtype=np.float32)
+    c, sk = O.cost(q, loss=1)
+    assert sk == 0 and np.isfinite(c)
+
+
+def reference_style_problem(seed, num_points=200, num_poses=5):
+    """The reference test generator's scene (sba_test.cpp:46-128): two-camera rig with a 0.125 m baseline,
+    poses Exp(omega ~ U(+-0.25), t ~ U(+-2)), points xy ~ U(+-4), z ~ U(2, 16), every (pose, point, camera)
+    with camera depth > 1 observed, identity information, one fixed pose."""
+    rng = np.random.default_rng(seed)
+    poses = []
+    for _ in range(num_poses):
+        tw = np.concatenate([rng.uniform(-0.25, 0.25, 3), rng.uniform(-2, 2, 3)])
+        poses.append(O.exp(tw).astype(np.float32))
+    pts = np.stack([rng.uniform(-4, 4, num_points), rng.uniform(-4, 4, num_points), rng.uniform(2, 16, num_points)], 1)
+    cams = np.stack([np.eye(4), np.eye(4)]).astype(np.float32)
+    cams[1, 0, 3] = 0.125
+    xy, pid, qid, cid = [], [], [], []
+    for j in range(num_poses):
+        for i in range(num_points):
+            for c in range(2):
+                q = cams[c].astype(np.float64) @ poses[j].astype(np.float64) @ np.append(pts[i], 1)
+                if q[2] <= 1.0:
+                    continue
+                xy.append(q[:2] / q[2])
+                pid.append(i)
+                qid.append(j)
+                cid.append(c)
+    n = len(xy)
+    return ca.BundleAdjustmentProblem(pts.astype(np.float32), np.array(xy, np.float32),
+                                      np.tile(np.eye(2, dtype=np.float32), (n, 1, 1)), np.array(pid, np.int32),
+                                      np.array(qid, np.int32), np.array(cid, np.int8), np.stack(poses), cams,
+                                      num_fixed_points=0, num_fixed_key_frames=1, max_iterations=50, robustifier_scale=1.0)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_converges_on_exact_observations(precision):
+    # sba_test.cpp:1322-1355: exact observations, free key frames shifted by (0.01, -0.01, 0.01), 50 iterations
+    p = reference_style_problem(1)
+    p.rig_from_world[:-1, :3, 3] += np.array([0.01, -0.01, 0.01], np.float32)
+    kw = dict(loss=0) if precision == "f64" else dict(loss=0, rcs_solver=1, dense_schur=1)
+    r = O.solve(p, precision, **kw)
+    assert r["status"] == 0
+    assert np.isfinite(r["initial_cost"]) and np.isfinite(r["last_cost"])
+    assert r["last_cost"] < r["initial_cost"]
+    assert r["iterations"] < 50
+
+
+def test_initial_cost_below_epsilon_returns_immediately():
+    p, _, _ = ca.scene.generate(seed=5, num_points=60, num_keyframes=3, pixel_noise=0.0, outlier_rate=0.0,
+                                pose_sigma_m=0.0, pose_sigma_deg=0.0, point_sigma_m=0.0)
+    q = p.copy()
+    q.observation_infos *= 1e-12  # exact projections; information scaled so the cost is below FLT_EPSILON
+    r = O.solve(q, "f64", loss=1)
+    assert r["status"] == 0 and r["iterations"] == 0
+
+
+def test_fp32_reference_path_tracks_fp64_oracle():
+    p, _, _ = ca.scene.generate("c1", seed=1, num_fixed_keyframes=2, num_points=600, max_iterations=10)
+    a = O.solve(p, "f64", loss=1)
+    b = O.solve(p, "f32", loss=1, rcs_solver=1, dense_schur=1)
+    q = p.copy()
+    q.points = b["points"].astype(np.float32)
+    q.rig_from_world = b["poses"].astype(np.float32)
+    c, _ = O.cost(q, loss=1)
+    assert abs(c - a["last_cost"]) / a["last_cost"] < 1e-6
