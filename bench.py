@@ -124,10 +124,16 @@ def main():
         nf = p0.rig_from_world.shape[0] - p0.num_fixed_key_frames
         dt = np.abs(p0.rig_from_world[:nf, :3, 3] - ref["poses"][:nf, :3, 3]).max()
         dR = p0.rig_from_world[:nf, :3, :3] @ np.transpose(ref["poses"][:nf, :3, :3], (0, 2, 1))
-        ang = np.degrees(np.arccos(np.clip((np.trace(dR, axis1=1, axis2=2) - 1) / 2, -1, 1))).max()
+        # rotation angle from the skew part (arcsin |vee(dR - dR^T)| / 2), exact for small angles where arccos of
+        # the trace loses half the digits
+        sk = np.stack([dR[:, 2, 1] - dR[:, 1, 2], dR[:, 0, 2] - dR[:, 2, 0], dR[:, 1, 0] - dR[:, 0, 1]], 1)
+        ang = np.degrees(np.arcsin(np.clip(np.linalg.norm(sk, axis=1) / 2, 0, 1))).max()
         extra["pose_err_m"] = float(dt)
         extra["pose_err_deg"] = float(ang)
-        extra["point_rms_vs_oracle_m"] = float(np.sqrt(((p0.points - ref["points"]) ** 2).mean()))
+        perr = np.linalg.norm(p0.points - ref["points"], axis=1)
+        extra["point_rms_vs_oracle_m"] = float(np.sqrt((perr ** 2).mean()))
+        extra["point_median_vs_oracle_m"] = float(np.median(perr))
+        extra["point_max_vs_oracle_m"] = float(perr.max())
         extra["oracle_iterations"] = ref["iterations"]
         log(f"[bench] oracle FP64 solve {time.time() - t0:.1f}s: {extra}")
     extra["iterations"] = int(p0.iterations)

@@ -113,4 +113,25 @@ struct Batch {
   int32_t factor_lds_words;  // LDS words of the factor kernel
 };
 
+// Orthonormal polar factor of a near-rotation (Newton iteration X <- (X + X^-T) / 2, Higham 1986); for a
+// matrix with positive determinant it is U V^T of its SVD, which common/rotation_utils.h:36-50 returns.
+__device__ __forceinline__ void polar3(double X[3][3]) {
+  for (int it = 0; it < 3; ++it) {
+    const double c00 = X[1][1] * X[2][2] - X[1][2] * X[2][1];
+    const double c01 = X[1][2] * X[2][0] - X[1][0] * X[2][2];
+    const double c02 = X[1][0] * X[2][1] - X[1][1] * X[2][0];
+    const double c10 = X[0][2] * X[2][1] - X[0][1] * X[2][2];
+    const double c11 = X[0][0] * X[2][2] - X[0][2] * X[2][0];
+    const double c12 = X[0][1] * X[2][0] - X[0][0] * X[2][1];
+    const double c20 = X[0][1] * X[1][2] - X[0][2] * X[1][1];
+    const double c21 = X[0][2] * X[1][0] - X[0][0] * X[1][2];
+    const double c22 = X[0][0] * X[1][1] - X[0][1] * X[1][0];
+    const double det = X[0][0] * c00 + X[0][1] * c01 + X[0][2] * c02;
+    const double id = 1.0 / det;  // X^-T = cofactor(X) / det
+    const double C[3][3] = {{c00, c01, c02}, {c10, c11, c12}, {c20, c21, c22}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) X[i][j] = 0.5 * (X[i][j] + C[i][j] * id);
+  }
+}
+
 }  // namespace sba_dev

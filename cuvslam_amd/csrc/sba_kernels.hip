@@ -23,6 +23,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "sba_device.h"
 
 using namespace sba_dev;
@@ -196,70 +198,84 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTilesPerWave = (kNT * (kNT + 1) / 2 + 3) / 4;  // 30
 
-// Upper tile T (row-major over the kNT x kNT upper triangle) -> (I, J).
-constexpr int tile_i(int T) {
+// Upper tile T (row-major over the NT x NT upper triangle) -> (I, J).
+constexpr int tri_i(int nt, int T) {
   int I = 0;
-  while (T >= kNT - I) {
-    T -= kNT - I;
+  while (T >= nt - I) {
+    T -= nt - I;
     ++I;
   }
   return I;
 }
-constexpr int tile_j(int T) {
-  int I = 0;
-  while (T >= kNT - I) {
-    T -= kNT - I;
-    ++I;
-  }
-  return I + T;
-}
+constexpr int tri_j(int nt, int T) { return tri_i(nt, T) + (T - (tri_i(nt, T) * nt - tri_i(nt, T) * (tri_i(nt, T) - 1) / 2)); }
 
 __device__ __forceinline__ int upper_tile(int I, int J, int nt) { return I * nt - (I * (I - 1)) / 2 + (J - I); }
 
-// The matrix-core part of one tile: acc(I, J) += Zc[16 I.., k] Zc[16 J.., k]^T over its k-steps, for the
-// tiles of wave WAVE (T = WAVE, WAVE + 4, ...). Lane layout of v_mfma_f32_16x16x4_f32: A lane l holds
-// A[l % 16][l / 16], B lane l holds B[l / 16][l % 16], D lane l holds D[4 (l / 16) + v][l % 16].
-template <int WAVE>
-__device__ __forceinline__ void mfma_tile(const float* sZc, int nslab, int nt, int lane, floatx4 (&acc)[kTilesPerWave]) {
-  for (int s = 0; s < nslab; ++s) {
-    const float* col = sZc + (4 * s + (lane >> 4)) * kZcLd + (lane & 15);
-    float f[kNT];
-#pragma unroll
-    for (int I = 0; I < kNT; ++I) f[I] = I < nt ? col[16 * I] : 0.f;
-#pragma unroll
-    for (int t = 0; t < kTilesPerWave; ++t) {
-      const int T = WAVE + 4 * t;
-      if (T < kNT * (kNT + 1) / 2) {
-        const int I = tile_i(T), J = tile_j(T);
-        if (J < nt) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[I], f[J], acc[t], 0, 0, 0);
-      }
-    }
+template <int NT>
+constexpr int tiles_per_wave() { return (NT * (NT + 1) / 2 + 3) / 4; }
+
+// The matrix-core part of one tile of points: acc(I, J) += Zc[16 I.., k] Zc[16 J.., k]^T over its k-steps, for
+// the upper tiles T = WAVE, WAVE + 4, ... of the window's NT x NT tile grid. Lane layout of
+// v_mfma_f32_16x16x4_f32: A lane l holds A[l % 16][l / 16], B lane l holds B[l / 16][l % 16], D lane l holds
+// D[4 (l / 16) + v][l % 16]. NT and the tile coordinates are template parameters: every register index is a
+// compile-time constant and each k-step is straight-line code (NT loads, then this wave's instructions).
+template <int NT, int WAVE, int t>
+__device__ __forceinline__ void mfma_one(const float (&f)[NT], floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  constexpr int T = WAVE + 4 * t;
+  if constexpr (T < NT * (NT + 1) / 2) {
+    constexpr int I = tri_i(NT, T), J = tri_j(NT, T);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f[I], f[J], acc[t], 0, 0, 0);
   }
 }
 
-template <int WAVE>
-__device__ __forceinline__ void store_tiles(float* rec, int nt, int lane, const floatx4 (&acc)[kTilesPerWave]) {
+template <int NT, int WAVE, int... ts>
+__device__ __forceinline__ void mfma_all(const float (&f)[NT], floatx4 (&acc)[tiles_per_wave<NT>()],
+                                         std::integer_sequence<int, ts...>) {
+  (mfma_one<NT, WAVE, ts>(f, acc), ...);
+}
+
+template <int NT, int WAVE>
+__device__ __forceinline__ void mfma_tile(const float* sZc, int nslab, int lane, floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  const float* col = sZc + (lane >> 4) * kZcLd + (lane & 15);
+  for (int s = 0; s < nslab; ++s, col += 4 * kZcLd) {
+    float f[NT];
 #pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) {
-    const int T = WAVE + 4 * t;
-    if (T < kNT * (kNT + 1) / 2) {
-      const int I = tile_i(T), J = tile_j(T);
-      if (J < nt) {
-        float* dst = rec + 256 * upper_tile(I, J, nt);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) dst[64 * v + lane] = acc[t][v];
-      }
-    }
+    for (int I = 0; I < NT; ++I) f[I] = col[16 * I];
+    mfma_all<NT, WAVE>(f, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
   }
 }
 
+template <int NT, int WAVE, int t>
+__device__ __forceinline__ void store_one(float* rec, int lane, const floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  constexpr int T = WAVE + 4 * t;
+  if constexpr (T < NT * (NT + 1) / 2) {
+    constexpr int I = tri_i(NT, T), J = tri_j(NT, T);
+    float* dst = rec + 256 * upper_tile(I, J, NT);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[64 * v + lane] = acc[t][v];
+  }
+}
+
+template <int NT, int WAVE, int... ts>
+__device__ __forceinline__ void store_all(float* rec, int lane, const floatx4 (&acc)[tiles_per_wave<NT>()],
+                                          std::integer_sequence<int, ts...>) {
+  (store_one<NT, WAVE, ts>(rec, lane, acc), ...);
+}
+
+template <int NT, int WAVE>
+__device__ __forceinline__ void store_tiles(float* rec, int lane, const floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  store_all<NT, WAVE>(rec, lane, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
+}
+
+// One chunk of tiles of one window with NT = ceil(6 N_c,free / 16) tiles per side of its reduced system.
+template <int NT>
 __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, const WindowDesc& W, float lam, int cur,
                                             float* sm) {
+  constexpr int nt = NT;
   const float4* __restrict__ pts = cur ? b.pts1 : b.pts0;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int ncf = W.ncf;
-  const int nt = W.nt;
   float* sPose = sm + b.lds_pose;
   float* sCam = sm + b.lds_cam;
   float* sU = sm + b.lds_u;  // U (21 per free key frame), e (6), diag U (6)
@@ -273,9 +289,9 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const float delta = W.robust;
   const int pf_end = W.point_off + W.npf;
 
-  floatx4 acc[kTilesPerWave];
+  floatx4 acc[tiles_per_wave<NT>()];
 #pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < tiles_per_wave<NT>(); ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
     const Tile T = b.tiles[t];
@@ -292,7 +308,11 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
       for (int k = 0; k < 12; ++k) sAcc[12 * tid + k] = 0.f;
     }
-    for (int k = tid; k < kpad * kZcLd; k += kBlock) sZc[k] = 0.f;
+    {  // zero the staged columns that the matrix cores read: kpad k-rows x 16 nt rows, 16-byte stores
+      const int q = 4 * nt;
+      float4* z4 = reinterpret_cast<float4*>(sZc);
+      for (int k = tid; k < kpad * q; k += kBlock) z4[(k / q) * (kZcLd / 4) + k % q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     __syncthreads();
     // (b) one thread per pair: linearize its observations (one per camera) and accumulate
     if (tid < npair) {
@@ -436,19 +456,19 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     __syncthreads();
     // (e) matrix cores: acc += Zc Zc^T over the tile's k-steps (only this loop is specialised per wave)
     switch (tid >> 6) {
-      case 0: mfma_tile<0>(sZc, kpad / 4, nt, lane, acc); break;
-      case 1: mfma_tile<1>(sZc, kpad / 4, nt, lane, acc); break;
-      case 2: mfma_tile<2>(sZc, kpad / 4, nt, lane, acc); break;
-      default: mfma_tile<3>(sZc, kpad / 4, nt, lane, acc); break;
+      case 0: mfma_tile<NT, 0>(sZc, kpad / 4, lane, acc); break;
+      case 1: mfma_tile<NT, 1>(sZc, kpad / 4, lane, acc); break;
+      case 2: mfma_tile<NT, 2>(sZc, kpad / 4, lane, acc); break;
+      default: mfma_tile<NT, 3>(sZc, kpad / 4, lane, acc); break;
     }
   }
   // the chunk's record: Schur tiles (lane order), then U, e, diag U
   float* rec = b.part + ch.part_off;
   switch (tid >> 6) {
-    case 0: store_tiles<0>(rec, nt, lane, acc); break;
-    case 1: store_tiles<1>(rec, nt, lane, acc); break;
-    case 2: store_tiles<2>(rec, nt, lane, acc); break;
-    default: store_tiles<3>(rec, nt, lane, acc); break;
+    case 0: store_tiles<NT, 0>(rec, lane, acc); break;
+    case 1: store_tiles<NT, 1>(rec, lane, acc); break;
+    case 2: store_tiles<NT, 2>(rec, lane, acc); break;
+    default: store_tiles<NT, 3>(rec, lane, acc); break;
   }
   __syncthreads();
   const int ntu = nt * (nt + 1) / 2;
@@ -470,7 +490,24 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
   load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
   for (int k = tid; k < 33 * W.ncf; k += kBlock) sm[b.lds_u + k] = 0.f;
-  schur_chunk(b, ch, W, lam, cur, sm);
+  static_assert(kNT == 15, "the dispatch below covers 1..15 tiles");
+  switch (W.nt) {
+    case 1: schur_chunk<1>(b, ch, W, lam, cur, sm); break;
+    case 2: schur_chunk<2>(b, ch, W, lam, cur, sm); break;
+    case 3: schur_chunk<3>(b, ch, W, lam, cur, sm); break;
+    case 4: schur_chunk<4>(b, ch, W, lam, cur, sm); break;
+    case 5: schur_chunk<5>(b, ch, W, lam, cur, sm); break;
+    case 6: schur_chunk<6>(b, ch, W, lam, cur, sm); break;
+    case 7: schur_chunk<7>(b, ch, W, lam, cur, sm); break;
+    case 8: schur_chunk<8>(b, ch, W, lam, cur, sm); break;
+    case 9: schur_chunk<9>(b, ch, W, lam, cur, sm); break;
+    case 10: schur_chunk<10>(b, ch, W, lam, cur, sm); break;
+    case 11: schur_chunk<11>(b, ch, W, lam, cur, sm); break;
+    case 12: schur_chunk<12>(b, ch, W, lam, cur, sm); break;
+    case 13: schur_chunk<13>(b, ch, W, lam, cur, sm); break;
+    case 14: schur_chunk<14>(b, ch, W, lam, cur, sm); break;
+    default: schur_chunk<15>(b, ch, W, lam, cur, sm); break;
+  }
 }
 
 // Sums the chunk records of each window in chunk order (deterministic; no atomics).
@@ -524,30 +561,10 @@ __device__ void twist_exp64(const double* tw, double R[3][3], double t[3]) {
   for (int i = 0; i < 3; ++i) t[i] = v[i] + ((1.0 - c) / theta * nxv[i] + (1.0 - st) * nnv[i]);
 }
 
-// Orthonormal polar factor of a near-rotation (Newton iteration X <- (X + X^-T) / 2, Higham 1986); for a
-// matrix with positive determinant it is U V^T of its SVD, which common/rotation_utils.h:36-50 returns.
-__device__ void polar3(double X[3][3]) {
-  for (int it = 0; it < 3; ++it) {
-    const double c00 = X[1][1] * X[2][2] - X[1][2] * X[2][1];
-    const double c01 = X[1][2] * X[2][0] - X[1][0] * X[2][2];
-    const double c02 = X[1][0] * X[2][1] - X[1][1] * X[2][0];
-    const double c10 = X[0][2] * X[2][1] - X[0][1] * X[2][2];
-    const double c11 = X[0][0] * X[2][2] - X[0][2] * X[2][0];
-    const double c12 = X[0][1] * X[2][0] - X[0][0] * X[2][1];
-    const double c20 = X[0][1] * X[1][2] - X[0][2] * X[1][1];
-    const double c21 = X[0][2] * X[1][0] - X[0][0] * X[1][2];
-    const double c22 = X[0][0] * X[1][1] - X[0][1] * X[1][0];
-    const double det = X[0][0] * c00 + X[0][1] * c01 + X[0][2] * c02;
-    const double id = 1.0 / det;  // X^-T = cofactor(X) / det
-    const double C[3][3] = {{c00, c01, c02}, {c10, c11, c12}, {c20, c21, c22}};
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) X[i][j] = 0.5 * (X[i][j] + C[i][j] * id);
-  }
-}
 
 }  // namespace
 
-__global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_S) {
+__global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_S, double* store_L) {
   extern __shared__ double smd[];
   const int w = blockIdx.x;
   const WindowDesc W = b.win[w];
@@ -565,7 +582,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   double* panel = smd + 512;       // 256 x kPanel
   double* red = panel + 256 * kPanel;  // 32: reductions
   int* flag = reinterpret_cast<int*>(red + 32);
-  double* Lp = W.factor_in_lds ? (red + 48) : (b.A + W.a_off);
+  double* idiag = red + 48;            // 256: 1 / L_ii
+  double* Lp = W.factor_in_lds ? (idiag + 256) : (b.A + W.a_off);
   const int nt = W.nt;
   const float* R = b.sred + W.sred_off;  // sum Z Z^T tiles, then U, e, diag U
   const float* RU = R + 256 * (nt * (nt + 1) / 2);
@@ -624,8 +642,11 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
         if (c < kb) {
           const double d = readlane_d(p[0][c], c);
           if (!(d > 0.0)) ok = false;
-          const double l = sqrt(fmax(d, 1e-300));
-          const double il = 1.0 / l;
+          // 1/sqrt(d) by the hardware estimate refined with one Newton-Raphson step, l = d / sqrt(d)
+          const double dd = fmax(d, 1e-300);
+          double il = __builtin_amdgcn_rsq(dd);
+          il = il * (1.5 - 0.5 * dd * il * il);
+          const double l = dd * il;
           const double y = readlane_d(rhs[0], c) * il;
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
@@ -636,6 +657,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
             } else if (row == k0 + c) {
               p[s][c] = l;
               rhs[s] = y;
+              idiag[row] = il;
             }
           }
 #pragma unroll
@@ -697,28 +719,40 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   }
   const bool ok = flag[0] == 0;
 
+  if (store_L && w == 0) {
+    // keep the factor (packed lower L, then 1 / L_ii) for later solves with other right-hand sides
+    for (int64_t idx = tid; idx < npk; idx += kFactorThreads) store_L[idx] = Lp[idx];
+    for (int i = tid; i < n; i += kFactorThreads) store_L[npk + i] = idiag[i];
+    if (tid == 0) St->chol_ok = ok ? 1 : 0;
+    return;
+  }
+
   if (ok) {
-    // 3. backward substitution L^T x = y, blocks of kPanel rows from the bottom
-    const int nblk = (n + kPanel - 1) / kPanel;
-    for (int blk = nblk - 1; blk >= 0; --blk) {
-      const int k0 = blk * kPanel;
-      const int kb = min(kPanel, n - k0);
-      if (tid == 0) {
-        for (int c = kb - 1; c >= 0; --c) {
-          const int row = k0 + c;
-          double s = vec[row];
-          for (int c2 = c + 1; c2 < kb; ++c2) s -= Lp[pk(k0 + c2, row)] * vec[k0 + c2];
-          vec[row] = s / Lp[pk(row, row)];
+    // 3. backward substitution L^T x = y in one wavefront: y stays in registers (row k0 + lane + 64 s),
+    //    x_i = y_i / L_ii is broadcast from its lane, and every lane subtracts L_ik x_i from its rows k < i
+    if (warp == 0) {
+      double yv[4];
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) yv[s2] = (lane + 64 * s2 < n) ? vec[lane + 64 * s2] : 0.0;
+#pragma unroll
+      for (int s2 = 3; s2 >= 0; --s2) {
+        const int lo = 64 * s2;
+        for (int ii = min(n - 1, lo + 63); ii >= lo; --ii) {
+          const double xi = readlane_d(yv[s2], ii - lo) * idiag[ii];
+          const int64_t rb = pk(ii, 0);
+#pragma unroll
+          for (int s3 = 0; s3 <= s2; ++s3) {
+            const int k = lane + 64 * s3;
+            if (k < ii) yv[s3] -= Lp[rb + k] * xi;
+            else if (k == ii) yv[s3] = xi;
+          }
         }
       }
-      __syncthreads();
-      for (int i = tid; i < k0; i += kFactorThreads) {
-        double s = 0.0;
-        for (int c = 0; c < kb; ++c) s += Lp[pk(k0 + c, i)] * vec[k0 + c];
-        vec[i] -= s;
-      }
-      __syncthreads();
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+        if (lane + 64 * s2 < n) vec[lane + 64 * s2] = yv[s2];
     }
+    __syncthreads();
     // 4. the pose step, its max-norm and dc^T diag(U) dc
     double sc = 0.0;
     float mx = 0.f;
@@ -1106,9 +1140,9 @@ hipError_t launch_reduce(const Batch& b, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s) {
+hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s, double* store_L) {
   hipLaunchKernelGGL(k_factor, dim3(b.n_windows), dim3(kFactorThreads), static_cast<size_t>(b.factor_lds_words) * 4, s,
-                     b, dbg_S);
+                     b, dbg_S, store_L);
   return hipGetLastError();
 }
 

@@ -23,7 +23,15 @@ namespace sba_dev {
 hipError_t launch_reset(const Batch& b, hipStream_t s);
 hipError_t launch_schur(const Batch& b, hipStream_t s);
 hipError_t launch_reduce(const Batch& b, hipStream_t s);
-hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s);
+hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s, double* store_L = nullptr);
+hipError_t launch_adj_obs(const Batch& b, int mode, const double* spts, const double* sposes, const double* v, double* y,
+                          double* ec, const double* zc, double* dxy, double* dinfo, const int32_t* orig, int32_t* done,
+                          hipStream_t s);
+hipError_t launch_adj_state(const Batch& b, double* spts, double* sposes, const double* a, int op, hipStream_t s);
+hipError_t launch_adj_chol(const double* Lst, int n, const double* ec, double* zc, const int32_t* done, hipStream_t s);
+hipError_t launch_adj_vec(const Batch& b, double* x, double* r, double* z, double* p, double* y, double* ec, double* zc,
+                          double* acc, double* g, double* sc, int32_t* done, double tol, int nx, int n6, int op,
+                          hipStream_t s);
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_gather(const Batch& b, hipStream_t s);
@@ -52,6 +60,9 @@ struct sba_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double* dbg_S = nullptr;  // debug buffer of the reduced system of window 0
+  int32_t* obs_orig = nullptr;  // packed observation -> index in its window's input order
+  double* adj = nullptr;        // adjoint workspace of window 0 (allocated on first use)
+  bool adj_ready = false;       // the workspace holds the factor and the FP64 state of the current solution
   hipEvent_t ev[8] = {};
   float class_ms[5] = {0, 0, 0, 0, 0};
   int class_launches[5] = {0, 0, 0, 0, 0};
@@ -73,7 +84,7 @@ int fail(sba_handle* h, int code, const std::string& msg) {
 struct HostPack {
   std::vector<float2> obs_xy;
   std::vector<float4> obs_info;
-  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree;
+  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
   std::vector<float4> pts, poses, cams;
   std::vector<Tile> tiles, btiles;
 };
@@ -126,6 +137,7 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   const int obs_base = static_cast<int>(H.obs_xy.size());
   for (int k = 0; k < no; ++k) {
     const int o = order[k];
+    H.obs_orig.push_back(o);
     H.obs_xy.push_back(make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]));
     const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
     float4 inf;
@@ -326,7 +338,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     dc_off += 6 * W.ncf;
     const int nn = 6 * W.ncf;
     const int64_t packed = static_cast<int64_t>(nn) * (nn + 1) / 2;
-    const int64_t fixed_doubles = 512 + 256 * kPanel + 48;
+    const int64_t fixed_doubles = 512 + 256 * kPanel + 48 + 256;
     W.factor_in_lds = (fixed_doubles + packed) * 8 <= 160 * 1024 ? 1 : 0;
     if (W.factor_in_lds) {
       max_lds_packed = std::max<int>(max_lds_packed, static_cast<int>(packed));
@@ -423,6 +435,10 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(H.poses, poses_init);
   UP(H.cams, cams);
 #undef UP
+  if ((rc = upload(h, b, H.obs_orig, &b->obs_orig)) != SBA_OK) {
+    free_batch(b);
+    return rc;
+  }
 #define AL(count, field)                                \
   if ((rc = alloc(h, b, count, &d.field)) != SBA_OK) { \
     free_batch(b);                                      \
@@ -461,7 +477,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   d.lds_zc = off;
   off += kKt * kZcLd;
   d.lds_words = off;
-  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + max_lds_packed);
+  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + 256 + max_lds_packed);
   if (static_cast<int64_t>(d.lds_words) * 4 + 64 > 160 * 1024)
     return free_batch(b), fail(h, SBA_E_CAPACITY, "Schur partial does not fit in shared memory");
   for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e));
@@ -487,6 +503,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
 int sba_batch_run(sba_handle* h, sba_batch* b) {
   if (!h || !b) return SBA_E_INVALID;
   HIP_TRY(hipSetDevice(h->opt.device));
+  b->adj_ready = false;
   if (b->gexec) {
     HIP_TRY(hipGraphLaunch(b->gexec, h->stream));
     return SBA_OK;
@@ -585,6 +602,7 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
                          double* scalars) {
   if (!h || !b) return SBA_E_INVALID;
   HIP_TRY(hipSetDevice(h->opt.device));
+  b->adj_ready = false;
   Batch d = b->dev;
   const WindowDesc& W = b->win[0];
   hipStream_t s = h->stream;
@@ -679,7 +697,7 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
   W.no = 1;
   W.max_iterations = 1;
   const int64_t packed = static_cast<int64_t>(n) * (n + 1) / 2;
-  W.factor_in_lds = (512 + 256 * kPanel + 48 + packed) * 8 <= 160 * 1024 ? 1 : 0;
+  W.factor_in_lds = (512 + 256 * kPanel + 48 + 256 + packed) * 8 <= 160 * 1024 ? 1 : 0;
   std::vector<float4> poses(3 * ncf);
   for (int j = 0; j < ncf; ++j) {
     poses[3 * j] = make_float4(1, 0, 0, 0);
@@ -700,7 +718,7 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
     free_batch(b);
     return rc;
   }
-  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + (W.factor_in_lds ? static_cast<int>(packed) : 0));
+  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + 256 + (W.factor_in_lds ? static_cast<int>(packed) : 0));
   if (launch_factor(d, nullptr, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
     free_batch(b);
     return fail(h, SBA_E_DEVICE, "factor launch failed");
@@ -755,6 +773,194 @@ int sba_debug_copy(sba_handle* h, int64_t bytes, int32_t repeats, float* ms) {
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
   (void)hipFree(c);
+  return SBA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Workspace of the implicit-differentiation path of window 0 (FP64, one allocation).
+struct AdjWork {
+  int n6, nx, no;
+  double *x, *r, *z, *p, *y, *ec, *zc, *acc, *gd, *sc, *Lst, *dxy, *dinfo, *spts, *sposes;
+  int32_t* done;
+};
+
+int adj_workspace(sba_handle* h, sba_batch* b, AdjWork& A) {
+  const WindowDesc& W = b->win[0];
+  A.n6 = 6 * W.ncf;
+  A.nx = A.n6 + 3 * W.npf;
+  A.no = W.no;
+  const int64_t npk = static_cast<int64_t>(A.n6) * (A.n6 + 1) / 2;
+  // x r z p y ec zc acc g (nx each), sc (8), done (2), factor (npk + n6), dxy (2 no), dinfo (3 no),
+  // FP64 points (3 np) and key frames (12 nc)
+  const size_t nx = static_cast<size_t>(A.nx);
+  const size_t nd = 9 * nx + 8 + 2 + static_cast<size_t>(npk + A.n6) + 5 * static_cast<size_t>(A.no) +
+                    3 * static_cast<size_t>(W.np) + 12 * static_cast<size_t>(W.nc);
+  if (!b->adj) {
+    HIP_TRY(hipMalloc(&b->adj, nd * sizeof(double)));
+    b->allocs.push_back(b->adj);
+  }
+  A.x = b->adj;
+  A.r = A.x + nx;
+  A.z = A.r + nx;
+  A.p = A.z + nx;
+  A.y = A.p + nx;
+  A.ec = A.y + nx;
+  A.zc = A.ec + nx;
+  A.acc = A.zc + nx;
+  A.gd = A.acc + nx;
+  A.sc = A.gd + nx;
+  A.done = reinterpret_cast<int32_t*>(A.sc + 8);
+  A.Lst = A.sc + 10;
+  A.dxy = A.Lst + npk + A.n6;
+  A.dinfo = A.dxy + 2 * static_cast<size_t>(A.no);
+  A.spts = A.dinfo + 3 * static_cast<size_t>(A.no);
+  A.sposes = A.spts + 3 * static_cast<size_t>(W.np);
+  return SBA_OK;
+}
+
+// Factors the Gauss-Newton system of the solution of window 0 (lambda = 0) once and copies its state to FP64.
+// The batch's LM state is left as it was.
+int adj_prepare(sba_handle* h, sba_batch* b, AdjWork& A) {
+  int rc = adj_workspace(h, b, A);
+  if (rc != SBA_OK || b->adj_ready) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  Batch d = b->dev;
+  hipStream_t s = h->stream;
+  WindowState st0;
+  HIP_TRY(hipMemcpy(&st0, d.st, sizeof(st0), hipMemcpyDeviceToHost));
+  WindowState st1 = st0;
+  st1.lambda = 0.f;
+  st1.done = 0;
+  HIP_TRY(hipMemcpy(d.st, &st1, sizeof(st1), hipMemcpyHostToDevice));
+  std::vector<WindowState> others;
+  if (b->n > 1) {  // only window 0 runs: mark the others done for these launches
+    others.resize(b->n - 1);
+    HIP_TRY(hipMemcpy(others.data(), d.st + 1, sizeof(WindowState) * (b->n - 1), hipMemcpyDeviceToHost));
+    std::vector<WindowState> tmp = others;
+    for (auto& t : tmp) t.done = 1;
+    HIP_TRY(hipMemcpy(d.st + 1, tmp.data(), sizeof(WindowState) * (b->n - 1), hipMemcpyHostToDevice));
+  }
+  WindowState chk{};
+  if (launch_adj_state(d, A.spts, A.sposes, nullptr, 0, s) != hipSuccess || launch_schur(d, s) != hipSuccess ||
+      launch_reduce(d, s) != hipSuccess || launch_factor(d, nullptr, s, A.Lst) != hipSuccess) {
+    rc = fail(h, SBA_E_DEVICE, "adjoint factor launch failed");
+  } else if (hipMemcpyAsync(&chk, d.st, sizeof(chk), hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess) {
+    rc = fail(h, SBA_E_DEVICE, "adjoint state read failed");
+  } else if (!chk.chol_ok) {
+    rc = fail(h, SBA_E_INFEASIBLE, "the Gauss-Newton system of the solution is not positive definite");
+  }
+  HIP_TRY(hipMemcpy(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice));
+  if (!others.empty())
+    HIP_TRY(hipMemcpy(d.st + 1, others.data(), sizeof(WindowState) * others.size(), hipMemcpyHostToDevice));
+  if (rc == SBA_OK) b->adj_ready = true;
+  return rc;
+}
+
+// x = H^-1 gd by preconditioned conjugate gradients (enqueued; the iteration count lands in sc[3]).
+hipError_t adj_pcg(const Batch& d, const AdjWork& A, int max_cg, double tol, hipStream_t s) {
+  auto vec = [&](int op) {
+    return launch_adj_vec(d, A.x, A.r, A.z, A.p, A.y, A.ec, A.zc, A.acc, A.gd, A.sc, A.done, tol, A.nx, A.n6, op, s);
+  };
+  auto precond = [&]() {
+    hipError_t e = launch_adj_obs(d, 1, A.spts, A.sposes, A.r, nullptr, A.ec, nullptr, nullptr, nullptr, nullptr, A.done, s);
+    if (e == hipSuccess) e = launch_adj_chol(A.Lst, A.n6, A.ec, A.zc, A.done, s);
+    if (e == hipSuccess)
+      e = launch_adj_obs(d, 2, A.spts, A.sposes, nullptr, A.acc, nullptr, A.zc, nullptr, nullptr, nullptr, A.done, s);
+    if (e == hipSuccess) e = vec(1);
+    return e;
+  };
+  hipError_t e = hipMemsetAsync(A.done, 0, sizeof(int32_t), s);
+  if (e == hipSuccess) e = vec(0);
+  if (e == hipSuccess) e = precond();
+  if (e == hipSuccess) e = vec(2);
+  for (int it = 0; it < max_cg && e == hipSuccess; ++it) {
+    e = launch_adj_obs(d, 0, A.spts, A.sposes, A.p, A.y, nullptr, nullptr, nullptr, nullptr, nullptr, A.done, s);
+    if (e == hipSuccess) e = vec(3);
+    if (e == hipSuccess) e = precond();
+    if (e == hipSuccess) e = vec(4);
+  }
+  return e;
+}
+
+// gd = gradient of the cost at the FP64 state.
+hipError_t adj_grad(const Batch& d, const AdjWork& A, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(A.gd, 0, sizeof(double) * A.nx, s);
+  if (e == hipSuccess)
+    e = launch_adj_obs(d, 4, A.spts, A.sposes, nullptr, A.gd, nullptr, nullptr, nullptr, nullptr, nullptr, A.done, s);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t newton_steps, int32_t max_cg, double tol, double* points,
+                     double* rig_from_world, double* grad_norm) {
+  if (!h || !b || newton_steps < 0 || max_cg < 1) return fail(h, SBA_E_INVALID, "bad sba_batch_refine arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  AdjWork A{};
+  int rc = adj_prepare(h, b, A);
+  if (rc != SBA_OK) return rc;
+  const Batch d = b->dev;
+  const WindowDesc& W = b->win[0];
+  hipStream_t s = h->stream;
+  // Newton steps x <- x - H^-1 grad, full FP64 Hessian, the stored factor as preconditioner
+  for (int k = 0; k < newton_steps; ++k) {
+    HIP_TRY(adj_grad(d, A, s));
+    HIP_TRY(adj_pcg(d, A, max_cg, tol, s));
+    HIP_TRY(launch_adj_state(d, A.spts, A.sposes, A.x, 1, s));
+  }
+  if (grad_norm) {
+    HIP_TRY(adj_grad(d, A, s));
+    std::vector<double> g(A.nx);
+    HIP_TRY(hipMemcpyAsync(g.data(), A.gd, sizeof(double) * A.nx, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double n2 = 0.0;
+    for (double v : g) n2 += v * v;
+    *grad_norm = std::sqrt(n2);
+  }
+  if (points || rig_from_world) {
+    std::vector<double> P(3 * static_cast<size_t>(W.np)), T(12 * static_cast<size_t>(W.nc));
+    HIP_TRY(hipMemcpyAsync(P.data(), A.spts, sizeof(double) * P.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(T.data(), A.sposes, sizeof(double) * T.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (points) std::copy(P.begin(), P.end(), points);
+    if (rig_from_world)
+      for (int j = 0; j < W.nc; ++j) {
+        double* M = rig_from_world + 16 * static_cast<size_t>(j);  // column-major 4x4
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 4; ++c) M[4 * c + r] = T[12 * static_cast<size_t>(j) + 4 * r + c];
+        M[3] = M[7] = M[11] = 0.0;
+        M[15] = 1.0;
+      }
+  }
+  return SBA_OK;
+}
+
+int sba_batch_adjoint(sba_handle* h, sba_batch* b, const double* g, double* dxy, double* dinfo, int32_t max_cg,
+                      double tol, int32_t* cg_iterations, double* a_out) {
+  if (!h || !b || !g || !dxy || !dinfo || max_cg < 1) return fail(h, SBA_E_INVALID, "bad sba_batch_adjoint arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  AdjWork A{};
+  int rc = adj_prepare(h, b, A);
+  if (rc != SBA_OK) return rc;
+  const Batch d = b->dev;
+  hipStream_t s = h->stream;
+  HIP_TRY(hipMemcpyAsync(A.gd, g, sizeof(double) * A.nx, hipMemcpyHostToDevice, s));
+  HIP_TRY(adj_pcg(d, A, max_cg, tol, s));
+  // per-observation gradients from the adjoint
+  HIP_TRY(launch_adj_obs(d, 3, A.spts, A.sposes, A.x, nullptr, nullptr, nullptr, A.dxy, A.dinfo, b->obs_orig, A.done, s));
+  double scal[8];
+  HIP_TRY(hipMemcpyAsync(scal, A.sc, sizeof(scal), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(dxy, A.dxy, sizeof(double) * 2 * A.no, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(dinfo, A.dinfo, sizeof(double) * 3 * A.no, hipMemcpyDeviceToHost, s));
+  if (a_out) HIP_TRY(hipMemcpyAsync(a_out, A.x, sizeof(double) * A.nx, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (cg_iterations) *cg_iterations = static_cast<int32_t>(scal[3]);
   return SBA_OK;
 }
 

@@ -245,6 +245,36 @@ class DeviceBatch:
             raise SbaError(rc, self.solver.last_error())
         return S, e, dc, dp, sc, rc
 
+    def refine(self, newton_steps: int = 3, max_cg: int = 200, tol: float = 1e-12):
+        """FP64 Newton refinement of the solved state of window 0 (the state ``adjoint`` then differentiates at).
+        Returns (rig_from_world (N_c, 4, 4) float64, points (N_p, 3) float64, |grad F| after the last step)."""
+        p = self.problems[0]
+        pts = np.zeros((p.points.shape[0], 3))
+        T = np.zeros((p.rig_from_world.shape[0], 16))
+        gn = C.c_double()
+        d = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        rc = self.solver._L.sba_batch_refine(self.solver._h, self._b, newton_steps, max_cg, tol, d(pts), d(T),
+                                             C.byref(gn))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.solver.last_error())
+        return T.reshape(-1, 4, 4).transpose(0, 2, 1).copy(), pts, gn.value
+
+    def adjoint(self, g, max_cg: int = 200, tol: float = 1e-10):
+        """Implicit gradient through the solved state of window 0: g is dL/dx in tangent coordinates (left twist
+        (omega, v) per free key frame, then each free point). Returns (dL/dxy (N_o, 2), dL/dinfo (N_o, 3) as
+        (L00, L01 shared, L11), conjugate-gradient iterations, adjoint a)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        p = self.problems[0]
+        n = p.num_observations
+        dxy, dinfo, a = np.zeros((n, 2)), np.zeros((n, 3)), np.zeros_like(g)
+        it = C.c_int32()
+        d = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        rc = self.solver._L.sba_batch_adjoint(self.solver._h, self._b, d(g), d(dxy), d(dinfo), max_cg, tol,
+                                              C.byref(it), d(a))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.solver.last_error())
+        return dxy, dinfo, it.value, a
+
     def close(self):
         if getattr(self, "_b", None):
             self.solver._L.sba_batch_destroy(self.solver._h, self._b)

@@ -136,6 +136,31 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc,
                          double* dp, double* scalars);
 
+/* FP64 Newton refinement of the solved state of window 0 of a batch, for the differentiable path: the FP32 LM
+ * solve stops at its own rounding floor, and an implicit gradient taken there carries that residual gradient.
+ * Each step solves H a = grad F with the full FP64 Hessian (the conjugate gradients of sba_batch_adjoint) and
+ * applies x <- x (-) a (T_j <- Exp(-a_j) T_j, p_i <- p_i - a_i). The refined state stays with the batch and is
+ * the state sba_batch_adjoint differentiates at, until the next sba_batch_run; the batch's FP32 state and its
+ * sba_batch_read results are unchanged. newton_steps = 0 only copies the FP32 state. Outputs (any may be null):
+ *   points         [num_points][3] FP64
+ *   rig_from_world [num_poses][16] FP64 4x4, column-major (the Eigen layout of Isometry3T::matrix())
+ *   grad_norm      |grad F| at the refined state.
+ * No reference counterpart. */
+int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t newton_steps, int32_t max_cg, double tol, double* points,
+                     double* rig_from_world, double* grad_norm);
+
+/* Implicit gradient through the solved state of window 0 of a batch (run sba_batch_run first; the solve should
+ * have converged, and sba_batch_refine polishes it in FP64). g is dL/dx for a loss L of the solution in the solver's tangent coordinates: a left twist
+ * (omega, v) of each free key frame (6 each, stored order), then each free point (3 each). Solves H a = g with
+ * H the full FP64 Hessian of the robust cost at the solution (preconditioned conjugate gradients, the
+ * preconditioner being the Gauss-Newton system factored once through its reduced camera system), and returns
+ *   dxy   [num_observations][2] dL/d observation_xys, in the input observation order
+ *   dinfo [num_observations][3] dL/d observation_infos as (L00, L01 = L10 shared, L11)
+ *   a_out [optional] the adjoint a.
+ * No reference counterpart (the reference has no differentiation). */
+int sba_batch_adjoint(sba_handle* h, sba_batch* b, const double* g, double* dxy, double* dinfo, int32_t max_cg,
+                      double tol, int32_t* cg_iterations, double* a_out);
+
 /* Factor/solve of one symmetric n x n system (n a multiple of 6, n <= 240) with the reduced-camera-system
  * kernel: A row-major (only its lower triangle is read), b [n] -> x [n]. Returns SBA_OK, or
  * SBA_E_INFEASIBLE when A is not positive definite (the reference's devInfo != 0,

@@ -292,9 +292,22 @@ struct Solver {
   std::vector<T> pts;
   std::vector<Pose<T>> poses;
   std::vector<Pose<T>> cams;
+  std::vector<T> xy_, info_;  // observations in the solver's precision (info column-major 2x2)
   const OrcProblem* P;
   OrcOptions opt;
   T delta;
+
+  // replaces the FP32 state of the problem by an FP64 one (points [np][3], poses [nc][16] column-major)
+  void override_state(const double* p, const double* m) {
+    if (p)
+      for (size_t i = 0; i < pts.size(); ++i) pts[i] = static_cast<T>(p[i]);
+    if (m)
+      for (int j = 0; j < nc; ++j)
+        for (int r = 0; r < 3; ++r) {
+          for (int c = 0; c < 3; ++c) poses[j].R[r][c] = static_cast<T>(m[16 * j + c * 4 + r]);
+          poses[j].t[r] = static_cast<T>(m[16 * j + 12 + r]);
+        }
+  }
 
   void load(const OrcProblem* p, const OrcOptions* o) {
     P = p;
@@ -312,9 +325,21 @@ struct Solver {
     cams.resize(ncam);
     for (int c = 0; c < ncam; ++c) cams[c] = load_pose<T>(p->camera_from_rig + 16 * c);
     delta = static_cast<T>(p->robustifier_scale);
+    xy_.resize(2 * static_cast<size_t>(no));
+    info_.resize(4 * static_cast<size_t>(no));
+    for (size_t k = 0; k < xy_.size(); ++k) xy_[k] = static_cast<T>(p->observation_xys[k]);
+    for (size_t k = 0; k < info_.size(); ++k) info_[k] = static_cast<T>(p->observation_infos[k]);
   }
 
-  T info(int o, int r, int c) const { return static_cast<T>(P->observation_infos[4 * o + c * 2 + r]); }
+  // replaces the observations by FP64 ones (xy [no][2], info [no][4] column-major)
+  void override_observations(const double* xy, const double* inf) {
+    if (xy)
+      for (size_t k = 0; k < xy_.size(); ++k) xy_[k] = static_cast<T>(xy[k]);
+    if (inf)
+      for (size_t k = 0; k < info_.size(); ++k) info_[k] = static_cast<T>(inf[k]);
+  }
+
+  T info(int o, int r, int c) const { return info_[4 * o + c * 2 + r]; }
 
   // UpdateModel, cpu.cpp:64-131.
   void update_model(Model<T>& m) const {
@@ -330,8 +355,8 @@ struct Solver {
       apply(Tcw, &pts[3 * pi], pc);
       if (!(pc[2] > T(kHither))) continue;
       const T pred[2] = {pc[0] / pc[2], pc[1] / pc[2]};
-      const T r[2] = {static_cast<T>(P->observation_xys[2 * o]) - pred[0],
-                      static_cast<T>(P->observation_xys[2 * o + 1]) - pred[1]};
+      const T r[2] = {xy_[2 * o] - pred[0],
+                      xy_[2 * o + 1] - pred[1]};
       const T iz = T(1) / pc[2];
       const T dproj[2][3] = {{iz, T(0), -pred[0] * iz}, {T(0), iz, -pred[1] * iz}};
       T* Jp = &m.Jp[6 * o];
@@ -558,8 +583,8 @@ struct Solver {
         std::copy(pr0, pr0 + 3, pr);
       apply(cams[cam], pr, pc);
       if (pc[2] > T(kHither)) {
-        const T r[2] = {static_cast<T>(P->observation_xys[2 * o]) - pc[0] / pc[2],
-                        static_cast<T>(P->observation_xys[2 * o + 1]) - pc[1] / pc[2]};
+        const T r[2] = {xy_[2 * o] - pc[0] / pc[2],
+                        xy_[2 * o + 1] - pc[1] / pc[2]};
         const T sq = r[0] * (info(o, 0, 0) * r[0] + info(o, 0, 1) * r[1]) + r[1] * (info(o, 1, 0) * r[0] + info(o, 1, 1) * r[1]);
         cost += loss_rho<T>(opt.loss, sq, delta);
       } else {
@@ -703,11 +728,280 @@ struct Solver {
   }
 };
 
+// ---------------------------------------------------------------------------------------------------------
+// Implicit gradient of a loss of the converged state (no reference counterpart; the reference's analytic
+// Jacobians of cpu.cpp:101-118 and the information conversion of camera/observation.cpp:57-67 are the anchors).
+// At a solution x* the gradient of F(x) = sum rho(r^T L r) vanishes, so for parameters theta of the
+// observations (xy, L): dx*/dtheta = -H^-1 d(grad F)/dtheta, with H the full Hessian of F at x*, and
+// dL/dtheta = -a^T d(grad F)/dtheta where H a = dL/dx. H includes the rho'' terms and the second-order terms
+// of the projection and of the left-multiplied twist; H a = g is solved by preconditioned conjugate gradients,
+// the preconditioner being the Gauss-Newton system 2 sum rho' J^T L J solved through its Schur complement
+// (lambda = 0). Tangent coordinates: left twists (omega, v) of the free key frames, then the free points.
+struct ObsLin {
+  double r[2], J[2][9], L[2][2], w, w2, q0[3], Rcr[3][3], Rrw[3][3], P[3];
+  int pose, point;
+  bool valid;
+};
+
+double loss_d2rho(int kind, double s, double d) {
+  if (kind == 0) return -(d + 1.0) / ((d + s) * (d + s));
+  if (d <= 0.0 || s < d * d) return 0.0;
+  return -0.25 * d / (s * std::sqrt(s));
+}
+
+struct Adjoint {
+  Solver<double>& S;
+  int nx, ncf, npf;
+  std::vector<ObsLin> obs;
+  Full<double> F;
+  Solver<double>::Reduced R;
+
+  explicit Adjoint(Solver<double>& s) : S(s) {
+    ncf = S.ncf;
+    npf = S.npf;
+    nx = 6 * ncf + 3 * npf;
+    obs.resize(S.no);
+    for (int o = 0; o < S.no; ++o) {
+      ObsLin& L = obs[o];
+      L.pose = S.P->pose_ids[o];
+      L.point = S.P->point_ids[o];
+      const int cam = S.P->camera_ids[o];
+      const Pose<double>& Tcr = S.cams[cam];
+      const Pose<double>& Trw = S.poses[L.pose];
+      apply(Trw, &S.pts[3 * L.point], L.q0);
+      apply(Tcr, L.q0, L.P);
+      std::memcpy(L.Rcr, Tcr.R, sizeof(L.Rcr));
+      std::memcpy(L.Rrw, Trw.R, sizeof(L.Rrw));
+      L.valid = L.P[2] > double(kHither);
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) L.L[a][b] = S.info(o, a, b);
+      std::memset(L.J, 0, sizeof(L.J));
+      L.w = L.w2 = 0;
+      L.r[0] = L.r[1] = 0;
+      if (!L.valid) continue;
+      const double iz = 1.0 / L.P[2], u = L.P[0] * iz, v = L.P[1] * iz;
+      L.r[0] = S.xy_[2 * o] - u;
+      L.r[1] = S.xy_[2 * o + 1] - v;
+      const double dproj[2][3] = {{iz, 0, -u * iz}, {0, iz, -v * iz}};
+      // D1 = R_cr [-[q0]x, I, R_rw]
+      const double mq[3][3] = {{0, L.q0[2], -L.q0[1]}, {-L.q0[2], 0, L.q0[0]}, {L.q0[1], -L.q0[0], 0}};
+      double D1[3][9];
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 3; ++k) {
+          double s1 = 0, s3 = 0;
+          for (int m = 0; m < 3; ++m) {
+            s1 += L.Rcr[a][m] * mq[m][k];
+            s3 += L.Rcr[a][m] * L.Rrw[m][k];
+          }
+          D1[a][k] = s1;
+          D1[a][3 + k] = L.Rcr[a][k];
+          D1[a][6 + k] = s3;
+        }
+      for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < 9; ++k) L.J[a][k] = dproj[a][0] * D1[0][k] + dproj[a][1] * D1[1][k] + dproj[a][2] * D1[2][k];
+      const double Lr0 = L.L[0][0] * L.r[0] + L.L[0][1] * L.r[1], Lr1 = L.L[1][0] * L.r[0] + L.L[1][1] * L.r[1];
+      const double sq = L.r[0] * Lr0 + L.r[1] * Lr1;
+      L.w = loss_drho<double>(S.opt.loss, sq, S.delta);
+      L.w2 = loss_d2rho(S.opt.loss, sq, S.delta);
+    }
+    Model<double> m;
+    S.update_model(m);
+    S.build_full(F, m);
+    S.build_reduced(R, F, 0.0);
+  }
+
+  // gather the observation's 9 tangent entries of a global vector (zero for fixed entries)
+  void gather(const ObsLin& L, const double* x, double v[9]) const {
+    for (int k = 0; k < 9; ++k) v[k] = 0;
+    if (L.pose < ncf)
+      for (int k = 0; k < 6; ++k) v[k] = x[6 * L.pose + k];
+    if (L.point < npf)
+      for (int k = 0; k < 3; ++k) v[6 + k] = x[6 * ncf + 3 * L.point + k];
+  }
+  void scatter(const ObsLin& L, const double h[9], double* y) const {
+    if (L.pose < ncf)
+      for (int k = 0; k < 6; ++k) y[6 * L.pose + k] += h[k];
+    if (L.point < npf)
+      for (int k = 0; k < 3; ++k) y[6 * ncf + 3 * L.point + k] += h[6 + k];
+  }
+
+  // y = H x with the full Hessian of F
+  void hvp(const double* x, double* y) const {
+    std::fill(y, y + nx, 0.0);
+    for (const ObsLin& L : obs) {
+      if (!L.valid) continue;
+      double v[9];
+      gather(L, x, v);
+      double u[2] = {0, 0};
+      for (int k = 0; k < 9; ++k) {
+        u[0] += L.J[0][k] * v[k];
+        u[1] += L.J[1][k] * v[k];
+      }
+      const double Lu[2] = {L.L[0][0] * u[0] + L.L[0][1] * u[1], L.L[1][0] * u[0] + L.L[1][1] * u[1]};
+      const double Lr[2] = {L.L[0][0] * L.r[0] + L.L[0][1] * L.r[1], L.L[1][0] * L.r[0] + L.L[1][1] * L.r[1]};
+      const double rLu = L.r[0] * Lu[0] + L.r[1] * Lu[1];
+      double h[9];
+      for (int k = 0; k < 9; ++k) {
+        const double JLr = L.J[0][k] * Lr[0] + L.J[1][k] * Lr[1];
+        h[k] = 2 * L.w * (L.J[0][k] * Lu[0] + L.J[1][k] * Lu[1]) + 4 * L.w2 * JLr * rLu;
+      }
+      // - 2 Hess(psi) v with psi = c^T pi(p_c), c = rho' L r
+      const double c[2] = {L.w * Lr[0], L.w * Lr[1]};
+      const double iz = 1.0 / L.P[2];
+      const double g3[3] = {c[0] * iz, c[1] * iz, -(c[0] * L.P[0] + c[1] * L.P[1]) * iz * iz};
+      const double Hp[3][3] = {{0, 0, -c[0] * iz * iz},
+                               {0, 0, -c[1] * iz * iz},
+                               {-c[0] * iz * iz, -c[1] * iz * iz, 2 * (c[0] * L.P[0] + c[1] * L.P[1]) * iz * iz * iz}};
+      // D1 v and D1^T Hp (D1 v)
+      const double mq[3][3] = {{0, L.q0[2], -L.q0[1]}, {-L.q0[2], 0, L.q0[0]}, {L.q0[1], -L.q0[0], 0}};
+      double D1[3][9];
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 3; ++k) {
+          double s1 = 0, s3 = 0;
+          for (int m = 0; m < 3; ++m) {
+            s1 += L.Rcr[a][m] * mq[m][k];
+            s3 += L.Rcr[a][m] * L.Rrw[m][k];
+          }
+          D1[a][k] = s1;
+          D1[a][3 + k] = L.Rcr[a][k];
+          D1[a][6 + k] = s3;
+        }
+      double dv[3] = {0, 0, 0};
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 9; ++k) dv[a] += D1[a][k] * v[k];
+      double hd[3];
+      for (int a = 0; a < 3; ++a) hd[a] = Hp[a][0] * dv[0] + Hp[a][1] * dv[1] + Hp[a][2] * dv[2];
+      double hpsi[9];
+      for (int k = 0; k < 9; ++k) hpsi[k] = D1[0][k] * hd[0] + D1[1][k] * hd[1] + D1[2][k] * hd[2];
+      // second-order part of p_r along v: g' = R_cr^T g3
+      double gp[3];
+      for (int m = 0; m < 3; ++m) gp[m] = L.Rcr[0][m] * g3[0] + L.Rcr[1][m] * g3[1] + L.Rcr[2][m] * g3[2];
+      const double* wv = v;
+      const double* vv = v + 3;
+      double Rdp[3];
+      for (int m = 0; m < 3; ++m) Rdp[m] = L.Rrw[m][0] * v[6] + L.Rrw[m][1] * v[7] + L.Rrw[m][2] * v[8];
+      auto cross = [](const double a[3], const double b[3], double o[3]) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+      };
+      double wxq[3], avec[3], gxw[3], t1[3], t2[3];
+      cross(wv, L.q0, wxq);
+      for (int m = 0; m < 3; ++m) avec[m] = 0.5 * wxq[m] + 0.5 * vv[m] + Rdp[m];
+      cross(avec, gp, t1);      // grad_omega of g'.(omega x a)
+      cross(gp, wv, gxw);       // g' x omega_v
+      cross(L.q0, gxw, t2);     // grad_omega of g'.(omega_v x (omega x q0)) / 2 * 2
+      for (int m = 0; m < 3; ++m) hpsi[m] += t1[m] + 0.5 * t2[m];
+      for (int m = 0; m < 3; ++m) hpsi[3 + m] += 0.5 * gxw[m];
+      for (int m = 0; m < 3; ++m) hpsi[6 + m] += L.Rrw[0][m] * gxw[0] + L.Rrw[1][m] * gxw[1] + L.Rrw[2][m] * gxw[2];
+      if (L.pose >= ncf)
+        for (int k = 0; k < 6; ++k) hpsi[k] = 0;
+      if (L.point >= npf)
+        for (int k = 0; k < 3; ++k) hpsi[6 + k] = 0;
+      for (int k = 0; k < 9; ++k) h[k] -= 2 * hpsi[k];
+      scatter(L, h, y);
+    }
+  }
+
+  // z = M^-1 q with M = 2 sum rho' J^T L J (the LM system at lambda = 0, doubled), via its Schur complement
+  bool precond(const double* q, double* z) const {
+    const int n = 6 * ncf;
+    std::vector<double> e(q, q + n), zc(n, 0.0);
+    std::vector<double> t(3 * static_cast<size_t>(npf));
+    for (int i = 0; i < npf; ++i)
+      for (int a = 0; a < 3; ++a)
+        t[3 * i + a] = R.inv[9 * i + 3 * a] * q[n + 3 * i] + R.inv[9 * i + 3 * a + 1] * q[n + 3 * i + 1] +
+                       R.inv[9 * i + 3 * a + 2] * q[n + 3 * i + 2];
+    for (int i = 0; i < npf; ++i)
+      for (auto& y : R.Y[i])
+        for (int a = 0; a < 6; ++a)
+          e[6 * y.first + a] -= y.second[3 * a] * q[n + 3 * i] + y.second[3 * a + 1] * q[n + 3 * i + 1] +
+                                y.second[3 * a + 2] * q[n + 3 * i + 2];
+    if (n > 0 && !chol_solve<double>(n, R.S.data(), e.data(), zc.data())) return false;
+    for (int k = 0; k < n; ++k) z[k] = 0.5 * zc[k];
+    for (int i = 0; i < npf; ++i) {
+      double d[3] = {t[3 * i], t[3 * i + 1], t[3 * i + 2]};
+      for (auto& y : R.Y[i])
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < 6; ++a) d[c] -= y.second[a * 3 + c] * zc[6 * y.first + a];
+      for (int c = 0; c < 3; ++c) z[n + 3 * i + c] = 0.5 * d[c];
+    }
+    return true;
+  }
+
+  // preconditioned conjugate gradients on H a = g; returns iterations used (negative on failure)
+  int pcg(const double* g, double* a, int max_it, double tol) const {
+    std::vector<double> r(g, g + nx), z(nx), p(nx), q(nx);
+    std::fill(a, a + nx, 0.0);
+    if (!precond(r.data(), z.data())) return -1;
+    p = z;
+    double rz = 0, gg = 0;
+    for (int k = 0; k < nx; ++k) {
+      rz += r[k] * z[k];
+      gg += g[k] * g[k];
+    }
+    if (gg == 0) return 0;
+    for (int it = 1; it <= max_it; ++it) {
+      hvp(p.data(), q.data());
+      double pq = 0;
+      for (int k = 0; k < nx; ++k) pq += p[k] * q[k];
+      const double alpha = rz / pq;
+      double rr = 0;
+      for (int k = 0; k < nx; ++k) {
+        a[k] += alpha * p[k];
+        r[k] -= alpha * q[k];
+        rr += r[k] * r[k];
+      }
+      if (std::sqrt(rr / gg) < tol) return it;
+      precond(r.data(), z.data());
+      double rz2 = 0;
+      for (int k = 0; k < nx; ++k) rz2 += r[k] * z[k];
+      const double beta = rz2 / rz;
+      rz = rz2;
+      for (int k = 0; k < nx; ++k) p[k] = z[k] + beta * p[k];
+    }
+    return max_it;
+  }
+
+  // dL/dxy (2 per observation) and dL/dL (3 per observation: L00, symmetric L01, L11) from the adjoint a
+  void observation_grads(const double* a, double* dxy, double* dinfo) const {
+    for (int o = 0; o < S.no; ++o) {
+      const ObsLin& L = obs[o];
+      double* gx = dxy + 2 * o;
+      double* gi = dinfo + 3 * o;
+      gx[0] = gx[1] = gi[0] = gi[1] = gi[2] = 0;
+      if (!L.valid) continue;
+      double v[9];
+      gather(L, a, v);
+      double Ja[2] = {0, 0};
+      for (int k = 0; k < 9; ++k) {
+        Ja[0] += L.J[0][k] * v[k];
+        Ja[1] += L.J[1][k] * v[k];
+      }
+      const double LJa[2] = {L.L[0][0] * Ja[0] + L.L[0][1] * Ja[1], L.L[1][0] * Ja[0] + L.L[1][1] * Ja[1]};
+      const double Lr[2] = {L.L[0][0] * L.r[0] + L.L[0][1] * L.r[1], L.L[1][0] * L.r[0] + L.L[1][1] * L.r[1]};
+      const double rLJa = L.r[0] * LJa[0] + L.r[1] * LJa[1];
+      gx[0] = 2 * (L.w * LJa[0] + 2 * L.w2 * rLJa * Lr[0]);
+      gx[1] = 2 * (L.w * LJa[1] + 2 * L.w2 * rLJa * Lr[1]);
+      // G = 2 [rho' (J a) r^T + rho'' ((J a)^T L r) r r^T]
+      double G[2][2];
+      for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 2; ++q) G[p][q] = 2 * (L.w * Ja[p] * L.r[q] + L.w2 * rLJa * L.r[p] * L.r[q]);
+      gi[0] = G[0][0];
+      gi[1] = G[0][1] + G[1][0];
+      gi[2] = G[1][1];
+    }
+  }
+};
+
 template <class T>
-int solve_impl(const OrcProblem* p, const OrcOptions* o, OrcResult* r) {
+int solve_impl(const OrcProblem* p, const OrcOptions* o, OrcResult* r, const double* sp = nullptr,
+               const double* sq = nullptr, const double* oxy = nullptr, const double* oinfo = nullptr) {
   if (!p || !o || !r) return 5;
   Solver<T> s;
   s.load(p, o);
+  s.override_state(sp, sq);
+  s.override_observations(oxy, oinfo);
   const int status = s.solve(r);
   r->status = status;
   s.export_state(r);
@@ -719,6 +1013,11 @@ int solve_impl(const OrcProblem* p, const OrcOptions* o, OrcResult* r) {
 extern "C" {
 
 int orc_solve_f64(const OrcProblem* p, const OrcOptions* o, OrcResult* r) { return solve_impl<double>(p, o, r); }
+// The FP64 solve from an FP64 initial state (points [np][3], poses [nc][16] column-major).
+int orc_solve_state_f64(const OrcProblem* p, const OrcOptions* o, const double* sp, const double* sq,
+                        const double* oxy, const double* oinfo, OrcResult* r) {
+  return solve_impl<double>(p, o, r, sp, sq, oxy, oinfo);
+}
 int orc_solve_f32(const OrcProblem* p, const OrcOptions* o, OrcResult* r) { return solve_impl<float>(p, o, r); }
 
 // Per-observation residual (2), pose Jacobian (2x6 row-major), point Jacobian (2x3), weight, at the input state.
@@ -800,6 +1099,50 @@ void orc_nearest_rotation_f64(const double* M9, double* R9) {
   for (int i = 0; i < 9; ++i) M[i / 3][i % 3] = M9[i];
   nearest_rotation<double>(M, R);
   for (int i = 0; i < 9; ++i) R9[i] = R[i / 3][i % 3];
+}
+
+// Implicit gradient at the input state (which should be a converged solution): g_tangent (6 per free key
+// frame, then 3 per free point) -> adjoint a, dL/dxy (2 per observation), dL/dinfo (3 per observation).
+// Returns the number of conjugate-gradient iterations (negative when the preconditioner failed).
+int orc_implicit_grad_f64(const OrcProblem* p, const OrcOptions* o, const double* sp, const double* sq,
+                          const double* oxy, const double* oinfo, const double* g, double* a, double* dxy,
+                          double* dinfo, int32_t max_it, double tol) {
+  Solver<double> s;
+  s.load(p, o);
+  s.override_state(sp, sq);
+  s.override_observations(oxy, oinfo);
+  Adjoint adj(s);
+  const int it = adj.pcg(g, a, max_it, tol);
+  if (it < 0) return it;
+  adj.observation_grads(a, dxy, dinfo);
+  return it;
+}
+
+// y = H x with the full Hessian of the cost at the input state (tangent coordinates as above).
+void orc_hvp_f64(const OrcProblem* p, const OrcOptions* o, const double* sp, const double* sq, const double* x, double* y) {
+  Solver<double> s;
+  s.load(p, o);
+  s.override_state(sp, sq);
+  Adjoint adj(s);
+  adj.hvp(x, y);
+}
+
+// Gradient of the cost (tangent coordinates), -2 sum rho' J^T L r.
+void orc_grad_f64(const OrcProblem* p, const OrcOptions* o, const double* sp, const double* sq, const double* oxy,
+                  const double* oinfo, double* g) {
+  Solver<double> s;
+  s.load(p, o);
+  s.override_state(sp, sq);
+  s.override_observations(oxy, oinfo);
+  Adjoint adj(s);
+  std::fill(g, g + adj.nx, 0.0);
+  for (const ObsLin& L : adj.obs) {
+    if (!L.valid) continue;
+    const double Lr[2] = {L.L[0][0] * L.r[0] + L.L[0][1] * L.r[1], L.L[1][0] * L.r[0] + L.L[1][1] * L.r[1]};
+    double h[9];
+    for (int k = 0; k < 9; ++k) h[k] = -2 * L.w * (L.J[0][k] * Lr[0] + L.J[1][k] * Lr[1]);
+    adj.scatter(L, h, g);
+  }
 }
 
 double orc_loss_f64(int32_t kind, double s, double d, int32_t derivative) {

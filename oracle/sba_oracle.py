@@ -61,6 +61,15 @@ def lib():
         L.orc_exp_f64.restype = None
         L.orc_nearest_rotation_f64.argtypes = [dp, dp]
         L.orc_nearest_rotation_f64.restype = None
+        L.orc_implicit_grad_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp, dp, dp, dp, dp, C.c_int32,
+                                            C.c_double]
+        L.orc_implicit_grad_f64.restype = C.c_int
+        L.orc_hvp_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp]
+        L.orc_hvp_f64.restype = None
+        L.orc_grad_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp, dp]
+        L.orc_grad_f64.restype = None
+        L.orc_solve_state_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp, C.POINTER(OrcResult)]
+        L.orc_solve_state_f64.restype = C.c_int
         L.orc_loss_f64.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32]
         L.orc_loss_f64.restype = C.c_double
         _lib = L
@@ -170,3 +179,76 @@ def nearest_rotation(M):
 
 def loss(kind, s, d, derivative=False):
     return lib().orc_loss_f64(kind, s, d, int(derivative))
+
+
+def _nx(problem):
+    ncf = problem.rig_from_world.shape[0] - problem.num_fixed_key_frames
+    npf = problem.points.shape[0] - problem.num_fixed_points
+    return 6 * ncf + 3 * npf
+
+
+def _state(points, poses):
+    """FP64 state overrides: points (N,3), poses (N,4,4) [row, col] -> C arrays (column-major poses)."""
+    if points is None:
+        return None, None, []
+    P = np.ascontiguousarray(points, dtype=np.float64)
+    Q = np.ascontiguousarray(np.asarray(poses, dtype=np.float64).transpose(0, 2, 1))
+    return _d(P), _d(Q), [P, Q]
+
+
+def _obs(xy, info):
+    """FP64 observation overrides: xy (N,2), info (N,2,2) [row, col] -> C arrays (column-major info)."""
+    keep = [None, None]
+    if xy is not None:
+        keep[0] = np.ascontiguousarray(xy, dtype=np.float64)
+    if info is not None:
+        keep[1] = np.ascontiguousarray(np.asarray(info, dtype=np.float64).transpose(0, 2, 1))
+    return (_d(keep[0]) if xy is not None else None), (_d(keep[1]) if info is not None else None), keep
+
+
+def grad(problem, points=None, poses=None, xy=None, info=None, **opt):
+    """Gradient of the cost in tangent coordinates (left twists of free key frames, then free points)."""
+    g = np.zeros(_nx(problem))
+    sp, sq, keep = _state(points, poses)
+    ox, oi, keep2 = _obs(xy, info)
+    lib().orc_grad_f64(C.byref(problem.view()), C.byref(options(**opt)), sp, sq, ox, oi, _d(g))
+    return g
+
+
+def hvp(problem, x, points=None, poses=None, **opt):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    sp, sq, keep = _state(points, poses)
+    lib().orc_hvp_f64(C.byref(problem.view()), C.byref(options(**opt)), sp, sq, _d(x), _d(y))
+    return y
+
+
+def implicit_grad(problem, g, points=None, poses=None, xy=None, info=None, max_it=200, tol=1e-12, **opt):
+    """(a, dL/dxy (N_o, 2), dL/dinfo (N_o, 3) [L00, L01 shared, L11], cg iterations) at the given state."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    a = np.zeros_like(g)
+    n = problem.num_observations
+    dxy, dinfo = np.zeros((n, 2)), np.zeros((n, 3))
+    sp, sq, keep = _state(points, poses)
+    ox, oi, keep2 = _obs(xy, info)
+    it = lib().orc_implicit_grad_f64(C.byref(problem.view()), C.byref(options(**opt)), sp, sq, ox, oi, _d(g), _d(a),
+                                     _d(dxy), _d(dinfo), max_it, tol)
+    return a, dxy, dinfo, it
+
+
+def solve_state(problem, points, poses, xy=None, info=None, trace=0, **opt):
+    """The FP64 solve from an FP64 initial state; returns final points and poses in FP64."""
+    L = lib()
+    v = problem.view()
+    o = options(**opt)
+    n_p, n_c = problem.points.shape[0], problem.rig_from_world.shape[0]
+    pts = np.zeros((n_p, 3))
+    pq = np.zeros((n_c, 16))
+    r = OrcResult()
+    r.points, r.poses = _d(pts), _d(pq)
+    r.trace_capacity = 0
+    sp, sq, keep = _state(points, poses)
+    ox, oi, keep2 = _obs(xy, info)
+    rc = L.orc_solve_state_f64(C.byref(v), C.byref(o), sp, sq, ox, oi, C.byref(r))
+    return dict(status=rc, points=pts, poses=pq.reshape(n_c, 4, 4).transpose(0, 2, 1), iterations=r.iterations,
+                last_cost=r.last_cost)

@@ -272,3 +272,83 @@ def test_fp32_reference_path_tracks_fp64_oracle():
     q.rig_from_world = b["poses"].astype(np.float32)
     c, _ = O.cost(q, loss=1)
     assert abs(c - a["last_cost"]) / a["last_cost"] < 1e-6
+
+
+def _retract(poses, pts, x, ncf, npf):
+    poses = poses.copy()
+    pts = pts.copy()
+    for j in range(ncf):
+        poses[j] = O.exp(x[6 * j:6 * j + 6]) @ poses[j]
+    pts[:npf] += x[6 * ncf:].reshape(-1, 3)
+    return poses, pts
+
+
+@pytest.mark.parametrize("loss", [ca.LOSS_HUBER, ca.LOSS_STUDENT_T])
+def test_cost_gradient_vs_central_differences(loss):
+    # grad F in the solver's tangent coordinates against central differences of the FP64 cost with the
+    # update applied the way the LM step applies it (left twist of each free key frame, point offset)
+    p, _, _ = ca.scene.generate(seed=3, num_points=40, num_keyframes=4, num_fixed_keyframes=2, pixel_noise=2.0)
+    ncf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
+    npf = p.points.shape[0] - p.num_fixed_points
+    g = O.grad(p, p.points.astype(np.float64), p.rig_from_world.astype(np.float64), loss=loss)
+    fd = np.zeros_like(g)
+    h = 1e-6
+    for k in range(g.size):
+        e = np.zeros(g.size)
+        e[k] = h
+        cp, _ = O.cost(p, dc=e[:6 * ncf], dp=e[6 * ncf:], loss=loss)
+        cm, _ = O.cost(p, dc=-e[:6 * ncf], dp=-e[6 * ncf:], loss=loss)
+        fd[k] = (cp - cm) / (2 * h)
+    assert rel(g, fd) < 1e-6
+
+
+def test_hessian_product_vs_central_differences_of_gradient():
+    # the full Hessian (rho'' and the second-order projection / twist terms) against central differences of
+    # grad F along a random tangent direction v, from a state off the optimum so that every term contributes.
+    # grad F at Exp(h v) T is taken in coordinates centred there: by Baker-Campbell-Hausdorff,
+    # log(Exp(e) Exp(h v)) = h v + (I - (h/2) ad_v) e, so the difference quotient of the key-frame rows is
+    # H v - (1/2) ad_v^T g with ad_(w,u) = [[w]x, 0; [u]x, [w]x]; the point rows are plain H v.
+    loss = ca.LOSS_STUDENT_T
+    p, _, _ = ca.scene.generate(seed=3, num_points=40, num_keyframes=4, num_fixed_keyframes=2, pixel_noise=2.0)
+    ncf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
+    npf = p.points.shape[0] - p.num_fixed_points
+    P0, T0 = p.points.astype(np.float64), p.rig_from_world.astype(np.float64)
+    v = np.random.default_rng(0).standard_normal(6 * ncf + 3 * npf)
+    Hv = O.hvp(p, v, P0, T0, loss=loss)
+    g = O.grad(p, P0, T0, loss=loss)
+    h = 1e-6
+    Tp, Pp = _retract(T0, P0, h * v, ncf, npf)
+    Tm, Pm = _retract(T0, P0, -h * v, ncf, npf)
+    fd = (O.grad(p, Pp, Tp, loss=loss) - O.grad(p, Pm, Tm, loss=loss)) / (2 * h)
+
+    def sk(a):
+        return np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+
+    expect = Hv.copy()
+    for j in range(ncf):
+        w, u = v[6 * j:6 * j + 3], v[6 * j + 3:6 * j + 6]
+        ad = np.zeros((6, 6))
+        ad[:3, :3], ad[3:, :3], ad[3:, 3:] = sk(w), sk(u), sk(w)
+        expect[6 * j:6 * j + 6] -= 0.5 * ad.T @ g[6 * j:6 * j + 6]
+    assert rel(expect[6 * ncf:], fd[6 * ncf:]) < 1e-6
+    assert rel(expect[:6 * ncf], fd[:6 * ncf]) < 1e-6
+    # without the second-order terms the products would be off by far more than that
+    assert rel(Hv[:6 * ncf], fd[:6 * ncf]) > 1e-3
+
+
+def test_implicit_gradient_vs_finite_differences_of_converged_solves():
+    # dL/d pixels and dL/d intrinsics from the oracle's adjoint at the FP64 stationary point, against central
+    # differences of fully converged FP64 solves (tests/adjoint_fd.py)
+    from adjoint_fd import K0, case, chain_to_pixels, obs
+    c = case()
+    pts, poses = c.optimum
+    xy, info = obs(c.uv, K0)
+    g0 = O.grad(c.problem, c.init_pts, c.init_poses, xy=xy, info=info, loss=c.loss)
+    assert np.linalg.norm(O.grad(c.problem, pts, poses, xy=xy, info=info, loss=c.loss)) < 1e-9 * np.linalg.norm(g0)
+    a, dxy, dinfo, it = O.implicit_grad(c.problem, c.g_of(poses), pts, poses, xy=xy, info=info, loss=c.loss)
+    assert it > 0
+    duv, dK = chain_to_pixels(dxy, dinfo, c.uv, K0)
+    fd_uv, fd_K = c.fd
+    assert rel(c.pick(duv), fd_uv) < 1e-4
+    for k in range(4):
+        assert abs(dK[k] - fd_K[k]) <= 1e-4 * abs(fd_K[k]), (k, dK[k], fd_K[k])
