@@ -72,7 +72,8 @@ SIGNATURES = {
     "sba_batch_debug_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
     "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     "sba_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
-    "sba_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
+    "sba_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp, _dp,
+                                   _dp]),
     "sba_debug_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _fp]),
     "sba_debug_mfma_layout": (C.c_int, [C.c_void_p, _fp, _fp, _fp]),
     "sba_scene_default_config": (None, [C.POINTER(SceneConfig)]),

@@ -397,6 +397,12 @@ __global__ __launch_bounds__(1024) void k_adj_vec(Batch b, Cg c, int nx, int n6,
     return;
   }
   if (op == 3) {
+    // negative or zero curvature along p: the Hessian is not positive definite here; stop with the current x
+    // (truncated conjugate gradients, as in Steihaug's method)
+    if (!(t0 > 0.0)) {
+      if (tid == 0) *c.done = 1;
+      return;
+    }
     const double alpha = c.sc[0] / t0;
     double rr = 0.0;
     for (int k = tid; k < nx; k += 1024) {

@@ -130,9 +130,39 @@ __device__ __forceinline__ void jrot(float a[3][3], float v[3][3], int p, int q)
 // Symmetric factor G of the point-block pseudo-inverse V*^+ = G G^T with the reference's rank rule
 // (cpu.cpp:218-262, sba_v1.cu:1016-1069): eigenvalues above 1e-6 of the largest count; rank 3 inverts,
 // rank 2 drops the smallest direction, rank <= 1 gives zero.
+//
+// Well-conditioned blocks take a short path: for a symmetric positive definite 3x3 with eigenvalues
+// l1 >= l2 >= l3, l3 / l1 >= det / trace^3, so det > 4e-6 trace^3 puts every eigenvalue above the rank
+// threshold; then V*^-1 = L^-T L^-1 with the Cholesky factor L, and G = L^-T has the same G G^T as the
+// eigen-decomposition (only the rank decision needs the eigenvalues).
 __device__ __forceinline__ void point_factor(const float V[6], float lambda, float G[3][3]) {
   float a[3][3] = {{V[0] * (1.f + lambda), V[1], V[2]}, {V[1], V[3] * (1.f + lambda), V[4]},
                    {V[2], V[4], V[5] * (1.f + lambda)}};
+  {
+    const float tr = a[0][0] + a[1][1] + a[2][2];
+    const float det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[1][2]) -
+                      a[0][1] * (a[0][1] * a[2][2] - a[0][2] * a[1][2]) +
+                      a[0][2] * (a[0][1] * a[1][2] - a[0][2] * a[1][1]);
+    if (tr > 0.f && det > 4e-6f * tr * tr * tr && a[0][0] > 0.f) {
+      const float l00 = sqrtf(a[0][0]), i00 = 1.f / l00;
+      const float l10 = a[0][1] * i00, l20 = a[0][2] * i00;
+      const float d11 = a[1][1] - l10 * l10;
+      if (d11 > 0.f) {
+        const float l11 = sqrtf(d11), i11 = 1.f / l11;
+        const float l21 = (a[1][2] - l20 * l10) * i11;
+        const float d22 = a[2][2] - l20 * l20 - l21 * l21;
+        if (d22 > 0.f) {
+          const float i22 = 1.f / sqrtf(d22);
+          // M = L^-1 (lower triangular); G = M^T
+          const float m10 = -l10 * i00 * i11, m21 = -l21 * i11 * i22, m20 = (l10 * l21 * i11 - l20) * i00 * i22;
+          G[0][0] = i00; G[0][1] = m10; G[0][2] = m20;
+          G[1][0] = 0.f; G[1][1] = i11; G[1][2] = m21;
+          G[2][0] = 0.f; G[2][1] = 0.f; G[2][2] = i22;
+          return;
+        }
+      }
+    }
+  }
   float v[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
   const float fro = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2] +
                     2.f * (a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2]);

@@ -886,6 +886,22 @@ hipError_t adj_pcg(const Batch& d, const AdjWork& A, int max_cg, double tol, hip
   return e;
 }
 
+// z = M^-1 gd with the factored Gauss-Newton preconditioner M = 2 sum rho' J^T L J (one application, no
+// conjugate gradients): a Gauss-Newton step.
+hipError_t adj_gn_step(const Batch& d, const AdjWork& A, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(A.done, 0, sizeof(int32_t), s);
+  if (e == hipSuccess)
+    e = launch_adj_vec(d, A.x, A.r, A.z, A.p, A.y, A.ec, A.zc, A.acc, A.gd, A.sc, A.done, 0.0, A.nx, A.n6, 0, s);
+  if (e == hipSuccess)
+    e = launch_adj_obs(d, 1, A.spts, A.sposes, A.r, nullptr, A.ec, nullptr, nullptr, nullptr, nullptr, A.done, s);
+  if (e == hipSuccess) e = launch_adj_chol(A.Lst, A.n6, A.ec, A.zc, A.done, s);
+  if (e == hipSuccess)
+    e = launch_adj_obs(d, 2, A.spts, A.sposes, nullptr, A.acc, nullptr, A.zc, nullptr, nullptr, nullptr, A.done, s);
+  if (e == hipSuccess)
+    e = launch_adj_vec(d, A.x, A.r, A.z, A.p, A.y, A.ec, A.zc, A.acc, A.gd, A.sc, A.done, 0.0, A.nx, A.n6, 1, s);
+  return e;
+}
+
 // gd = gradient of the cost at the FP64 state.
 hipError_t adj_grad(const Batch& d, const AdjWork& A, hipStream_t s) {
   hipError_t e = hipMemsetAsync(A.gd, 0, sizeof(double) * A.nx, s);
@@ -898,9 +914,10 @@ hipError_t adj_grad(const Batch& d, const AdjWork& A, hipStream_t s) {
 
 extern "C" {
 
-int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t newton_steps, int32_t max_cg, double tol, double* points,
-                     double* rig_from_world, double* grad_norm) {
-  if (!h || !b || newton_steps < 0 || max_cg < 1) return fail(h, SBA_E_INVALID, "bad sba_batch_refine arguments");
+int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t gauss_newton_steps, int32_t newton_steps, int32_t max_cg,
+                     double tol, double* points, double* rig_from_world, double* grad_norm) {
+  if (!h || !b || gauss_newton_steps < 0 || newton_steps < 0 || max_cg < 1)
+    return fail(h, SBA_E_INVALID, "bad sba_batch_refine arguments");
   HIP_TRY(hipSetDevice(h->opt.device));
   AdjWork A{};
   int rc = adj_prepare(h, b, A);
@@ -908,6 +925,13 @@ int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t newton_steps, int32_t 
   const Batch d = b->dev;
   const WindowDesc& W = b->win[0];
   hipStream_t s = h->stream;
+  // Gauss-Newton steps x <- x - M^-1 grad: the factored system is positive definite, so these descend safely
+  // from where the FP32 solve stopped, into the region where the full Hessian is positive definite too
+  for (int k = 0; k < gauss_newton_steps; ++k) {
+    HIP_TRY(adj_grad(d, A, s));
+    HIP_TRY(adj_gn_step(d, A, s));
+    HIP_TRY(launch_adj_state(d, A.spts, A.sposes, A.z, 1, s));
+  }
   // Newton steps x <- x - H^-1 grad, full FP64 Hessian, the stored factor as preconditioner
   for (int k = 0; k < newton_steps; ++k) {
     HIP_TRY(adj_grad(d, A, s));

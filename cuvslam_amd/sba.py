@@ -245,16 +245,18 @@ class DeviceBatch:
             raise SbaError(rc, self.solver.last_error())
         return S, e, dc, dp, sc, rc
 
-    def refine(self, newton_steps: int = 3, max_cg: int = 200, tol: float = 1e-12):
-        """FP64 Newton refinement of the solved state of window 0 (the state ``adjoint`` then differentiates at).
-        Returns (rig_from_world (N_c, 4, 4) float64, points (N_p, 3) float64, |grad F| after the last step)."""
+    def refine(self, newton_steps: int = 3, max_cg: int = 200, tol: float = 1e-12, gauss_newton_steps: int = 4):
+        """FP64 refinement of the solved state of window 0 (the state ``adjoint`` then differentiates at):
+        ``gauss_newton_steps`` steps with the factored Gauss-Newton system, then ``newton_steps`` full-Hessian
+        Newton steps. Returns (rig_from_world (N_c, 4, 4) float64, points (N_p, 3) float64, |grad F| after the
+        last step)."""
         p = self.problems[0]
         pts = np.zeros((p.points.shape[0], 3))
         T = np.zeros((p.rig_from_world.shape[0], 16))
         gn = C.c_double()
         d = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
-        rc = self.solver._L.sba_batch_refine(self.solver._h, self._b, newton_steps, max_cg, tol, d(pts), d(T),
-                                             C.byref(gn))
+        rc = self.solver._L.sba_batch_refine(self.solver._h, self._b, gauss_newton_steps, newton_steps, max_cg, tol,
+                                             d(pts), d(T), C.byref(gn))
         if rc != SBA_OK:
             raise SbaError(rc, self.solver.last_error())
         return T.reshape(-1, 4, 4).transpose(0, 2, 1).copy(), pts, gn.value
@@ -273,6 +275,10 @@ class DeviceBatch:
                                               C.byref(it), d(a))
         if rc != SBA_OK:
             raise SbaError(rc, self.solver.last_error())
+        if it.value >= max_cg:
+            import warnings
+            warnings.warn(f"adjoint conjugate gradients stopped at max_cg={max_cg} before reaching tol={tol}: the "
+                          "Hessian at the solution is (nearly) singular or the solve has not converged", RuntimeWarning)
         return dxy, dinfo, it.value, a
 
     def close(self):

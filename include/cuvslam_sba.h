@@ -136,18 +136,22 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc,
                          double* dp, double* scalars);
 
-/* FP64 Newton refinement of the solved state of window 0 of a batch, for the differentiable path: the FP32 LM
- * solve stops at its own rounding floor, and an implicit gradient taken there carries that residual gradient.
- * Each step solves H a = grad F with the full FP64 Hessian (the conjugate gradients of sba_batch_adjoint) and
- * applies x <- x (-) a (T_j <- Exp(-a_j) T_j, p_i <- p_i - a_i). The refined state stays with the batch and is
- * the state sba_batch_adjoint differentiates at, until the next sba_batch_run; the batch's FP32 state and its
- * sba_batch_read results are unchanged. newton_steps = 0 only copies the FP32 state. Outputs (any may be null):
+/* FP64 refinement of the solved state of window 0 of a batch, for the differentiable path: the FP32 LM solve
+ * stops at its own rounding floor, and an implicit gradient taken there carries that residual gradient. The FP32
+ * state is copied to FP64 (free rotations projected onto SO(3)), then
+ *   gauss_newton_steps steps x <- x (-) M^-1 grad F with M = 2 sum rho' J^T L J, the Gauss-Newton system of the
+ *     FP32 solution factored once in FP64 (positive definite, so each step descends), then
+ *   newton_steps steps x <- x (-) H^-1 grad F with the full FP64 Hessian (the conjugate gradients of
+ *     sba_batch_adjoint, truncated at non-positive curvature),
+ * where x (-) a is T_j <- Exp(-a_j) T_j, p_i <- p_i - a_i. The refined state stays with the batch and is the state
+ * sba_batch_adjoint differentiates at, until the next sba_batch_run; the batch's FP32 state and its
+ * sba_batch_read results are unchanged. Zero steps only copy the FP32 state. Outputs (any may be null):
  *   points         [num_points][3] FP64
  *   rig_from_world [num_poses][16] FP64 4x4, column-major (the Eigen layout of Isometry3T::matrix())
  *   grad_norm      |grad F| at the refined state.
  * No reference counterpart. */
-int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t newton_steps, int32_t max_cg, double tol, double* points,
-                     double* rig_from_world, double* grad_norm);
+int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t gauss_newton_steps, int32_t newton_steps, int32_t max_cg,
+                     double tol, double* points, double* rig_from_world, double* grad_norm);
 
 /* Implicit gradient through the solved state of window 0 of a batch (run sba_batch_run first; the solve should
  * have converged, and sba_batch_refine polishes it in FP64). g is dL/dx for a loss L of the solution in the solver's tangent coordinates: a left twist

@@ -945,6 +945,7 @@ struct Adjoint {
       hvp(p.data(), q.data());
       double pq = 0;
       for (int k = 0; k < nx; ++k) pq += p[k] * q[k];
+      if (!(pq > 0)) return it - 1;  // non-positive curvature: stop with the current a (truncated CG)
       const double alpha = rz / pq;
       double rr = 0;
       for (int k = 0; k < nx; ++k) {

@@ -49,13 +49,14 @@ def chain_to_pixels(dxy, dinfo, uv, K):
 
 
 class Case:
-    """A small window (80 points, 5 key frames of which 2 fixed) with Student-t loss (smooth, so central
+    """A window (by default 80 points, 5 key frames of which 2 fixed) with Student-t loss (smooth, so central
     differences are meaningful for every observation)."""
 
-    def __init__(self, loss=ca.LOSS_STUDENT_T, seed=4):
+    def __init__(self, loss=ca.LOSS_STUDENT_T, seed=4, num_points=80, num_keyframes=5, max_iterations=50, **scene):
         self.loss = loss
-        self.problem, _, self.gt = ca.scene.generate(seed=seed, num_points=80, num_keyframes=5,
-                                                     num_fixed_keyframes=2, max_iterations=50)
+        self.max_iterations = max_iterations
+        self.problem, _, self.gt = ca.scene.generate(seed=seed, num_points=num_points, num_keyframes=num_keyframes,
+                                                     num_fixed_keyframes=2, max_iterations=max_iterations, **scene)
         p = self.problem
         self.ncf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
         self.npf = p.points.shape[0] - p.num_fixed_points
@@ -119,3 +120,11 @@ class Case:
 @functools.lru_cache(maxsize=None)
 def case(loss=ca.LOSS_STUDENT_T, seed=4):
     return Case(loss, seed)
+
+
+# The differentiation workload (20 key frames, 10,000 points). Student-t is smooth, so central differences see the
+# derivative itself. No outliers: with outliers the robust cost is not strictly convex at the solution (Student-t
+# curvature turns negative past s = delta; a Huber point whose observations all lie in the linear region has a flat
+# valley of optima), the solution is not a differentiable function of the observations and the implicit system is
+# singular.
+C4_SCENE = dict(loss=ca.LOSS_STUDENT_T, seed=4, num_points=10000, num_keyframes=20, max_iterations=50, outlier_rate=0.0)

@@ -31,7 +31,7 @@ def test_adjoint_matches_oracle_at_same_state(loss):
     q, solver, b = solved(p, loss)
     ncf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
     # the device's FP64 copy of the FP32 solution (free rotations projected onto the rotation group)
-    poses, pts, _ = b.refine(newton_steps=0)
+    poses, pts, _ = b.refine(newton_steps=0, gauss_newton_steps=0)
     assert np.max(np.abs(poses - q.rig_from_world)) < 1e-6 and np.array_equal(pts, q.points.astype(np.float64))
     g = np.concatenate([pose_loss_grad(poses, gt, ncf), np.zeros(3 * p.points.shape[0])])
     dxy, dinfo, it, a = b.adjoint(g, max_cg=300, tol=1e-12)
@@ -102,3 +102,28 @@ def test_torch_autograd_function_matches_adjoint():
     np.testing.assert_allclose(poses.detach().numpy(), T, rtol=0, atol=1e-12)
     dxy, dinfo, it, _ = b.adjoint(c.g_of(T), max_cg=300, tol=1e-12)
     np.testing.assert_allclose(pixels.grad.numpy()[:, 0], dxy[:, 0] / FX, rtol=1e-9, atol=1e-15)
+
+
+def test_c4_window_gradients_vs_finite_difference_fixture():
+    # 20 key frames, 10,000 points: device solve, FP64 refinement and adjoint against the committed FP64 central
+    # differences of converged oracle solves (tests/golden/make_c4_fd.py)
+    import json
+    import os
+    from adjoint_fd import C4_SCENE, Case
+    with open(os.path.join(os.path.dirname(__file__), "golden", "c4_fd.json")) as f:
+        fx = json.load(f)
+    c = Case(**C4_SCENE)
+    assert c.problem.num_observations == fx["num_observations"]
+    q, solver, b = solved(c.problem, c.loss)
+    # the FP32 solve of this window stops at |grad F| ~ 2e-4 of its start, where the full Hessian is not yet
+    # positive definite: Gauss-Newton steps first, then Newton steps
+    poses, pts, gn = b.refine(gauss_newton_steps=8, newton_steps=3, max_cg=300, tol=1e-12)
+    g0 = np.linalg.norm(O.grad(c.problem, c.init_pts, c.init_poses, loss=c.loss))
+    assert gn < 1e-12 * g0, (gn, g0)
+    assert abs(c.L(poses) - fx["loss_at_optimum"]) <= 1e-8 * fx["loss_at_optimum"], (c.L(poses), fx["loss_at_optimum"])
+    dxy, dinfo, it, _ = b.adjoint(c.g_of(poses), max_cg=300, tol=1e-12)
+    assert it < 300
+    duv, dK = chain_to_pixels(dxy, dinfo, c.uv, K0)
+    picked = np.array([duv[o, cc] for o, cc in fx["sampled"]])
+    assert rel(picked, fx["fd_pixels"]) < 1e-4
+    np.testing.assert_allclose(dK, fx["fd_intrinsics"], rtol=1e-4)
