@@ -9,6 +9,7 @@
 //                 free key frames come first:
 //                   pair_obs  int32 first observation of the pair (pair_obs[p+1] ends it)
 //                   pair_pose int32 key frame index inside the window
+//                   pair_tp   uint8 point index inside the pair's Schur tile
 //   points          point_pair int32 first pair of the point (point_pair[i+1] ends it)
 //                   point_nfree int32 number of the point's pairs with free key frames
 //                   pts[2]   float4 (x, y, z, 0), double-buffered: current and trial state
@@ -90,6 +91,7 @@ struct Batch {
   const float4* obs_info;
   const int32_t* pair_obs;
   const int32_t* pair_pose;
+  const uint8_t* pair_tp;    // the pair's point index inside its Schur tile (< kTilePoints)
   const int32_t* point_pair;
   const int32_t* point_nfree;
   float4* pts0;
@@ -109,7 +111,7 @@ struct Batch {
   float4* out_poses;
   float4* dbg_dp;    // optional: the point step of the last back-substitution (phase-level tests)
   // shared-memory layout of the Schur kernel (offsets in 4-byte words)
-  int32_t lds_pose, lds_cam, lds_u, lds_acc, lds_w, lds_pp, lds_zc, lds_words;
+  int32_t lds_pose, lds_cam, lds_acc, lds_ent, lds_ec, lds_w, lds_zc, lds_words;
   int32_t factor_lds_words;  // LDS words of the factor kernel
 };
 

@@ -226,7 +226,6 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTilesPerWave = (kNT * (kNT + 1) / 2 + 3) / 4;  // 30
 
 // Upper tile T (row-major over the NT x NT upper triangle) -> (I, J).
 constexpr int tri_i(int nt, int T) {
@@ -297,7 +296,58 @@ __device__ __forceinline__ void store_tiles(float* rec, int lane, const floatx4 
   store_all<NT, WAVE>(rec, lane, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
 }
 
+// Workgroup barrier for shared memory only: waits for this wave's LDS operations, not for its global loads, so
+// loads issued for the next tile stay in flight across it. The empty asm keeps the compiler from moving memory
+// accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left at their maxima (gfx9 encoding)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// A pair's first observation and its point, loaded one tile ahead.
+struct PairPre {
+  int tp, j, o0, o1;
+  float2 xy;
+  float4 inf, p;
+};
+
+__device__ __forceinline__ void prefetch_pair(const Batch& b, const float4* __restrict__ pts, int t, int tid, PairPre& q) {
+  const Tile T = b.tiles[t];
+  const int p0 = b.point_pair[T.point_begin];
+  const int npair = b.point_pair[T.point_end] - p0;
+  if (tid < npair) {
+    const int pair = p0 + tid;
+    q.tp = b.pair_tp[pair];
+    q.j = b.pair_pose[pair];
+    q.o0 = b.pair_obs[pair];
+    q.o1 = b.pair_obs[pair + 1];
+    q.xy = b.obs_xy[q.o0];
+    q.inf = b.obs_info[q.o0];
+    q.p = pts[T.point_begin + q.tp];
+  }
+}
+
+// Per-pair contributions are reduced without floating-point atomics: each pair thread stores its U/e_a (27) and
+// V/e_b (9) terms as columns of a shared scratch, and fixed owner threads sum them. A key frame's terms in a tile
+// are found through a (tile point, key frame) -> pair table; a point's pairs are contiguous.
+constexpr int kUVals = 27;               // U (21 upper entries) and e (6) of one key frame
+constexpr int kUGroup = 5;               // values per owner thread
+constexpr int kUGroups = (kUVals + kUGroup - 1) / kUGroup;  // 6
+static_assert(kUGroups * kMaxFreePoses <= kBlock - kTilePoints, "owner threads of U and V must not overlap");
+static_assert((kUVals + 9) * kTilePairs <= kKt * kZcLd, "the scratch aliases the staged Z columns");
+
 // One chunk of tiles of one window with NT = ceil(6 N_c,free / 16) tiles per side of its reduced system.
+//
+// Threads: pair thread e < npair handles pair p0 + e (linearize, W, Z = W G); U owner thread
+// tid < 6 ncf owns values [5 g, 5 g + 5) of key frame j = tid / 6 (g = tid % 6) for the whole chunk, in
+// registers; V owner thread kBlock - 16 + tp sums point tp's V, e_b and factors its block. Per tile:
+//   (b) linearize; scratch <- per-pair U/e_a and V/e_b terms; pair table <- pair index
+//   (c) owners sum the scratch (U into registers; V -> G, G^T e_b into sAcc and paux)
+//   (z) zero the staged Z columns (they alias the scratch)
+//   (d) Z = W G into the staged columns; e_c terms -Z G^T e_b into their own scratch
+//   (e) e owners sum the e_c terms; matrix cores: acc += Zc Zc^T
 template <int NT>
 __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, const WindowDesc& W, float lam, int cur,
                                             float* sm) {
@@ -308,16 +358,26 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const int ncf = W.ncf;
   float* sPose = sm + b.lds_pose;
   float* sCam = sm + b.lds_cam;
-  float* sU = sm + b.lds_u;  // U (21 per free key frame), e (6), diag U (6)
-  float* sE = sU + 21 * ncf;
-  float* sDu = sE + 6 * ncf;
-  float* sAcc = sm + b.lds_acc;  // 12 per tile point: V (6), e_b (3), then G (9) and G^T e_b (3)
-  float* sW = sm + b.lds_w;      // 18 per tile pair: W_ij (6x3)
-  int* sPP = reinterpret_cast<int*>(sm + b.lds_pp);
+  float* sAcc = sm + b.lds_acc;  // 12 per tile point: G (9) and G^T e_b (3)
+  int* sEnt = reinterpret_cast<int*>(sm + b.lds_ent);  // [tile point][free key frame]: (tile << 9) | pair
+  float* sEc = sm + b.lds_ec;    // [6][kTilePairs] e_c terms
+  float* sW = sm + b.lds_w;      // [18][kTilePairs] W_ij of each pair (6x3, row-major)
   float* sZc = sm + b.lds_zc;    // Z columns: [k][row], k = 3 * tile point + c, row = 6 * key frame + r
+  float* sScrU = sZc;            // [27][kTilePairs] per-pair U/e_a terms (aliases the Z columns)
+  float* sScrV = sZc + kUVals * kTilePairs;  // [9][kTilePairs] per-pair V/e_b terms
   const int kind = b.loss;
   const float delta = W.robust;
   const int pf_end = W.point_off + W.npf;
+  // U owner: key frame uj, values [u0, u1)
+  const bool u_owner = tid < kUGroups * ncf;
+  const int uj = tid / kUGroups, u0 = kUGroup * (tid % kUGroups), u1 = min(u0 + kUGroup, kUVals);
+  float accU[kUGroup] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  // V owner: tile point vp
+  const int vp = tid - (kBlock - kTilePoints);
+
+  for (int k = tid; k < kTilePoints * kMaxFreePoses; k += kBlock) sEnt[k] = -1;
+  PairPre pre{};
+  prefetch_pair(b, pts, ch.tile_begin, tid, pre);
 
   floatx4 acc[tiles_per_wave<NT>()];
 #pragma unroll
@@ -329,35 +389,28 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     const int npair = b.point_pair[T.point_end] - p0;
     const int npts = T.point_end - T.point_begin;
     const int kpad = (3 * npts + 3) & ~3;
-    __syncthreads();  // the previous tile's shared arrays are free
-    // (a) pair -> tile point map, zero the point accumulators and the staged columns
-    if (tid < npts) {
-      const int i = T.point_begin + tid;
-      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
-      for (int k = a; k < e; ++k) sPP[k] = tid;
-#pragma unroll
-      for (int k = 0; k < 12; ++k) sAcc[12 * tid + k] = 0.f;
-    }
-    {  // zero the staged columns that the matrix cores read: kpad k-rows x 16 nt rows, 16-byte stores
-      const int q = 4 * nt;
-      float4* z4 = reinterpret_cast<float4*>(sZc);
-      for (int k = tid; k < kpad * q; k += kBlock) z4[(k / q) * (kZcLd / 4) + k % q] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    __syncthreads();
-    // (b) one thread per pair: linearize its observations (one per camera) and accumulate
+    const int stamp = (t - ch.tile_begin) << 9;
+    lds_barrier();  // the previous tile's shared arrays are free
+    // (b) one thread per pair: linearize its observations (one per camera), store W and the U and V terms
+    int tp = 0, j = 0;
+    bool pair_pf = false, pair_cf = false;
     if (tid < npair) {
-      const int pair = p0 + tid;
-      const int tp = sPP[tid];
+      tp = pre.tp;
       const int i = T.point_begin + tp;
-      const int j = b.pair_pose[pair];
-      const bool point_free = i < pf_end;
-      const bool pose_free = j < ncf;
-      const float4 p = pts[i];
-      float Wm[6][3] = {}, U[21] = {}, ea[6] = {}, V[6] = {}, eb[3] = {};
-      const int o_end = b.pair_obs[pair + 1];
-      for (int o = b.pair_obs[pair]; o < o_end; ++o) {
-        const float2 xy = b.obs_xy[o];
-        const float4 inf = b.obs_info[o];
+      j = pre.j;
+      pair_pf = i < pf_end;
+      pair_cf = j < ncf;
+      const float4 p = pre.p;
+      // the pair's terms go straight to its own scratch column (no other thread touches it): the first valid
+      // observation stores, later ones (other cameras of a rig) add; a pair without one stores zeros
+      bool first = true;
+      auto put = [&](float* arr, int k, float v) {
+        float* q = arr + k * kTilePairs + tid;
+        *q = first ? v : *q + v;
+      };
+      for (int o = pre.o0; o < pre.o1; ++o) {
+        const float2 xy = o == pre.o0 ? pre.xy : b.obs_xy[o];
+        const float4 inf = o == pre.o0 ? pre.inf : b.obs_info[o];
         const int cam = __float_as_int(inf.w);
         Lin L;
         if (!linearize(sPose + 12 * j, sCam + 12 * cam, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L))
@@ -375,116 +428,148 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
           LJp[0][k] = L.w * (inf.x * L.jp[0][k] + inf.y * L.jp[1][k]);
           LJp[1][k] = L.w * (inf.y * L.jp[0][k] + inf.z * L.jp[1][k]);
         }
-        if (pose_free) {
+        if (pair_cf) {
           int u = 0;
 #pragma unroll
           for (int r = 0; r < 6; ++r) {
 #pragma unroll
-            for (int c = r; c < 6; ++c) U[u++] += L.jc[0][r] * LJc[0][c] + L.jc[1][r] * LJc[1][c];
-            ea[r] += L.jc[0][r] * wr0 + L.jc[1][r] * wr1;
+            for (int c = r; c < 6; ++c) put(sScrU, u++, L.jc[0][r] * LJc[0][c] + L.jc[1][r] * LJc[1][c]);
+            put(sScrU, 21 + r, L.jc[0][r] * wr0 + L.jc[1][r] * wr1);
           }
-          if (point_free) {
+          if (pair_pf) {
 #pragma unroll
             for (int r = 0; r < 6; ++r)
 #pragma unroll
-              for (int c = 0; c < 3; ++c) Wm[r][c] += L.jc[0][r] * LJp[0][c] + L.jc[1][r] * LJp[1][c];
+              for (int c = 0; c < 3; ++c) put(sW, 3 * r + c, L.jc[0][r] * LJp[0][c] + L.jc[1][r] * LJp[1][c]);
           }
         }
-        if (point_free) {
-          V[0] += L.jp[0][0] * LJp[0][0] + L.jp[1][0] * LJp[1][0];
-          V[1] += L.jp[0][0] * LJp[0][1] + L.jp[1][0] * LJp[1][1];
-          V[2] += L.jp[0][0] * LJp[0][2] + L.jp[1][0] * LJp[1][2];
-          V[3] += L.jp[0][1] * LJp[0][1] + L.jp[1][1] * LJp[1][1];
-          V[4] += L.jp[0][1] * LJp[0][2] + L.jp[1][1] * LJp[1][2];
-          V[5] += L.jp[0][2] * LJp[0][2] + L.jp[1][2] * LJp[1][2];
+        if (pair_pf) {
+          put(sScrV, 0, L.jp[0][0] * LJp[0][0] + L.jp[1][0] * LJp[1][0]);
+          put(sScrV, 1, L.jp[0][0] * LJp[0][1] + L.jp[1][0] * LJp[1][1]);
+          put(sScrV, 2, L.jp[0][0] * LJp[0][2] + L.jp[1][0] * LJp[1][2]);
+          put(sScrV, 3, L.jp[0][1] * LJp[0][1] + L.jp[1][1] * LJp[1][1]);
+          put(sScrV, 4, L.jp[0][1] * LJp[0][2] + L.jp[1][1] * LJp[1][2]);
+          put(sScrV, 5, L.jp[0][2] * LJp[0][2] + L.jp[1][2] * LJp[1][2]);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) eb[k] += L.jp[0][k] * wr0 + L.jp[1][k] * wr1;
+          for (int k = 0; k < 3; ++k) put(sScrV, 6 + k, L.jp[0][k] * wr0 + L.jp[1][k] * wr1);
         }
+        first = false;
       }
-      if (point_free) {
-        float* a = sAcc + 12 * tp;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) lds_add(a + k, V[k]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) lds_add(a + 6 + k, eb[k]);
+      if (first) {
+        if (pair_cf) {
+          for (int k = 0; k < kUVals; ++k) sScrU[k * kTilePairs + tid] = 0.f;
+          if (pair_pf)
+            for (int k = 0; k < 18; ++k) sW[k * kTilePairs + tid] = 0.f;
+        }
+        if (pair_pf)
+          for (int k = 0; k < 9; ++k) sScrV[k * kTilePairs + tid] = 0.f;
       }
-      if (pose_free) {
+      if (pair_cf) sEnt[tp * kMaxFreePoses + j] = stamp | tid;
+    }
+    lds_barrier();
+    // (c) owners sum the pair terms: U and e_a of key frame uj into registers; V, e_b of point vp, then its
+    // damped pseudo-inverse V*^+ = G G^T
+    if (u_owner) {
+      int ent[kTilePoints];
 #pragma unroll
-        for (int u = 0; u < 21; ++u) lds_add(sU + 21 * j + u, U[u]);
-        int d = 0;
+      for (int q = 0; q < kTilePoints; ++q) ent[q] = sEnt[q * kMaxFreePoses + uj];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          lds_add(sE + 6 * j + r, ea[r]);
-          lds_add(sDu + 6 * j + r, U[d]);
-          d += 6 - r;
-        }
-        if (point_free) {
-          float* z = sW + 18 * tid;
+      for (int q = 0; q < kTilePoints; ++q) {
+        const bool ok = q < npts && ent[q] >= 0 && (ent[q] & ~511) == stamp;
+        const int col = ok ? (ent[q] & 511) : 0;
 #pragma unroll
-          for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) z[3 * r + c] = Wm[r][c];
-        }
+        for (int v = 0; v < kUGroup; ++v)
+          if (u0 + v < u1) {
+            const float x = sScrU[(u0 + v) * kTilePairs + col];
+            accU[v] += ok ? x : 0.f;
+          }
       }
     }
-    __syncthreads();
-    // (c) one thread per free point: damped pseudo-inverse V*^+ = G G^T
-    if (tid < npts) {
-      const int i = T.point_begin + tid;
-      if (i < pf_end) {
-        float* a = sAcc + 12 * tid;
-        const float V[6] = {a[0], a[1], a[2], a[3], a[4], a[5]};
-        const float eb[3] = {a[6], a[7], a[8]};
-        float G[3][3];
-        point_factor(V, lam, G);
-        float tv[3], Vi[6], gb[3];
+    if (vp >= 0 && vp < npts && T.point_begin + vp < pf_end) {
+      const int i = T.point_begin + vp;
+      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
+      float V[6] = {}, eb[3] = {};
+#pragma unroll 4
+      for (int k = a; k < e; ++k) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) tv[k] = G[0][k] * eb[0] + G[1][k] * eb[1] + G[2][k] * eb[2];
-        Vi[0] = G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2];
-        Vi[1] = G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2];
-        Vi[2] = G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2];
-        Vi[3] = G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2];
-        Vi[4] = G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2];
-        Vi[5] = G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2];
+        for (int m = 0; m < 6; ++m) V[m] += sScrV[m * kTilePairs + k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gb[k] = G[k][0] * tv[0] + G[k][1] * tv[1] + G[k][2] * tv[2];
-        float4* ax = b.paux + 3 * i;
-        ax[0] = make_float4(Vi[0], Vi[1], Vi[2], Vi[3]);
-        ax[1] = make_float4(Vi[4], Vi[5], gb[0], gb[1]);
-        ax[2] = make_float4(gb[2], V[0], V[3], V[5]);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) a[3 * r + c] = G[r][c];
-        a[9] = tv[0];
-        a[10] = tv[1];
-        a[11] = tv[2];
+        for (int m = 0; m < 3; ++m) eb[m] += sScrV[(6 + m) * kTilePairs + k];
       }
+      float G[3][3];
+      point_factor(V, lam, G);
+      float tv[3], Vi[6], gb[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) tv[k] = G[0][k] * eb[0] + G[1][k] * eb[1] + G[2][k] * eb[2];
+      Vi[0] = G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2];
+      Vi[1] = G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2];
+      Vi[2] = G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2];
+      Vi[3] = G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2];
+      Vi[4] = G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2];
+      Vi[5] = G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gb[k] = G[k][0] * tv[0] + G[k][1] * tv[1] + G[k][2] * tv[2];
+      float4* ax = b.paux + 3 * i;
+      ax[0] = make_float4(Vi[0], Vi[1], Vi[2], Vi[3]);
+      ax[1] = make_float4(Vi[4], Vi[5], gb[0], gb[1]);
+      ax[2] = make_float4(gb[2], V[0], V[3], V[5]);
+      float* ac = sAcc + 12 * vp;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ac[3 * r + c] = G[r][c];
+      ac[9] = tv[0];
+      ac[10] = tv[1];
+      ac[11] = tv[2];
     }
-    __syncthreads();
-    // (d) one thread per (free point, free key frame) pair: Z = W G into the staged columns, e_j -= Z G^T e_b
-    if (tid < npair) {
-      const int pair = p0 + tid;
-      const int tp = sPP[tid];
-      const int i = T.point_begin + tp;
-      const int j = b.pair_pose[pair];
-      if (i < pf_end && j < ncf) {
+    lds_barrier();
+    // (z) zero the staged columns that the matrix cores read: kpad k-rows x 16 nt rows, 16-byte stores
+    {
+      const int q = 4 * nt;
+      float4* z4 = reinterpret_cast<float4*>(sZc);
+      for (int k = tid; k < kpad * q; k += kBlock) z4[(k / q) * (kZcLd / 4) + k % q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    lds_barrier();
+    // (d) one thread per (free point, free key frame) pair: Z = W G into the staged columns, e_c terms -Z G^T e_b
+    if (tid < npair && pair_cf) {
+      float ec[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (pair_pf) {
         const float* a = sAcc + 12 * tp;
-        const float* z = sW + 18 * tid;
         float* zc = sZc + 3 * tp * kZcLd + 6 * j;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           float zr[3];
+          const float w0 = sW[(3 * r) * kTilePairs + tid], w1 = sW[(3 * r + 1) * kTilePairs + tid],
+                      w2 = sW[(3 * r + 2) * kTilePairs + tid];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) zr[c] = z[3 * r] * a[c] + z[3 * r + 1] * a[3 + c] + z[3 * r + 2] * a[6 + c];
+          for (int c = 0; c < 3; ++c) zr[c] = w0 * a[c] + w1 * a[3 + c] + w2 * a[6 + c];
 #pragma unroll
           for (int c = 0; c < 3; ++c) zc[c * kZcLd + r] = zr[c];
-          lds_add(sE + 6 * j + r, -(zr[0] * a[9] + zr[1] * a[10] + zr[2] * a[11]));
+          ec[r] = -(zr[0] * a[9] + zr[1] * a[10] + zr[2] * a[11]);
         }
       }
+#pragma unroll
+      for (int r = 0; r < 6; ++r) sEc[r * kTilePairs + tid] = ec[r];
     }
-    __syncthreads();
-    // (e) matrix cores: acc += Zc Zc^T over the tile's k-steps (only this loop is specialised per wave)
+    lds_barrier();
+    // (e) the next tile's pair data is requested; the e owners sum the e_c terms; matrix cores: acc += Zc Zc^T
+    if (t + 1 < ch.tile_end) prefetch_pair(b, pts, t + 1, tid, pre);
+    if (u_owner && u1 > 21) {
+      int ent[kTilePoints];
+#pragma unroll
+      for (int q = 0; q < kTilePoints; ++q) ent[q] = sEnt[q * kMaxFreePoses + uj];
+#pragma unroll
+      for (int q = 0; q < kTilePoints; ++q) {
+        const bool ok = q < npts && ent[q] >= 0 && (ent[q] & ~511) == stamp;
+        const int col = ok ? (ent[q] & 511) : 0;
+#pragma unroll
+        for (int v = 0; v < kUGroup; ++v)
+          if (u0 + v >= 21 && u0 + v < u1) {
+            const float x = sEc[(u0 + v - 21) * kTilePairs + col];
+            accU[v] += ok ? x : 0.f;
+          }
+      }
+    }
     switch (tid >> 6) {
       case 0: mfma_tile<NT, 0>(sZc, kpad / 4, lane, acc); break;
       case 1: mfma_tile<NT, 1>(sZc, kpad / 4, lane, acc); break;
@@ -492,17 +577,31 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       default: mfma_tile<NT, 3>(sZc, kpad / 4, lane, acc); break;
     }
   }
-  // the chunk's record: Schur tiles (lane order), then U, e, diag U
+  // the chunk's record: Schur tiles (lane order), then U (21 per key frame), e (6), diag U (6)
   float* rec = b.part + ch.part_off;
-  switch (tid >> 6) {
+  if (W.nt > 0) switch (tid >> 6) {
     case 0: store_tiles<NT, 0>(rec, lane, acc); break;
     case 1: store_tiles<NT, 1>(rec, lane, acc); break;
     case 2: store_tiles<NT, 2>(rec, lane, acc); break;
     default: store_tiles<NT, 3>(rec, lane, acc); break;
   }
-  __syncthreads();
-  const int ntu = nt * (nt + 1) / 2;
-  for (int k = tid; k < 33 * ncf; k += kBlock) rec[256 * ntu + k] = sU[k];
+  if (u_owner) {
+    float* ru = rec + 256 * (nt * (nt + 1) / 2);
+    constexpr int kDiag[6] = {0, 6, 11, 15, 18, 20};  // diagonal entries of the packed upper 6x6
+#pragma unroll
+    for (int v = 0; v < kUGroup; ++v) {
+      const int u = u0 + v;
+      if (u >= u1) continue;
+      if (u < 21) {
+        ru[21 * uj + u] = accU[v];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+          if (u == kDiag[r]) ru[27 * ncf + 6 * uj + r] = accU[v];
+      } else {
+        ru[21 * ncf + 6 * uj + (u - 21)] = accU[v];
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -519,9 +618,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   const int tid = threadIdx.x;
   load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
   load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
-  for (int k = tid; k < 33 * W.ncf; k += kBlock) sm[b.lds_u + k] = 0.f;
   static_assert(kNT == 15, "the dispatch below covers 1..15 tiles");
   switch (W.nt) {
+    case 0:  // no free key frame: only the point blocks (V*^-1, V*^-1 e_b) are needed; the record is empty
     case 1: schur_chunk<1>(b, ch, W, lam, cur, sm); break;
     case 2: schur_chunk<2>(b, ch, W, lam, cur, sm); break;
     case 3: schur_chunk<3>(b, ch, W, lam, cur, sm); break;

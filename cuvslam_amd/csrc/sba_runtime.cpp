@@ -85,6 +85,7 @@ struct HostPack {
   std::vector<float2> obs_xy;
   std::vector<float4> obs_info;
   std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
+  std::vector<uint8_t> pair_tp;
   std::vector<float4> pts, poses, cams;
   std::vector<Tile> tiles, btiles;
 };
@@ -185,8 +186,19 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pairs});
     }
   };
+  const size_t first_tile = H.tiles.size();
   make_tiles(H.tiles, kTilePoints);
   make_tiles(H.btiles, kBlock);
+  // each pair's point index inside its Schur tile
+  H.pair_tp.resize(H.pair_pose.size());
+  for (size_t t = first_tile; t < H.tiles.size(); ++t) {
+    const Tile& T = H.tiles[t];
+    for (int i = T.point_begin; i < T.point_end; ++i) {
+      const int a = H.point_pair[i];
+      const int e = i + 1 < static_cast<int>(H.point_pair.size()) ? H.point_pair[i + 1] : static_cast<int>(H.pair_pose.size());
+      for (int q = a; q < e; ++q) H.pair_tp[q] = static_cast<uint8_t>(i - T.point_begin);
+    }
+  }
   return SBA_OK;
 }
 
@@ -429,6 +441,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(H.obs_info, obs_info);
   UP(H.pair_obs, pair_obs);
   UP(H.pair_pose, pair_pose);
+  UP(H.pair_tp, pair_tp);
   UP(H.point_pair, point_pair);
   UP(H.point_nfree, point_nfree);
   UP(H.pts, pts_init);
@@ -465,14 +478,14 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   off += 12 * max_nc;
   d.lds_cam = off;
   off += 12 * max_ncam;
-  d.lds_u = off;
-  off += 33 * max_ncf;
   d.lds_acc = off;
   off += 12 * kTilePoints;
+  d.lds_ent = off;
+  off += kTilePoints * kMaxFreePoses;
+  d.lds_ec = off;
+  off += 6 * kTilePairs;
   d.lds_w = off;
   off += 18 * kTilePairs;
-  d.lds_pp = off;
-  off += kTilePairs;
   off = (off + 63) & ~63;
   d.lds_zc = off;
   off += kKt * kZcLd;
