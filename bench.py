@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--chunk-pairs", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SBA batch is sharded across GPUs by window with no exchange, "
@@ -72,7 +73,18 @@ def main():
     n_obs = sum(p.num_observations for p in problems)
     log(f"[bench] generated {len(problems)} window(s), {n_obs} observations in {time.time() - t0:.1f}s")
 
-    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=args.chunk_pairs)
+    chunk_pairs = args.chunk_pairs
+    tuned = None
+    if args.workload == "c3" and chunk_pairs == 0 and not args.no_autotune:
+        # work granularity of the batched launch, picked by the autotuner before the timed region
+        t0 = time.time()
+        tuner = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+        cands = [750, 1500, 3000, 6000, 12000]
+        chunk_pairs, ms = tuner.autotune(problems, cands, repeats=1)
+        tuner.close()
+        tuned = {"candidates": cands, "ms_per_solve_batch": [round(m, 3) for m in ms], "chosen": chunk_pairs}
+        log(f"[bench] autotuned chunk_pairs in {time.time() - t0:.1f}s: {tuned}")
+    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=chunk_pairs)
     t0 = time.time()
     batch = solver.batch(problems)
     info = batch.info()
@@ -194,7 +206,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": workload, "windows": len(problems), "observations": n_obs,
-                   "pairs": info["pairs"], "lm_iterations": 10, "chunks": info["chunks"]},
+                   "pairs": info["pairs"], "lm_iterations": 10, "chunks": info["chunks"], "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_per_launch": {k: round(v, 5) for k, v in per_launch.items()},

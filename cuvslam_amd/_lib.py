@@ -72,6 +72,7 @@ SIGNATURES = {
     "sba_batch_debug_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
     "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     "sba_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
+    "sba_autotune_chunks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32, _ip, _fp]),
     "sba_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp, _dp,
                                    _dp]),
     "sba_debug_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _fp]),

@@ -9,7 +9,8 @@
 //                 free key frames come first:
 //                   pair_obs  int32 first observation of the pair (pair_obs[p+1] ends it)
 //                   pair_pose int32 key frame index inside the window
-//                   pair_tp   uint8 point index inside the pair's Schur tile
+//                   pair_rec  int4  (first observation, observation end, key frame, point index in its Schur tile)
+//                   pair_xy, pair_info  the pair's first observation, indexed by pair
 //   points          point_pair int32 first pair of the point (point_pair[i+1] ends it)
 //                   point_nfree int32 number of the point's pairs with free key frames
 //                   pts[2]   float4 (x, y, z, 0), double-buffered: current and trial state
@@ -69,7 +70,7 @@ struct WindowState {
 };
 
 struct Tile {
-  int32_t window, point_begin, point_end, pad;
+  int32_t window, point_begin, point_end, pair_begin, pair_end, pad[3];
 };
 
 struct Chunk {
@@ -91,7 +92,10 @@ struct Batch {
   const float4* obs_info;
   const int32_t* pair_obs;
   const int32_t* pair_pose;
-  const uint8_t* pair_tp;    // the pair's point index inside its Schur tile (< kTilePoints)
+  const int4* pair_rec;      // per pair: (first observation, end of its observations, key frame, point index
+                             // inside its Schur tile)
+  const float2* pair_xy;     // per pair: its first observation (the obs streams themselves when every pair has
+  const float4* pair_info;   // exactly one observation)
   const int32_t* point_pair;
   const int32_t* point_nfree;
   float4* pts0;
@@ -111,7 +115,7 @@ struct Batch {
   float4* out_poses;
   float4* dbg_dp;    // optional: the point step of the last back-substitution (phase-level tests)
   // shared-memory layout of the Schur kernel (offsets in 4-byte words)
-  int32_t lds_pose, lds_cam, lds_acc, lds_ent, lds_ec, lds_w, lds_zc, lds_words;
+  int32_t lds_pose, lds_cam, lds_acc, lds_ent, lds_ec, lds_w, lds_pts, lds_zc, lds_words;
   int32_t factor_lds_words;  // LDS words of the factor kernel
 };
 

@@ -306,28 +306,12 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// A pair's first observation and its point, loaded one tile ahead.
+// A pair's record and first observation, loaded one tile ahead.
 struct PairPre {
-  int tp, j, o0, o1;
+  int4 rec;  // (first observation, observation end, key frame, point index in the tile)
   float2 xy;
-  float4 inf, p;
+  float4 inf;
 };
-
-__device__ __forceinline__ void prefetch_pair(const Batch& b, const float4* __restrict__ pts, int t, int tid, PairPre& q) {
-  const Tile T = b.tiles[t];
-  const int p0 = b.point_pair[T.point_begin];
-  const int npair = b.point_pair[T.point_end] - p0;
-  if (tid < npair) {
-    const int pair = p0 + tid;
-    q.tp = b.pair_tp[pair];
-    q.j = b.pair_pose[pair];
-    q.o0 = b.pair_obs[pair];
-    q.o1 = b.pair_obs[pair + 1];
-    q.xy = b.obs_xy[q.o0];
-    q.inf = b.obs_info[q.o0];
-    q.p = pts[T.point_begin + q.tp];
-  }
-}
 
 // Per-pair contributions are reduced without floating-point atomics: each pair thread stores its U/e_a (27) and
 // V/e_b (9) terms as columns of a shared scratch, and fixed owner threads sum them. A key frame's terms in a tile
@@ -336,7 +320,8 @@ constexpr int kUVals = 27;               // U (21 upper entries) and e (6) of on
 constexpr int kUGroup = 5;               // values per owner thread
 constexpr int kUGroups = (kUVals + kUGroup - 1) / kUGroup;  // 6
 static_assert(kUGroups * kMaxFreePoses <= kBlock - kTilePoints, "owner threads of U and V must not overlap");
-static_assert((kUVals + 9) * kTilePairs <= kKt * kZcLd, "the scratch aliases the staged Z columns");
+constexpr int kScr = kTilePairs + 1;     // scratch row stride: owners reading one pair's values hit distinct banks
+static_assert((kUVals + 9) * kScr <= kKt * kZcLd, "the scratch aliases the staged Z columns");
 
 // One chunk of tiles of one window with NT = ceil(6 N_c,free / 16) tiles per side of its reduced system.
 //
@@ -360,11 +345,12 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   float* sCam = sm + b.lds_cam;
   float* sAcc = sm + b.lds_acc;  // 12 per tile point: G (9) and G^T e_b (3)
   int* sEnt = reinterpret_cast<int*>(sm + b.lds_ent);  // [tile point][free key frame]: (tile << 9) | pair
-  float* sEc = sm + b.lds_ec;    // [6][kTilePairs] e_c terms
-  float* sW = sm + b.lds_w;      // [18][kTilePairs] W_ij of each pair (6x3, row-major)
+  float* sEc = sm + b.lds_ec;    // [6][kScr] e_c terms
+  float* sW = sm + b.lds_w;      // [18][kScr] W_ij of each pair (6x3, row-major)
+  float4* sPts = reinterpret_cast<float4*>(sm + b.lds_pts);  // [2][kTilePoints] the tile's points, double-buffered
   float* sZc = sm + b.lds_zc;    // Z columns: [k][row], k = 3 * tile point + c, row = 6 * key frame + r
-  float* sScrU = sZc;            // [27][kTilePairs] per-pair U/e_a terms (aliases the Z columns)
-  float* sScrV = sZc + kUVals * kTilePairs;  // [9][kTilePairs] per-pair V/e_b terms
+  float* sScrU = sZc;            // [27][kScr] per-pair U/e_a terms (aliases the Z columns)
+  float* sScrV = sZc + kUVals * kScr;  // [9][kScr] per-pair V/e_b terms
   const int kind = b.loss;
   const float delta = W.robust;
   const int pf_end = W.point_off + W.npf;
@@ -376,17 +362,25 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const int vp = tid - (kBlock - kTilePoints);
 
   for (int k = tid; k < kTilePoints * kMaxFreePoses; k += kBlock) sEnt[k] = -1;
+  // the first tile's pair data and points
+  Tile Tn = b.tiles[ch.tile_begin];
   PairPre pre{};
-  prefetch_pair(b, pts, ch.tile_begin, tid, pre);
+  if (tid < Tn.pair_end - Tn.pair_begin) {
+    pre.rec = b.pair_rec[Tn.pair_begin + tid];
+    pre.xy = b.pair_xy[Tn.pair_begin + tid];
+    pre.inf = b.pair_info[Tn.pair_begin + tid];
+  }
+  if (tid < Tn.point_end - Tn.point_begin) sPts[(ch.tile_begin & 1) * kTilePoints + tid] = pts[Tn.point_begin + tid];
 
   floatx4 acc[tiles_per_wave<NT>()];
 #pragma unroll
   for (int t = 0; t < tiles_per_wave<NT>(); ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
-    const Tile T = b.tiles[t];
-    const int p0 = b.point_pair[T.point_begin];
-    const int npair = b.point_pair[T.point_end] - p0;
+    const Tile T = Tn;
+    if (t + 1 < ch.tile_end) Tn = b.tiles[t + 1];  // used by the prefetch in (e)
+    const int p0 = T.pair_begin;
+    const int npair = T.pair_end - p0;
     const int npts = T.point_end - T.point_begin;
     const int kpad = (3 * npts + 3) & ~3;
     const int stamp = (t - ch.tile_begin) << 9;
@@ -395,22 +389,22 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     int tp = 0, j = 0;
     bool pair_pf = false, pair_cf = false;
     if (tid < npair) {
-      tp = pre.tp;
+      tp = pre.rec.w;
       const int i = T.point_begin + tp;
-      j = pre.j;
+      j = pre.rec.z;
       pair_pf = i < pf_end;
       pair_cf = j < ncf;
-      const float4 p = pre.p;
+      const float4 p = sPts[(t & 1) * kTilePoints + tp];
       // the pair's terms go straight to its own scratch column (no other thread touches it): the first valid
       // observation stores, later ones (other cameras of a rig) add; a pair without one stores zeros
       bool first = true;
       auto put = [&](float* arr, int k, float v) {
-        float* q = arr + k * kTilePairs + tid;
+        float* q = arr + k * kScr + tid;
         *q = first ? v : *q + v;
       };
-      for (int o = pre.o0; o < pre.o1; ++o) {
-        const float2 xy = o == pre.o0 ? pre.xy : b.obs_xy[o];
-        const float4 inf = o == pre.o0 ? pre.inf : b.obs_info[o];
+      for (int o = pre.rec.x; o < pre.rec.y; ++o) {
+        const float2 xy = o == pre.rec.x ? pre.xy : b.obs_xy[o];
+        const float4 inf = o == pre.rec.x ? pre.inf : b.obs_info[o];
         const int cam = __float_as_int(inf.w);
         Lin L;
         if (!linearize(sPose + 12 * j, sCam + 12 * cam, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L))
@@ -457,12 +451,12 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       }
       if (first) {
         if (pair_cf) {
-          for (int k = 0; k < kUVals; ++k) sScrU[k * kTilePairs + tid] = 0.f;
+          for (int k = 0; k < kUVals; ++k) sScrU[k * kScr + tid] = 0.f;
           if (pair_pf)
-            for (int k = 0; k < 18; ++k) sW[k * kTilePairs + tid] = 0.f;
+            for (int k = 0; k < 18; ++k) sW[k * kScr + tid] = 0.f;
         }
         if (pair_pf)
-          for (int k = 0; k < 9; ++k) sScrV[k * kTilePairs + tid] = 0.f;
+          for (int k = 0; k < 9; ++k) sScrV[k * kScr + tid] = 0.f;
       }
       if (pair_cf) sEnt[tp * kMaxFreePoses + j] = stamp | tid;
     }
@@ -480,7 +474,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
         for (int v = 0; v < kUGroup; ++v)
           if (u0 + v < u1) {
-            const float x = sScrU[(u0 + v) * kTilePairs + col];
+            const float x = sScrU[(u0 + v) * kScr + col];
             accU[v] += ok ? x : 0.f;
           }
       }
@@ -492,9 +486,9 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll 4
       for (int k = a; k < e; ++k) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) V[m] += sScrV[m * kTilePairs + k];
+        for (int m = 0; m < 6; ++m) V[m] += sScrV[m * kScr + k];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) eb[m] += sScrV[(6 + m) * kTilePairs + k];
+        for (int m = 0; m < 3; ++m) eb[m] += sScrV[(6 + m) * kScr + k];
       }
       float G[3][3];
       point_factor(V, lam, G);
@@ -539,8 +533,8 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           float zr[3];
-          const float w0 = sW[(3 * r) * kTilePairs + tid], w1 = sW[(3 * r + 1) * kTilePairs + tid],
-                      w2 = sW[(3 * r + 2) * kTilePairs + tid];
+          const float w0 = sW[(3 * r) * kScr + tid], w1 = sW[(3 * r + 1) * kScr + tid],
+                      w2 = sW[(3 * r + 2) * kScr + tid];
 #pragma unroll
           for (int c = 0; c < 3; ++c) zr[c] = w0 * a[c] + w1 * a[3 + c] + w2 * a[6 + c];
 #pragma unroll
@@ -549,11 +543,20 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
         }
       }
 #pragma unroll
-      for (int r = 0; r < 6; ++r) sEc[r * kTilePairs + tid] = ec[r];
+      for (int r = 0; r < 6; ++r) sEc[r * kScr + tid] = ec[r];
     }
     lds_barrier();
-    // (e) the next tile's pair data is requested; the e owners sum the e_c terms; matrix cores: acc += Zc Zc^T
-    if (t + 1 < ch.tile_end) prefetch_pair(b, pts, t + 1, tid, pre);
+    // (e) the next tile's pair data is requested; the e owners sum the e_c terms; matrix cores: acc += Zc Zc^T;
+    // the next tile's points go to the other half of sPts (read after the next barrier)
+    const bool has_next = t + 1 < ch.tile_end;
+    float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int npts_next = has_next ? Tn.point_end - Tn.point_begin : 0;
+    if (has_next && tid < Tn.pair_end - Tn.pair_begin) {
+      pre.rec = b.pair_rec[Tn.pair_begin + tid];
+      pre.xy = b.pair_xy[Tn.pair_begin + tid];
+      pre.inf = b.pair_info[Tn.pair_begin + tid];
+    }
+    if (tid < npts_next) pn = pts[Tn.point_begin + tid];
     if (u_owner && u1 > 21) {
       int ent[kTilePoints];
 #pragma unroll
@@ -565,7 +568,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
         for (int v = 0; v < kUGroup; ++v)
           if (u0 + v >= 21 && u0 + v < u1) {
-            const float x = sEc[(u0 + v - 21) * kTilePairs + col];
+            const float x = sEc[(u0 + v - 21) * kScr + col];
             accU[v] += ok ? x : 0.f;
           }
       }
@@ -576,6 +579,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       case 2: mfma_tile<NT, 2>(sZc, kpad / 4, lane, acc); break;
       default: mfma_tile<NT, 3>(sZc, kpad / 4, lane, acc); break;
     }
+    if (tid < npts_next) sPts[((t + 1) & 1) * kTilePoints + tid] = pn;
   }
   // the chunk's record: Schur tiles (lane order), then U (21 per key frame), e (6), diag U (6)
   float* rec = b.part + ch.part_off;

@@ -85,7 +85,9 @@ struct HostPack {
   std::vector<float2> obs_xy;
   std::vector<float4> obs_info;
   std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
-  std::vector<uint8_t> pair_tp;
+  std::vector<int4> pair_rec;
+  std::vector<float2> pair_xy;
+  std::vector<float4> pair_info;
   std::vector<float4> pts, poses, cams;
   std::vector<Tile> tiles, btiles;
 };
@@ -183,21 +185,31 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
         pairs += pi;
         ++i;
       }
-      out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pairs});
+      const int pb = H.point_pair[W.point_off + begin];
+      out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pb, pb + pairs, {0, 0, 0}});
     }
   };
   const size_t first_tile = H.tiles.size();
   make_tiles(H.tiles, kTilePoints);
   make_tiles(H.btiles, kBlock);
-  // each pair's point index inside its Schur tile
-  H.pair_tp.resize(H.pair_pose.size());
+  // pair records: observation range, key frame, point index inside its Schur tile; first observation by pair
+  const size_t first_pair = H.pair_rec.size();
+  H.pair_rec.resize(H.pair_pose.size());
   for (size_t t = first_tile; t < H.tiles.size(); ++t) {
     const Tile& T = H.tiles[t];
     for (int i = T.point_begin; i < T.point_end; ++i) {
       const int a = H.point_pair[i];
       const int e = i + 1 < static_cast<int>(H.point_pair.size()) ? H.point_pair[i + 1] : static_cast<int>(H.pair_pose.size());
-      for (int q = a; q < e; ++q) H.pair_tp[q] = static_cast<uint8_t>(i - T.point_begin);
+      for (int q = a; q < e; ++q) {
+        const int ob = H.pair_obs[q];
+        const int oe = q + 1 < static_cast<int>(H.pair_obs.size()) ? H.pair_obs[q + 1] : static_cast<int>(H.obs_xy.size());
+        H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], i - T.point_begin);
+      }
     }
+  }
+  for (size_t q = first_pair; q < H.pair_rec.size(); ++q) {
+    H.pair_xy.push_back(H.obs_xy[H.pair_rec[q].x]);
+    H.pair_info.push_back(H.obs_info[H.pair_rec[q].x]);
   }
   return SBA_OK;
 }
@@ -232,6 +244,13 @@ int free_batch(sba_batch* b) {
   for (void* p : b->allocs) (void)hipFree(p);
   delete b;
   return SBA_OK;
+}
+
+// Pairs of work per chunk when the options leave it open: about 32 chunks per compute unit, so the hardware
+// dispatcher balances uneven windows, but at least 1024 pairs so that a chunk's record stays small against its
+// work (measured on the 512-window batch: 3000 pairs per chunk is within 2% of the best of 1000..25000).
+int64_t default_chunk_pairs(int64_t total_pairs) {
+  return std::min<int64_t>(std::max<int64_t>(1024, total_pairs / 8192), 65536);
 }
 
 // The launch sequence of one solve. With events, each kernel class is timed (eager mode only).
@@ -373,7 +392,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
 
   // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each; each owns one record
   int64_t target = h->opt.chunk_pairs;
-  if (target <= 0) target = std::max<int64_t>(1024, b->total_pairs / 512);
+  if (target <= 0) target = default_chunk_pairs(b->total_pairs);
   std::vector<Chunk> chunks;
   int64_t part_off = 0;
   {
@@ -390,7 +409,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
       const size_t begin = t;
       int64_t pairs = 0;
       while (t < H.tiles.size() && H.tiles[t].window == w && (t == begin || pairs < target)) {
-        pairs += H.tiles[t].pad;
+        pairs += H.tiles[t].pair_end - H.tiles[t].pair_begin;
         ++t;
       }
       chunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), part_off});
@@ -407,14 +426,12 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
       const size_t begin = t;
       int64_t pairs = 0;
       while (t < H.btiles.size() && H.btiles[t].window == w && (t == begin || pairs < target)) {
-        pairs += H.btiles[t].pad;
+        pairs += H.btiles[t].pair_end - H.btiles[t].pair_begin;
         ++t;
       }
       bchunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), 0});
     }
   }
-  for (auto& T : H.tiles) T.pad = 0;
-  for (auto& T : H.btiles) T.pad = 0;
 
   Batch& d = b->dev;
   d.n_windows = n;
@@ -441,7 +458,15 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(H.obs_info, obs_info);
   UP(H.pair_obs, pair_obs);
   UP(H.pair_pose, pair_pose);
-  UP(H.pair_tp, pair_tp);
+  UP(H.pair_rec, pair_rec);
+  if (H.obs_xy.size() == H.pair_rec.size()) {
+    // every pair has exactly one observation: the observation streams are already in pair order
+    d.pair_xy = d.obs_xy;
+    d.pair_info = d.obs_info;
+  } else {
+    UP(H.pair_xy, pair_xy);
+    UP(H.pair_info, pair_info);
+  }
   UP(H.point_pair, point_pair);
   UP(H.point_nfree, point_nfree);
   UP(H.pts, pts_init);
@@ -483,9 +508,12 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   d.lds_ent = off;
   off += kTilePoints * kMaxFreePoses;
   d.lds_ec = off;
-  off += 6 * kTilePairs;
+  off += 6 * (kTilePairs + 1);
   d.lds_w = off;
-  off += 18 * kTilePairs;
+  off += 18 * (kTilePairs + 1);
+  off = (off + 3) & ~3;  // float4 staging
+  d.lds_pts = off;
+  off += 2 * 4 * kTilePoints;
   off = (off + 63) & ~63;
   d.lds_zc = off;
   off += kKt * kZcLd;
@@ -976,6 +1004,63 @@ int sba_batch_refine(sba_handle* h, sba_batch* b, int32_t gauss_newton_steps, in
       }
   }
   return SBA_OK;
+}
+
+int sba_autotune_chunks(sba_handle* h, const sba_problem_view* problems, int32_t n, const int32_t* candidates,
+                        int32_t n_candidates, int32_t repeats, int32_t* best_chunk_pairs, float* ms_per_candidate) {
+  if (!h || !problems || n <= 0 || !best_chunk_pairs || repeats < 1 || n_candidates < 0 || (n_candidates > 0 && !candidates))
+    return fail(h, SBA_E_INVALID, "bad sba_autotune_chunks arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  int64_t total_pairs = 0;
+  {
+    // the default candidates bracket the heuristic: x1/4 .. x4
+    sba_batch* probe = nullptr;
+    const int32_t saved = h->opt.chunk_pairs;
+    h->opt.chunk_pairs = 0;
+    const int rc = sba_batch_create(h, problems, n, &probe);
+    h->opt.chunk_pairs = saved;
+    if (rc != SBA_OK) return rc;
+    total_pairs = probe->total_pairs;
+    sba_batch_destroy(h, probe);
+  }
+  std::vector<int32_t> cand(candidates, candidates + n_candidates);
+  if (cand.empty()) {
+    const int64_t base = default_chunk_pairs(total_pairs);
+    for (int64_t c : {base / 4, base / 2, base, 2 * base, 4 * base})
+      cand.push_back(static_cast<int32_t>(std::max<int64_t>(256, c)));
+  }
+  const int32_t saved = h->opt.chunk_pairs;
+  float best = 0.f;
+  int32_t best_c = cand[0];
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = SBA_OK;
+  for (size_t k = 0; k < cand.size() && rc == SBA_OK; ++k) {
+    h->opt.chunk_pairs = cand[k];
+    sba_batch* b = nullptr;
+    rc = sba_batch_create(h, problems, n, &b);
+    if (rc != SBA_OK) break;
+    float ms = 0.f;
+    if ((rc = sba_batch_run(h, b)) == SBA_OK && hipEventRecord(e0, h->stream) == hipSuccess) {
+      for (int r = 0; r < repeats && rc == SBA_OK; ++r) rc = sba_batch_run(h, b);
+      if (rc == SBA_OK && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(h, SBA_E_DEVICE, "autotune timing failed");
+    }
+    sba_batch_destroy(h, b);
+    ms /= static_cast<float>(repeats);
+    if (ms_per_candidate && k < static_cast<size_t>(n_candidates)) ms_per_candidate[k] = ms;
+    if (rc == SBA_OK && (k == 0 || ms < best)) {
+      best = ms;
+      best_c = cand[k];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->opt.chunk_pairs = saved;
+  if (rc == SBA_OK) *best_chunk_pairs = best_c;
+  return rc;
 }
 
 int sba_batch_adjoint(sba_handle* h, sba_batch* b, const double* g, double* dxy, double* dinfo, int32_t max_cg,

@@ -141,6 +141,21 @@ class SchurComplementBundlerGpu:
     def last_error(self) -> str:
         return self._L.sba_last_error(self._h).decode()
 
+    def autotune(self, problems: Sequence[BundleAdjustmentProblem], candidates: Optional[Sequence[int]] = None,
+                 repeats: int = 2):
+        """Times the batched solve of ``problems`` for each chunk size (pairs of work per Schur chunk) and returns
+        (best chunk_pairs, [ms per solve of each given candidate]). Use it as ``chunk_pairs`` of a new solver."""
+        views = (_lib.ProblemView * len(problems))(*[p.view() for p in problems])
+        cand = np.ascontiguousarray(candidates if candidates is not None else [], dtype=np.int32)
+        ms = np.zeros(max(len(cand), 1), np.float32)
+        best = C.c_int32()
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+        rc = self._L.sba_autotune_chunks(self._h, views, len(problems), ip(cand) if len(cand) else None, len(cand),
+                                         repeats, C.byref(best), ms.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.last_error())
+        return int(best.value), [float(x) for x in ms[:len(cand)]]
+
     def solve(self, problem: BundleAdjustmentProblem) -> bool:
         v = problem.view()
         r, op, oq = _result_for(problem)

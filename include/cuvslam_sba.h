@@ -125,6 +125,15 @@ int sba_batch_info(sba_handle* h, sba_batch* b, int64_t* num_observations, int64
                    int32_t* num_chunks, int32_t* num_windows);
 int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int32_t* launches_per_class);
 
+/* Autotuner of the batched launch's work granularity (pairs per Schur chunk; the back-substitution chunks follow
+ * it). Each candidate builds the batch, runs one warm-up solve and times `repeats` solves with HIP events on the
+ * handle's stream; the fastest is returned in best_chunk_pairs (set it as sba_options.chunk_pairs of a handle).
+ * candidates == NULL or n_candidates == 0 tries the built-in heuristic scaled by 1/4 .. 4; ms_per_candidate
+ * [optional, n_candidates] receives the mean time per solve of each given candidate. No reference counterpart
+ * (the reference solves one window per call). */
+int sba_autotune_chunks(sba_handle* h, const sba_problem_view* problems, int32_t n, const int32_t* candidates,
+                        int32_t n_candidates, int32_t repeats, int32_t* best_chunk_pairs, float* ms_per_candidate);
+
 /* One LM step of window 0 of a batch from its current state with damping lambda, for phase-level parity
  * tests (the reference tests its phases the same way: libs/cuda_modules/test/sba_test.cpp).
  * Outputs (any may be null):

@@ -158,3 +158,19 @@ def test_batch_matches_single(solver_huber):
     for a, s in zip(batch, singles):
         assert a.iterations == s.iterations
         assert abs(a.last_cost_f64 - s.last_cost_f64) / s.last_cost_f64 < 1e-6
+
+
+def test_autotuner_and_chunking_leave_results_unchanged():
+    # the work granularity changes only the order of the FP32 record sums; after 5 LM iterations (not yet
+    # converged) that rounding difference shows in the cost at the 1e-6 level
+    probs = ca.scene.batched_windows(6, seed=7, max_iterations=5)
+    s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    best, ms = s.autotune(probs, [512, 2048, 8192], repeats=1)
+    assert best in (512, 2048, 8192) and len(ms) == 3 and all(m > 0 for m in ms)
+    costs = []
+    for cp in (512, 8192):
+        qs = [p.copy() for p in probs]
+        sv = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=cp)
+        assert all(sv.solve_batch(qs))
+        costs.append(np.array([q.last_cost_f64 for q in qs]))
+    np.testing.assert_allclose(costs[0], costs[1], rtol=1e-5)
