@@ -40,6 +40,20 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter passes (profiles/hbm_traffic_pmc.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1] == workload and parts[2] == kernel:
+                    return float(parts[3])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +65,8 @@ def main():
     ap.add_argument("--chunk-pairs", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device copy kernel (k_copy_calibrate), the reference of HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SBA batch is sharded across GPUs by window with no exchange, "
@@ -90,6 +106,9 @@ def main():
     info = batch.info()
     log(f"[bench] packed and uploaded in {time.time() - t0:.1f}s: {info}")
 
+    if args.calibrate_copy:
+        ms_copy = solver.debug_copy(256 << 20, repeats=3)
+        log(f"[bench] calibration copy of 256 MiB: {ms_copy:.3f} ms ({2 * 256 * 1.048576 / ms_copy:.0f} GB/s)")
     for _ in range(args.warmup):
         batch.run()
     batch.sync()
@@ -122,7 +141,7 @@ def main():
     else:
         ach, peak, unit, bound = bytes_ / t_launch / 1e9, RF.HBM_PEAK_GBS, "GB/s", "hbm"
     roof = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": unit, "frac": round(ach / peak, 6),
-            "traffic": None, "kernel": dom, "launch_ms": round(per_launch[dom], 5),
+            "traffic": measured_traffic(args.workload, dom), "kernel": dom, "launch_ms": round(per_launch[dom], 5),
             "algorithmic": {"bytes": bytes_, "flops_f32": f32, "flops_f64": f64}}
 
     # accuracy against the FP64 oracle (window 0)

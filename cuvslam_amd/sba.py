@@ -156,6 +156,14 @@ class SchurComplementBundlerGpu:
             raise SbaError(rc, self.last_error())
         return int(best.value), [float(x) for x in ms[:len(cand)]]
 
+    def debug_copy(self, nbytes: int, repeats: int = 3) -> float:
+        """Device-to-device copy of nbytes by a plain kernel (the calibration of HBM byte counters); ms per copy."""
+        ms = C.c_float()
+        rc = self._L.sba_debug_copy(self._h, int(nbytes), int(repeats), C.byref(ms))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.last_error())
+        return ms.value
+
     def solve(self, problem: BundleAdjustmentProblem) -> bool:
         v = problem.view()
         r, op, oq = _result_for(problem)
