@@ -53,6 +53,19 @@ class SceneConfig(C.Structure):
     ]
 
 
+class LoopConfig(C.Structure):
+    _fields_ = [
+        ("num_keyframes", C.c_int32), ("num_points", C.c_int32), ("num_stereo_pairs", C.c_int32),
+        ("num_fixed_keyframes", C.c_int32), ("loop_length_m", C.c_double), ("stereo_baseline_m", C.c_double),
+        ("corridor_half_m", C.c_double), ("floor_y_m", C.c_double), ("ceiling_y_m", C.c_double),
+        ("clutter_fraction", C.c_double), ("max_range_m", C.c_double), ("fx", C.c_double), ("fy", C.c_double),
+        ("cx", C.c_double), ("cy", C.c_double), ("width", C.c_int32), ("height", C.c_int32),
+        ("keyframe_rotation_deg", C.c_double), ("pixel_noise", C.c_double), ("outlier_rate", C.c_double),
+        ("pose_sigma_m", C.c_double), ("pose_sigma_deg", C.c_double), ("point_sigma_m", C.c_double),
+        ("huber_px", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
 # Every exported symbol of include/cuvslam_sba.h with its signature.
 SIGNATURES = {
     "sba_default_options": (None, [C.POINTER(Options)]),
@@ -73,6 +86,8 @@ SIGNATURES = {
     "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     "sba_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
     "sba_autotune_chunks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32, _ip, _fp]),
+    "sba_loop_default_config": (None, [C.POINTER(LoopConfig)]),
+    "sba_scene_generate_loop": (C.c_int, [C.POINTER(LoopConfig), C.POINTER(C.c_void_p)]),
     "sba_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp, _dp,
                                    _dp]),
     "sba_debug_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _fp]),

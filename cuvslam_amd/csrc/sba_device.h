@@ -43,6 +43,7 @@ constexpr int kZcLd = 240;         // leading dimension of the staged Z columns 
 constexpr int kKt = 48;            // Schur columns per tile (3 per point)
 constexpr int kFactorThreads = 512;
 constexpr int kPanel = 8;          // Cholesky panel width
+constexpr float kFltEps = 1.1920928955078125e-7f;  // FLT_EPSILON, common/types.h:203-206
 
 struct WindowDesc {
   int32_t np, npf, nc, ncf, ncam, no;
@@ -57,6 +58,12 @@ struct WindowDesc {
   int32_t max_iterations;
   float robust;
   int32_t factor_in_lds;
+  // windows with more than kMaxFreePoses free key frames ("mapped"): each chunk stages its own <= kMaxFreePoses
+  // free key frames; chunk records are scattered into a dense FP32 sum (big) and a dense FP64 factor in A
+  int32_t mapped;
+  int32_t chunk_begin;  // the window's first Schur chunk
+  int64_t big_off;      // floats into big: sum Z Z^T, n x n row-major, upper triangle (n = 6 ncf)
+  int64_t ue_off;       // floats into big_ue: U (21 ncf), e (6 ncf), diag U (6 ncf)
 };
 
 struct WindowState {
@@ -76,6 +83,10 @@ struct Tile {
 struct Chunk {
   int32_t tile_begin, tile_end;
   int64_t part_off;  // floats into part: this chunk's record
+  int32_t nloc;      // free key frames staged by the chunk (the window's free key frames unless mapped)
+  int32_t nt;        // 16x16 tiles per side of the chunk's staged system, ceil(6 nloc / 16)
+  int32_t map_off;   // ints into chunk_map: the chunk's free key frames in increasing order (mapped windows)
+  int32_t window;
 };
 
 struct Batch {
@@ -93,7 +104,7 @@ struct Batch {
   const int32_t* pair_obs;
   const int32_t* pair_pose;
   const int4* pair_rec;      // per pair: (first observation, end of its observations, key frame, point index
-                             // inside its Schur tile)
+                             // inside its Schur tile | local free key frame index << 8, 255 when fixed)
   const float2* pair_xy;     // per pair: its first observation (the obs streams themselves when every pair has
   const float4* pair_info;   // exactly one observation)
   const int32_t* point_pair;
@@ -106,6 +117,9 @@ struct Batch {
   float4* poses1;
   const float4* poses_init;
   const float4* cams;
+  const int32_t* chunk_map;  // local -> window free key frame, per chunk of a mapped window
+  float* big;        // mapped windows: dense sums of Z Z^T
+  float* big_ue;     // mapped windows: U, e, diag U
   float* part;       // per-chunk Schur records
   float* sred;       // per-window reduced records
   double* A;
@@ -137,6 +151,57 @@ __device__ __forceinline__ void polar3(double X[3][3]) {
     const double C[3][3] = {{c00, c01, c02}, {c10, c11, c12}, {c20, c21, c22}};
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) X[i][j] = 0.5 * (X[i][j] + C[i][j] * id);
+  }
+}
+
+
+// Exp of a twist (omega, v) in FP64, the formula of math/twist.cpp:23-41 (theta clamped at FLT_EPSILON).
+__device__ __forceinline__ void twist_exp64(const double* tw, double R[3][3], double t[3]) {
+  const double w0 = tw[0], w1 = tw[1], w2 = tw[2];
+  const double theta = fmax(sqrt(w0 * w0 + w1 * w1 + w2 * w2), static_cast<double>(kFltEps));
+  const double c = cos(theta), st = sin(theta) / theta;
+  const double n[3] = {w0 / theta, w1 / theta, w2 / theta};
+  const double ws[3] = {w0 * st, w1 * st, w2 * st};
+  const double K[3][3] = {{c, -ws[2], ws[1]}, {ws[2], c, -ws[0]}, {-ws[1], ws[0], c}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i][j] = K[i][j] + (1.0 - c) * n[i] * n[j];
+  const double* v = tw + 3;
+  const double nxv[3] = {n[1] * v[2] - n[2] * v[1], n[2] * v[0] - n[0] * v[2], n[0] * v[1] - n[1] * v[0]};
+  const double nnv[3] = {n[1] * nxv[2] - n[2] * nxv[1], n[2] * nxv[0] - n[0] * nxv[2], n[0] * nxv[1] - n[1] * nxv[0]};
+  for (int i = 0; i < 3; ++i) t[i] = v[i] + ((1.0 - c) / theta * nxv[i] + (1.0 - st) * nnv[i]);
+}
+
+// Trial key frames of window W from the FP64 pose step dc: Exp(dc_j) * T_j, rotation re-orthonormalized, rounded
+// to FP32 into the non-current pose buffer; fixed key frames are copied.
+__device__ __forceinline__ void write_trial_poses(const Batch& b, const WindowDesc& W, int cur, const double* dc,
+                                                  int tid, int nthreads) {
+  const int ncf = W.ncf;
+  const float4* src = (cur ? b.poses1 : b.poses0) + 3 * W.pose_off;
+  float4* dst = (cur ? b.poses0 : b.poses1) + 3 * W.pose_off;
+  for (int j = tid; j < W.nc; j += nthreads) {
+    const float4 r0 = src[3 * j], r1 = src[3 * j + 1], r2 = src[3 * j + 2];
+    if (j >= ncf) {
+      dst[3 * j] = r0;
+      dst[3 * j + 1] = r1;
+      dst[3 * j + 2] = r2;
+      continue;
+    }
+    double E[3][3], et[3];
+    twist_exp64(dc + 6 * j, E, et);
+    const double R[3][3] = {{r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, {r2.x, r2.y, r2.z}};
+    const double t[3] = {r0.w, r1.w, r2.w};
+    double X[3][3], tn[3];
+    for (int a = 0; a < 3; ++a) {
+      for (int c = 0; c < 3; ++c) X[a][c] = E[a][0] * R[0][c] + E[a][1] * R[1][c] + E[a][2] * R[2][c];
+      tn[a] = E[a][0] * t[0] + E[a][1] * t[1] + E[a][2] * t[2] + et[a];
+    }
+    polar3(X);
+    dst[3 * j] = make_float4(static_cast<float>(X[0][0]), static_cast<float>(X[0][1]), static_cast<float>(X[0][2]),
+                             static_cast<float>(tn[0]));
+    dst[3 * j + 1] = make_float4(static_cast<float>(X[1][0]), static_cast<float>(X[1][1]), static_cast<float>(X[1][2]),
+                                 static_cast<float>(tn[1]));
+    dst[3 * j + 2] = make_float4(static_cast<float>(X[2][0]), static_cast<float>(X[2][1]), static_cast<float>(X[2][2]),
+                                 static_cast<float>(tn[2]));
   }
 }
 

@@ -32,7 +32,7 @@ using namespace sba_dev;
 namespace {
 
 constexpr float kHither = 0.1f;                  // epipolar/near_plane.h:33-38
-constexpr float kFltEps = 1.1920928955078125e-7f;  // common/types.h:203-206
+
 constexpr float kSqrtFltEps = 3.4526698300124393e-4f;
 
 __device__ __forceinline__ float drho(int kind, float s, float d) {
@@ -341,6 +341,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int ncf = W.ncf;
+  const int nloc = ch.nloc;  // free key frames staged by this chunk (local index jl; = ncf unless mapped)
   float* sPose = sm + b.lds_pose;
   float* sCam = sm + b.lds_cam;
   float* sAcc = sm + b.lds_acc;  // 12 per tile point: G (9) and G^T e_b (3)
@@ -355,7 +356,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const float delta = W.robust;
   const int pf_end = W.point_off + W.npf;
   // U owner: key frame uj, values [u0, u1)
-  const bool u_owner = tid < kUGroups * ncf;
+  const bool u_owner = tid < kUGroups * nloc;
   const int uj = tid / kUGroups, u0 = kUGroup * (tid % kUGroups), u1 = min(u0 + kUGroup, kUVals);
   float accU[kUGroup] = {0.f, 0.f, 0.f, 0.f, 0.f};
   // V owner: tile point vp
@@ -386,10 +387,11 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     const int stamp = (t - ch.tile_begin) << 9;
     lds_barrier();  // the previous tile's shared arrays are free
     // (b) one thread per pair: linearize its observations (one per camera), store W and the U and V terms
-    int tp = 0, j = 0;
+    int tp = 0, j = 0, jl = 0;
     bool pair_pf = false, pair_cf = false;
     if (tid < npair) {
-      tp = pre.rec.w;
+      tp = pre.rec.w & 255;
+      jl = pre.rec.w >> 8;
       const int i = T.point_begin + tp;
       j = pre.rec.z;
       pair_pf = i < pf_end;
@@ -458,7 +460,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
         if (pair_pf)
           for (int k = 0; k < 9; ++k) sScrV[k * kScr + tid] = 0.f;
       }
-      if (pair_cf) sEnt[tp * kMaxFreePoses + j] = stamp | tid;
+      if (pair_cf) sEnt[tp * kMaxFreePoses + jl] = stamp | tid;
     }
     lds_barrier();
     // (c) owners sum the pair terms: U and e_a of key frame uj into registers; V, e_b of point vp, then its
@@ -529,7 +531,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       float ec[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (pair_pf) {
         const float* a = sAcc + 12 * tp;
-        float* zc = sZc + 3 * tp * kZcLd + 6 * j;
+        float* zc = sZc + 3 * tp * kZcLd + 6 * jl;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           float zr[3];
@@ -583,7 +585,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   }
   // the chunk's record: Schur tiles (lane order), then U (21 per key frame), e (6), diag U (6)
   float* rec = b.part + ch.part_off;
-  if (W.nt > 0) switch (tid >> 6) {
+  if (nt > 0 && ch.nt > 0) switch (tid >> 6) {
     case 0: store_tiles<NT, 0>(rec, lane, acc); break;
     case 1: store_tiles<NT, 1>(rec, lane, acc); break;
     case 2: store_tiles<NT, 2>(rec, lane, acc); break;
@@ -600,9 +602,9 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
         ru[21 * uj + u] = accU[v];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
-          if (u == kDiag[r]) ru[27 * ncf + 6 * uj + r] = accU[v];
+          if (u == kDiag[r]) ru[27 * nloc + 6 * uj + r] = accU[v];
       } else {
-        ru[21 * ncf + 6 * uj + (u - 21)] = accU[v];
+        ru[21 * nloc + 6 * uj + (u - 21)] = accU[v];
       }
     }
   }
@@ -613,7 +615,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   extern __shared__ float sm[];
   const Chunk ch = b.chunks[blockIdx.x];
-  const int w = b.tiles[ch.tile_begin].window;
+  const int w = ch.window;
   const WindowDesc W = b.win[w];
   const WindowState* St = b.st + w;
   if (St->done) return;
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
   load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
   static_assert(kNT == 15, "the dispatch below covers 1..15 tiles");
-  switch (W.nt) {
+  switch (ch.nt) {
     case 0:  // no free key frame: only the point blocks (V*^-1, V*^-1 e_b) are needed; the record is empty
     case 1: schur_chunk<1>(b, ch, W, lam, cur, sm); break;
     case 2: schur_chunk<2>(b, ch, W, lam, cur, sm); break;
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
 __global__ __launch_bounds__(64) void k_reduce(Batch b) {
   const int w = blockIdx.y;
   const WindowDesc W = b.win[w];
-  if (b.st[w].done) return;
+  if (b.st[w].done || W.mapped) return;
   const float* src = b.part + W.part_off;
   float* dst = b.sred + W.sred_off;
   const int64_t rec = W.rec;
@@ -678,21 +680,6 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
 
 __device__ __forceinline__ int64_t pk(int i, int j) { return static_cast<int64_t>(i) * (i + 1) / 2 + j; }
 
-// Exp of a twist (omega, v) in FP64, the formula of math/twist.cpp:23-41 (theta clamped at FLT_EPSILON).
-__device__ void twist_exp64(const double* tw, double R[3][3], double t[3]) {
-  const double w0 = tw[0], w1 = tw[1], w2 = tw[2];
-  const double theta = fmax(sqrt(w0 * w0 + w1 * w1 + w2 * w2), static_cast<double>(kFltEps));
-  const double c = cos(theta), st = sin(theta) / theta;
-  const double n[3] = {w0 / theta, w1 / theta, w2 / theta};
-  const double ws[3] = {w0 * st, w1 * st, w2 * st};
-  const double K[3][3] = {{c, -ws[2], ws[1]}, {ws[2], c, -ws[0]}, {-ws[1], ws[0], c}};
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) R[i][j] = K[i][j] + (1.0 - c) * n[i] * n[j];
-  const double* v = tw + 3;
-  const double nxv[3] = {n[1] * v[2] - n[2] * v[1], n[2] * v[0] - n[0] * v[2], n[0] * v[1] - n[1] * v[0]};
-  const double nnv[3] = {n[1] * nxv[2] - n[2] * nxv[1], n[2] * nxv[0] - n[0] * nxv[2], n[0] * nxv[1] - n[1] * nxv[0]};
-  for (int i = 0; i < 3; ++i) t[i] = v[i] + ((1.0 - c) / theta * nxv[i] + (1.0 - st) * nnv[i]);
-}
 
 
 }  // namespace
@@ -702,7 +689,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   const int w = blockIdx.x;
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
-  if (St->done) return;
+  if (St->done || W.mapped) return;  // mapped windows: sba_large.hip
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int warp = tid >> 6;
@@ -914,31 +901,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
       St->maxdc = static_cast<float>(m);
     }
     // 5. trial key frames: Exp(dc_j) * T_j, rotation re-orthonormalized, rounded to FP32
-    const int cur = St->cur;
-    const float4* src = (cur ? b.poses1 : b.poses0) + 3 * W.pose_off;
-    float4* dst = (cur ? b.poses0 : b.poses1) + 3 * W.pose_off;
-    for (int j = tid; j < W.nc; j += kFactorThreads) {
-      const float4 r0 = src[3 * j], r1 = src[3 * j + 1], r2 = src[3 * j + 2];
-      if (j >= ncf) {
-        dst[3 * j] = r0;
-        dst[3 * j + 1] = r1;
-        dst[3 * j + 2] = r2;
-        continue;
-      }
-      double E[3][3], et[3];
-      twist_exp64(vec + 6 * j, E, et);
-      const double R[3][3] = {{r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, {r2.x, r2.y, r2.z}};
-      const double t[3] = {r0.w, r1.w, r2.w};
-      double X[3][3], tn[3];
-      for (int a = 0; a < 3; ++a) {
-        for (int c = 0; c < 3; ++c) X[a][c] = E[a][0] * R[0][c] + E[a][1] * R[1][c] + E[a][2] * R[2][c];
-        tn[a] = E[a][0] * t[0] + E[a][1] * t[1] + E[a][2] * t[2] + et[a];
-      }
-      polar3(X);
-      dst[3 * j] = make_float4(static_cast<float>(X[0][0]), static_cast<float>(X[0][1]), static_cast<float>(X[0][2]), static_cast<float>(tn[0]));
-      dst[3 * j + 1] = make_float4(static_cast<float>(X[1][0]), static_cast<float>(X[1][1]), static_cast<float>(X[1][2]), static_cast<float>(tn[1]));
-      dst[3 * j + 2] = make_float4(static_cast<float>(X[2][0]), static_cast<float>(X[2][1]), static_cast<float>(X[2][2]), static_cast<float>(tn[2]));
-    }
+    write_trial_poses(b, W, St->cur, vec, tid, kFactorThreads);
   }
   if (tid == 0) St->chol_ok = ok ? 1 : 0;
 }
@@ -953,7 +916,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   __shared__ float sCam[12 * kMaxCameras];
   __shared__ float sAcc[4 * kTilePoints];  // sum_j W_ij^T dc_j, then dp
   __shared__ int sPP[kTilePairs];
-  __shared__ float sDc[6 * kMaxFreePoses];
+  __shared__ float sDc[6 * kMaxPoses];
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];
   __shared__ float sRedM[kBlock / 64];

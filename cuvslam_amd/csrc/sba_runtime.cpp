@@ -33,6 +33,9 @@ hipError_t launch_adj_vec(const Batch& b, double* x, double* r, double* z, doubl
                           double* acc, double* g, double* sc, int32_t* done, double tol, int nx, int n6, int op,
                           hipStream_t s);
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
+hipError_t launch_large_zero(const Batch& b, int w, hipStream_t s);
+hipError_t launch_large_scatter(const Batch& b, int w, int nchunks, hipStream_t s);
+hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s);
 hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_gather(const Batch& b, hipStream_t s);
 hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s);
@@ -61,6 +64,7 @@ struct sba_batch {
   hipGraphExec_t gexec = nullptr;
   double* dbg_S = nullptr;  // debug buffer of the reduced system of window 0
   int32_t* obs_orig = nullptr;  // packed observation -> index in its window's input order
+  std::vector<int> mapped;      // windows with more than kMaxFreePoses free key frames
   double* adj = nullptr;        // adjoint workspace of window 0 (allocated on first use)
   bool adj_ready = false;       // the workspace holds the factor and the FP64 state of the current solution
   hipEvent_t ev[8] = {};
@@ -102,7 +106,6 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       (no && (!p.observation_xys || !p.observation_infos || !p.point_ids || !p.pose_ids || !p.camera_ids)))
     return fail(h, SBA_E_INVALID, "null buffer in window " + std::to_string(w));
   const int ncf = nc - p.num_fixed_key_frames;
-  if (ncf > kMaxFreePoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 40 free key frames");
   if (nc > kMaxPoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 256 key frames");
   if (ncam > kMaxCameras) return fail(h, SBA_E_CAPACITY, "more than 32 cameras");
   for (int o = 0; o < no; ++o) {
@@ -124,6 +127,7 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   W.cam_off = static_cast<int32_t>(H.cams.size() / 3);
   W.max_iterations = p.max_iterations;
   W.robust = p.robustifier_scale;
+  W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
   // counting sort by point, then (key frame, camera) inside each point
   std::vector<int32_t> start(np + 1, 0);
@@ -172,16 +176,41 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   };
   for (int j = 0; j < nc; ++j) push_pose(H.poses, p.rig_from_world + 16 * j);
   for (int c = 0; c < ncam; ++c) push_pose(H.cams, p.camera_from_rig + 16 * c);
-  // tiles: consecutive points with at most kTilePairs pairs and max_points points
+  // tiles: consecutive points with at most kTilePairs pairs and max_points points; in a mapped window also at
+  // most kMaxFreePoses distinct free key frames (one tile is staged by one chunk)
+  auto pairs_of = [&](int i) {
+    return static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
+                            H.point_pair[W.point_off + i]);
+  };
+  std::vector<uint8_t> in_tile(W.mapped ? ncf : 0);
+  for (int i = 0; i < np && W.mapped; ++i) {
+    int nfree = 0;
+    for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pairs_of(i); ++q) nfree += H.pair_pose[q] < ncf;
+    if (nfree > kMaxFreePoses)
+      return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + ": a point is observed from more than 40 free key frames");
+  }
   auto make_tiles = [&](std::vector<Tile>& out, int max_points) {
     int i = 0;
     while (i < np) {
       const int begin = i;
-      int pairs = 0;
+      int pairs = 0, nkf = 0;
+      std::fill(in_tile.begin(), in_tile.end(), 0);
       while (i < np && i - begin < max_points) {
-        const int pi = static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
-                                        H.point_pair[W.point_off + i]);
+        const int pi = pairs_of(i);
         if (pairs + pi > kTilePairs && i > begin) break;
+        if (W.mapped && max_points == kTilePoints) {
+          int add = 0;
+          for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pi; ++q) {
+            const int j = H.pair_pose[q];
+            if (j < ncf && !in_tile[j]) ++add;
+          }
+          if (nkf + add > kMaxFreePoses && i > begin) break;
+          for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pi; ++q) {
+            const int j = H.pair_pose[q];
+            if (j < ncf && !in_tile[j]) in_tile[j] = 1;
+          }
+          nkf += add;
+        }
         pairs += pi;
         ++i;
       }
@@ -203,7 +232,9 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       for (int q = a; q < e; ++q) {
         const int ob = H.pair_obs[q];
         const int oe = q + 1 < static_cast<int>(H.pair_obs.size()) ? H.pair_obs[q + 1] : static_cast<int>(H.obs_xy.size());
-        H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], i - T.point_begin);
+        // local free key frame index: the key frame itself unless the window is mapped (set with its chunks)
+        const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
+        H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - T.point_begin) | (jl << 8));
       }
     }
   }
@@ -262,11 +293,14 @@ int enqueue_solve(sba_handle* h, sba_batch* b, bool timed) {
   HIP_TRY(launch_decide(d, true, s));
   for (int it = 0; it < b->max_iter; ++it) {
     if (timed) HIP_TRY(hipEventRecord(b->ev[0], s));
+    for (int w : b->mapped) HIP_TRY(launch_large_zero(d, w, s));
     HIP_TRY(launch_schur(d, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[1], s));
     HIP_TRY(launch_reduce(d, s));
+    for (int w : b->mapped) HIP_TRY(launch_large_scatter(d, w, b->win[w].nchunks, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[2], s));
     HIP_TRY(launch_factor(d, nullptr, s));
+    for (int w : b->mapped) HIP_TRY(launch_large_factor(d, w, 6 * b->win[w].ncf, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[3], s));
     HIP_TRY(launch_backsub(d, false, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[4], s));
@@ -349,7 +383,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   b->n = n;
   b->win.resize(n);
   b->status_pre.assign(n, SBA_OK);
-  int64_t sred_off = 0, a_off = 0, dc_off = 0;
+  int64_t sred_off = 0, a_off = 0, dc_off = 0, big_off = 0, ue_off = 0;
   int max_nc = 1, max_ncam = 1, max_ncf = 1, max_lds_packed = 0, max_rec = 0;
   for (int w = 0; w < n; ++w) {
     const sba_problem_view& p = problems[w];
@@ -360,8 +394,8 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
     WindowDesc& W = b->win[w];
     if (W.np == 0 || W.nc == 0 || W.no == 0) b->status_pre[w] = SBA_E_EMPTY;
-    W.nt = (6 * W.ncf + 15) / 16;
-    W.rec = 256 * (W.nt * (W.nt + 1) / 2) + 33 * W.ncf;
+    W.nt = W.mapped ? 0 : (6 * W.ncf + 15) / 16;
+    W.rec = W.mapped ? 0 : 256 * (W.nt * (W.nt + 1) / 2) + 33 * W.ncf;
     W.sred_off = sred_off;
     sred_off += W.rec;
     max_rec = std::max(max_rec, W.rec);
@@ -370,8 +404,17 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     const int nn = 6 * W.ncf;
     const int64_t packed = static_cast<int64_t>(nn) * (nn + 1) / 2;
     const int64_t fixed_doubles = 512 + 256 * kPanel + 48 + 256;
-    W.factor_in_lds = (fixed_doubles + packed) * 8 <= 160 * 1024 ? 1 : 0;
-    if (W.factor_in_lds) {
+    W.factor_in_lds = !W.mapped && (fixed_doubles + packed) * 8 <= 160 * 1024 ? 1 : 0;
+    if (W.mapped) {
+      // dense FP32 sum of Z Z^T and U, e, diag U; FP64 A (n x n), rhs, diag U
+      W.big_off = big_off;
+      big_off += static_cast<int64_t>(nn) * nn;
+      W.ue_off = ue_off;
+      ue_off += 33 * static_cast<int64_t>(W.ncf);
+      W.a_off = a_off;
+      a_off += static_cast<int64_t>(nn) * nn + 2 * nn;
+      b->mapped.push_back(w);
+    } else if (W.factor_in_lds) {
       max_lds_packed = std::max<int>(max_lds_packed, static_cast<int>(packed));
       W.a_off = 0;
     } else {
@@ -393,27 +436,64 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each; each owns one record
   int64_t target = h->opt.chunk_pairs;
   if (target <= 0) target = default_chunk_pairs(b->total_pairs);
+  // In a mapped window a chunk also stages at most kMaxFreePoses free key frames: its map lists them in
+  // increasing order and its pairs carry their index in that list.
   std::vector<Chunk> chunks;
+  std::vector<int32_t> chunk_map;
   int64_t part_off = 0;
   {
     size_t t = 0;
     int last_w = -1;
+    std::vector<int> slot;
     while (t < H.tiles.size()) {
       const int w = H.tiles[t].window;
       WindowDesc& W = b->win[w];
       if (w != last_w) {
         W.part_off = part_off;
         W.nchunks = 0;
+        W.chunk_begin = static_cast<int32_t>(chunks.size());
         last_w = w;
+        if (W.mapped) slot.assign(W.ncf, -1);
       }
       const size_t begin = t;
       int64_t pairs = 0;
+      std::vector<int> kfs;  // free key frames of the chunk (mapped windows)
       while (t < H.tiles.size() && H.tiles[t].window == w && (t == begin || pairs < target)) {
-        pairs += H.tiles[t].pair_end - H.tiles[t].pair_begin;
+        const Tile& T = H.tiles[t];
+        if (W.mapped) {
+          std::vector<int> add;
+          for (int q = T.pair_begin; q < T.pair_end; ++q) {
+            const int j = H.pair_pose[q];
+            if (j < W.ncf && slot[j] < 0 && std::find(add.begin(), add.end(), j) == add.end()) add.push_back(j);
+          }
+          if (t > begin && kfs.size() + add.size() > static_cast<size_t>(kMaxFreePoses)) break;
+          for (int j : add) {
+            slot[j] = 0;
+            kfs.push_back(j);
+          }
+        }
+        pairs += T.pair_end - T.pair_begin;
         ++t;
       }
-      chunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), part_off});
-      part_off += W.rec;
+      Chunk c{static_cast<int32_t>(begin), static_cast<int32_t>(t), part_off, W.ncf, W.nt, 0, w};
+      int64_t rec = W.rec;
+      if (W.mapped) {
+        std::sort(kfs.begin(), kfs.end());
+        for (size_t k = 0; k < kfs.size(); ++k) slot[kfs[k]] = static_cast<int>(k);
+        for (size_t tt = begin; tt < t; ++tt)
+          for (int q = H.tiles[tt].pair_begin; q < H.tiles[tt].pair_end; ++q) {
+            const int j = H.pair_pose[q];
+            if (j < W.ncf) H.pair_rec[q].w = (H.pair_rec[q].w & 255) | (slot[j] << 8);
+          }
+        for (int j : kfs) slot[j] = -1;
+        c.nloc = static_cast<int32_t>(kfs.size());
+        c.nt = (6 * c.nloc + 15) / 16;
+        c.map_off = static_cast<int32_t>(chunk_map.size());
+        chunk_map.insert(chunk_map.end(), kfs.begin(), kfs.end());
+        rec = 256 * (c.nt * (c.nt + 1) / 2) + 33 * c.nloc;
+      }
+      chunks.push_back(c);
+      part_off += rec;
       W.nchunks += 1;
     }
   }
@@ -429,7 +509,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
         pairs += H.btiles[t].pair_end - H.btiles[t].pair_begin;
         ++t;
       }
-      bchunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), 0});
+      bchunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), 0, 0, 0, 0, w});
     }
   }
 
@@ -452,6 +532,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(b->win, win);
   UP(H.tiles, tiles);
   UP(chunks, chunks);
+  UP(chunk_map, chunk_map);
   UP(H.btiles, btiles);
   UP(bchunks, bchunks);
   UP(H.obs_xy, obs_xy);
@@ -491,6 +572,8 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   AL(static_cast<size_t>(part_off), part);
   AL(static_cast<size_t>(sred_off), sred);
   AL(static_cast<size_t>(a_off), A);
+  AL(static_cast<size_t>(big_off), big);
+  AL(static_cast<size_t>(ue_off), big_ue);
   AL(static_cast<size_t>(dc_off), dc);
   AL(static_cast<size_t>(dc_off), dc64);
   AL(H.pts.size(), out_pts);
@@ -642,6 +725,7 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc, double* dp,
                          double* scalars) {
   if (!h || !b) return SBA_E_INVALID;
+  if (b->win[0].mapped) return fail(h, SBA_E_CAPACITY, "the phase-level step covers windows of at most 40 free key frames");
   HIP_TRY(hipSetDevice(h->opt.device));
   b->adj_ready = false;
   Batch d = b->dev;
@@ -865,6 +949,8 @@ int adj_workspace(sba_handle* h, sba_batch* b, AdjWork& A) {
 // Factors the Gauss-Newton system of the solution of window 0 (lambda = 0) once and copies its state to FP64.
 // The batch's LM state is left as it was.
 int adj_prepare(sba_handle* h, sba_batch* b, AdjWork& A) {
+  if (b->win[0].mapped)
+    return fail(h, SBA_E_CAPACITY, "implicit differentiation supports windows of at most 40 free key frames");
   int rc = adj_workspace(h, b, A);
   if (rc != SBA_OK || b->adj_ready) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));

@@ -306,3 +306,224 @@ extern "C" int sba_scene_ground_truth(const sba_scene* s, const float** points, 
 }
 
 extern "C" void sba_scene_free(sba_scene* s) { delete s; }
+
+// ---------------------------------------------------------------------------------------------------------
+// The large multi-camera map: a stereo rig on a closed loop inside a corridor.
+// ---------------------------------------------------------------------------------------------------------
+
+extern "C" void sba_loop_default_config(sba_loop_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->num_keyframes = 200;
+  c->num_points = 200000;
+  c->num_stereo_pairs = 4;
+  c->num_fixed_keyframes = 1;
+  c->loop_length_m = 20.0;
+  c->stereo_baseline_m = 0.12;
+  c->corridor_half_m = 1.5;
+  c->floor_y_m = 1.2;     // y points down in the rig and camera frames
+  c->ceiling_y_m = -1.5;
+  c->clutter_fraction = 0.3;
+  c->max_range_m = 1.6;
+  c->fx = c->fy = 450.0;
+  c->cx = 376.0;
+  c->cy = 240.0;
+  c->width = 752;
+  c->height = 480;
+  c->keyframe_rotation_deg = 1.0;
+  c->pixel_noise = 0.5;
+  c->outlier_rate = 0.03;
+  c->pose_sigma_m = 0.02;
+  c->pose_sigma_deg = 0.5;
+  c->point_sigma_m = 0.05;
+  c->huber_px = 1.0;
+  c->seed = 5;
+}
+
+extern "C" int sba_scene_generate_loop(const sba_loop_config* cfg, sba_scene** out) {
+  if (!cfg || !out || cfg->num_keyframes < 2 || cfg->num_points < 0 || cfg->num_stereo_pairs < 1 ||
+      2 * cfg->num_stereo_pairs > 32 || cfg->num_fixed_keyframes < 1 || cfg->max_range_m <= 0.1)
+    return SBA_E_INVALID;
+  sba_scene* sc = new (std::nothrow) sba_scene;
+  if (!sc) return SBA_E_INVALID;
+  Rng rng(cfg->seed);
+  const double deg = M_PI / 180.0;
+  const int nk = cfg->num_keyframes;
+  const double r = cfg->loop_length_m / (2.0 * M_PI);
+  // key frames: rig at angle a on the loop, z (forward) along the tangent, y down, x = y cross z
+  std::vector<Pose> gt(nk);
+  for (int k = 0; k < nk; ++k) {
+    const double a = 2.0 * M_PI * k / nk;
+    const double cpos[3] = {r * std::cos(a), 0.0, r * std::sin(a)};
+    const double zf[3] = {-std::sin(a), 0.0, std::cos(a)};
+    const double yd[3] = {0.0, 1.0, 0.0};
+    const double xr[3] = {yd[1] * zf[2] - yd[2] * zf[1], yd[2] * zf[0] - yd[0] * zf[2], yd[0] * zf[1] - yd[1] * zf[0]};
+    double Rwr0[3][3];  // columns: rig axes in the world
+    for (int i = 0; i < 3; ++i) {
+      Rwr0[i][0] = xr[i];
+      Rwr0[i][1] = yd[i];
+      Rwr0[i][2] = zf[i];
+    }
+    double w[3] = {rng.normal() * cfg->keyframe_rotation_deg * deg, rng.normal() * cfg->keyframe_rotation_deg * deg,
+                   rng.normal() * cfg->keyframe_rotation_deg * deg};
+    double J[3][3], Rwr[3][3];
+    rot_exp(w, J);
+    matmul3(Rwr0, J, Rwr);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gt[k].R[i][j] = Rwr[j][i];
+    for (int i = 0; i < 3; ++i)
+      gt[k].t[i] = -(gt[k].R[i][0] * cpos[0] + gt[k].R[i][1] * cpos[1] + gt[k].R[i][2] * cpos[2]);
+  }
+  // rig: stereo pair q looks along yaw 2 pi q / n (about the rig's y axis); its cameras sit +-baseline/2 along
+  // the camera's own x axis
+  const int ncam = 2 * cfg->num_stereo_pairs;
+  std::vector<Pose> cam(ncam);
+  for (int c = 0; c < ncam; ++c) {
+    const double yaw = 2.0 * M_PI * (c / 2) / cfg->num_stereo_pairs;
+    const double side = (c % 2 ? 0.5 : -0.5) * cfg->stereo_baseline_m;
+    // rotation rig -> camera: R_cr = Ry(yaw)^T
+    const double cy_ = std::cos(yaw), sy = std::sin(yaw);
+    const double Rrc[3][3] = {{cy_, 0, sy}, {0, 1, 0}, {-sy, 0, cy_}};  // camera axes in the rig frame
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) cam[c].R[i][j] = Rrc[j][i];
+    const double center[3] = {Rrc[0][0] * side, Rrc[1][0] * side, Rrc[2][0] * side};
+    for (int i = 0; i < 3; ++i)
+      cam[c].t[i] = -(cam[c].R[i][0] * center[0] + cam[c].R[i][1] * center[1] + cam[c].R[i][2] * center[2]);
+  }
+  const int nf = std::max(1, std::min(cfg->num_fixed_keyframes, nk));
+  auto stored = [nk, nf](int k) { return k < nf ? nk - nf + k : k - nf; };
+  auto sees = [&](int k, int c, const double p[3], double& u, double& v) -> bool {
+    double q[3], s[3];
+    apply(gt[k], p, q);
+    apply(cam[c], q, s);
+    if (!(s[2] > 0.1)) return false;
+    if (s[0] * s[0] + s[1] * s[1] + s[2] * s[2] > cfg->max_range_m * cfg->max_range_m) return false;
+    u = cfg->fx * s[0] / s[2] + cfg->cx;
+    v = cfg->fy * s[1] / s[2] + cfg->cy;
+    return u >= 0.0 && u < cfg->width && v >= 0.0 && v < cfg->height;
+  };
+  const double band = cfg->corridor_half_m;
+  auto sample_point = [&](double p[3], double& theta) {
+    theta = rng.uniform(0.0, 2.0 * M_PI);
+    double rad, y;
+    if (rng.uniform() >= cfg->clutter_fraction) {
+      const int face = rng.uniform_int(0, 3);
+      rad = face == 0 ? r - band : face == 1 ? r + band : rng.uniform(r - band, r + band);
+      y = face == 2 ? cfg->floor_y_m : face == 3 ? cfg->ceiling_y_m : rng.uniform(cfg->ceiling_y_m, cfg->floor_y_m);
+    } else {
+      double off = rng.uniform(0.4, band);  // clutter keeps 0.4 m off the path
+      rad = rng.uniform() < 0.5 ? r - off : r + off;
+      y = rng.uniform(cfg->ceiling_y_m, cfg->floor_y_m);
+    }
+    p[0] = rad * std::cos(theta);
+    p[1] = y;
+    p[2] = rad * std::sin(theta);
+  };
+  // points with at least two key frames that see them, in order of angle
+  const int np = cfg->num_points;
+  struct Pt {
+    double theta, p[3];
+    std::vector<std::pair<int, int>> views;  // (generation key frame, camera)
+  };
+  std::vector<Pt> pts(np);
+  std::vector<int> kfs;
+  for (int i = 0; i < np; ++i) {
+    Pt& q = pts[i];
+    for (int attempt = 0;; ++attempt) {
+      sample_point(q.p, q.theta);
+      q.views.clear();
+      kfs.clear();
+      // key frames near the point's angle only: the range bounds visibility to a short arc
+      const int k0 = static_cast<int>(std::floor(q.theta / (2.0 * M_PI) * nk));
+      const int span = static_cast<int>(std::ceil((cfg->max_range_m + band) / cfg->loop_length_m * nk)) + 2;
+      for (int dk = -span; dk <= span; ++dk) {
+        const int k = ((k0 + dk) % nk + nk) % nk;
+        bool any = false;
+        for (int c = 0; c < ncam; ++c) {
+          double u, v;
+          if (sees(k, c, q.p, u, v)) {
+            q.views.emplace_back(k, c);
+            any = true;
+          }
+        }
+        if (any) kfs.push_back(k);
+      }
+      if (kfs.size() >= 2 || attempt >= 256) break;
+    }
+    if (kfs.size() < 2) {
+      q.views.clear();
+      continue;
+    }
+    // keep U{2..n} of the n key frames
+    const int m = rng.uniform_int(2, static_cast<int>(kfs.size()));
+    for (int a = 0; a < m; ++a) {
+      const int b = a + static_cast<int>(rng.uniform() * (kfs.size() - a));
+      std::swap(kfs[a], kfs[b]);
+    }
+    kfs.resize(m);
+    std::vector<std::pair<int, int>> keep;
+    for (auto& vw : q.views)
+      if (std::find(kfs.begin(), kfs.end(), vw.first) != kfs.end()) keep.push_back(vw);
+    q.views.swap(keep);
+  }
+  std::sort(pts.begin(), pts.end(), [](const Pt& a, const Pt& b) { return a.theta < b.theta; });
+  // observations ordered by stored key frame, then point id, then camera
+  std::vector<std::vector<std::pair<int, int>>> kf_obs(nk);  // stored key frame -> (point, camera)
+  for (int i = 0; i < np; ++i)
+    for (auto& vw : pts[i].views) kf_obs[stored(vw.first)].emplace_back(i, vw.second);
+  std::vector<int> gen_of_stored(nk);
+  for (int k = 0; k < nk; ++k) gen_of_stored[stored(k)] = k;
+  const float info_x = static_cast<float>(cfg->fx * cfg->fx / (cfg->huber_px * cfg->huber_px));
+  const float info_y = static_cast<float>(cfg->fy * cfg->fy / (cfg->huber_px * cfg->huber_px));
+  for (int s = 0; s < nk; ++s) {
+    std::sort(kf_obs[s].begin(), kf_obs[s].end());
+    for (auto& pc : kf_obs[s]) {
+      const int i = pc.first, c = pc.second;
+      double u = 0, v = 0;
+      sees(gen_of_stored[s], c, pts[i].p, u, v);
+      if (rng.uniform() < cfg->outlier_rate) {
+        u = rng.uniform(0.0, cfg->width);
+        v = rng.uniform(0.0, cfg->height);
+      } else {
+        u += rng.normal() * cfg->pixel_noise;
+        v += rng.normal() * cfg->pixel_noise;
+      }
+      sc->obs_xy.push_back(static_cast<float>((u - cfg->cx) / cfg->fx));
+      sc->obs_xy.push_back(static_cast<float>((v - cfg->cy) / cfg->fy));
+      const float info[4] = {info_x, 0.f, 0.f, info_y};
+      sc->obs_info.insert(sc->obs_info.end(), info, info + 4);
+      sc->point_ids.push_back(i);
+      sc->pose_ids.push_back(s);
+      sc->camera_ids.push_back(static_cast<int8_t>(c));
+    }
+  }
+  sc->gt_points.resize(3 * static_cast<size_t>(np));
+  sc->points.resize(3 * static_cast<size_t>(np));
+  for (int i = 0; i < np; ++i)
+    for (int d = 0; d < 3; ++d) {
+      sc->gt_points[3 * i + d] = static_cast<float>(pts[i].p[d]);
+      sc->points[3 * i + d] = static_cast<float>(pts[i].p[d] + rng.normal() * cfg->point_sigma_m);
+    }
+  sc->gt_poses.resize(16 * static_cast<size_t>(nk));
+  sc->poses.resize(16 * static_cast<size_t>(nk));
+  for (int s = 0; s < nk; ++s) {
+    const Pose& T = gt[gen_of_stored[s]];
+    store_pose(T, &sc->gt_poses[16 * s]);
+    Pose Tp = T;
+    if (s < nk - nf) {
+      double w[3] = {rng.normal() * cfg->pose_sigma_deg * deg, rng.normal() * cfg->pose_sigma_deg * deg,
+                     rng.normal() * cfg->pose_sigma_deg * deg};
+      double dR[3][3];
+      rot_exp(w, dR);
+      const double dt[3] = {rng.normal() * cfg->pose_sigma_m, rng.normal() * cfg->pose_sigma_m,
+                            rng.normal() * cfg->pose_sigma_m};
+      matmul3(dR, T.R, Tp.R);
+      for (int i = 0; i < 3; ++i) Tp.t[i] = dR[i][0] * T.t[0] + dR[i][1] * T.t[1] + dR[i][2] * T.t[2] + dt[i];
+    }
+    store_pose(Tp, &sc->poses[16 * s]);
+  }
+  sc->cams.resize(16 * static_cast<size_t>(ncam));
+  for (int c = 0; c < ncam; ++c) store_pose(cam[c], &sc->cams[16 * c]);
+  sc->num_fixed_kf = nf;
+  *out = sc;
+  return SBA_OK;
+}

@@ -42,6 +42,26 @@ def generate(preset: str | None = None, seed: int = 1, max_iterations: int = 10,
     rc = L.sba_scene_generate(C.byref(cfg), C.byref(sc))
     if rc != 0:
         raise RuntimeError(f"sba_scene_generate failed ({rc})")
+    return _take(L, sc, max_iterations)
+
+
+def loop(seed: int = 5, max_iterations: int = 10, **overrides):
+    """The large multi-camera map (defaults: 4 stereo pairs, 200 key frames on a 20 m loop, 200,000 points in a
+    corridor, 3% outliers, one fixed key frame). Returns (problem, gt_points, gt_rig_from_world)."""
+    cfg = _lib.LoopConfig()
+    L = _lib.lib()
+    L.sba_loop_default_config(C.byref(cfg))
+    cfg.seed = seed
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    sc = C.c_void_p()
+    rc = L.sba_scene_generate_loop(C.byref(cfg), C.byref(sc))
+    if rc != 0:
+        raise RuntimeError(f"sba_scene_generate_loop failed ({rc})")
+    return _take(L, sc, max_iterations)
+
+
+def _take(L, sc, max_iterations):
     try:
         v = _lib.ProblemView()
         L.sba_scene_view(sc, C.byref(v))

@@ -208,9 +208,26 @@ typedef struct sba_scene_config {
   uint64_t seed;
 } sba_scene_config;
 
+/* The large multi-camera map: a rig of stereo pairs (yaw offsets 360/num_stereo_pairs apart, baseline
+ * stereo_baseline_m) on num_keyframes key frames spaced evenly on a closed loop of loop_length_m, inside a corridor
+ * of half-width corridor_half_m around the loop (inner and outer walls, floor, ceiling) with a fraction of clutter
+ * points in the corridor volume. A camera sees a point that projects in-image within max_range_m; each point keeps
+ * a random subset U{2..n} of the n key frames that see it (all of their cameras that see it). Points are stored in
+ * order of their angle around the loop. */
+typedef struct sba_loop_config {
+  int32_t num_keyframes, num_points, num_stereo_pairs, num_fixed_keyframes;
+  double loop_length_m, stereo_baseline_m, corridor_half_m, floor_y_m, ceiling_y_m, clutter_fraction, max_range_m;
+  double fx, fy, cx, cy;
+  int32_t width, height;
+  double keyframe_rotation_deg, pixel_noise, outlier_rate, pose_sigma_m, pose_sigma_deg, point_sigma_m, huber_px;
+  uint64_t seed;
+} sba_loop_config;
+
 typedef struct sba_scene sba_scene;
 
 void sba_scene_default_config(sba_scene_config* cfg);
+void sba_loop_default_config(sba_loop_config* cfg);
+int sba_scene_generate_loop(const sba_loop_config* cfg, sba_scene** out);
 int sba_scene_generate(const sba_scene_config* cfg, sba_scene** out);
 /* Fills a problem view that points into the scene's own storage (valid until sba_scene_free). */
 int sba_scene_view(const sba_scene* s, sba_problem_view* view);

@@ -174,3 +174,31 @@ def test_autotuner_and_chunking_leave_results_unchanged():
         assert all(sv.solve_batch(qs))
         costs.append(np.array([q.last_cost_f64 for q in qs]))
     np.testing.assert_allclose(costs[0], costs[1], rtol=1e-5)
+
+
+def test_large_multicamera_window_matches_oracle():
+    # 59 free key frames (reduced system of order 354, above the 40 key frames one chunk stages): the mapped
+    # path (per-chunk key frame maps, dense FP32 scatter, blocked FP64 Cholesky over several workgroups)
+    p, _, _ = ca.scene.loop(num_keyframes=60, num_points=20000, loop_length_m=12.0, max_iterations=10)
+    assert p.rig_from_world.shape[0] - p.num_fixed_key_frames > 40 and p.camera_from_rig.shape[0] == 8
+    q = p.copy()
+    s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    assert s.solve(q)
+    r = O.solve(p, "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
+    assert q.iterations == r["iterations"] == 10
+    assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < 1e-6
+    nf = p.rig_from_world.shape[0] - 1
+    assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
+
+
+def test_large_window_in_a_batch_with_small_windows():
+    big, _, _ = ca.scene.loop(num_keyframes=48, num_points=6000, loop_length_m=9.6, max_iterations=6)
+    small = ca.scene.batched_windows(3, seed=11, max_iterations=6)
+    probs = [small[0], big, small[1], small[2]]
+    s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    together = [p.copy() for p in probs]
+    assert all(s.solve_batch(together))
+    for p, t in zip(probs, together):
+        alone = p.copy()
+        assert s.solve(alone)
+        assert abs(alone.last_cost_f64 - t.last_cost_f64) <= 1e-6 * alone.last_cost_f64
