@@ -8,9 +8,12 @@
 //                 the additions, and so the last bits of the sum, is not fixed for these windows)
 //   k_lassemble   A = U + lambda diag(U) - sum Z Z^T in FP64 (dense, row-major lower triangle), rhs e
 //   k_potrf / k_trsm / k_syrk   right-looking blocked Cholesky (Golub & Van Loan, Matrix Computations,
-//                 Algorithm 4.2.10) with 64 x 64 blocks: factor of the diagonal block in one workgroup, the
-//                 panel below it one block per workgroup, the trailing update one block pair per workgroup
-//   k_ltrsv       L y = e, L^T x = y in one workgroup, 64-row blocks
+//                 Algorithm 4.2.10) with 64 x 64 blocks: the diagonal block is factored in one wavefront with
+//                 its rows in registers, and its triangular inverse kept; the panel below it is then a product
+//                 with that inverse (one block per workgroup), the trailing update one block pair per workgroup.
+//                 The forward substitution L y = e rides along: y_k = L_kk^-1 e_k in k_potrf, e_i -= L_ik y_k
+//                 in k_trsm
+//   k_bsolve      L^T x = y, one launch per 64-row block with the kept inverses
 //   k_lepilogue   the pose step, its max-norm and dc^T diag(U) dc, trial key frames
 // A non-positive pivot clears the window's chol_ok; the later kernels then return, and the LM decision raises
 // lambda (schur_complement_bundler_gpu.cpp:175-185), as for the small windows.
@@ -28,19 +31,19 @@ namespace {
 constexpr int kLB = 64;        // block of the large factorization
 constexpr int kLds = kLB + 1;  // padded LDS row
 
-__device__ __forceinline__ int nblocks(int n) { return (n + kLB - 1) / kLB; }
-
 struct Large {
-  double* A;    // n x n, row-major; the lower triangle holds A, then L
-  double* rhs;  // e, then y, then x
-  double* du;   // diag(U)
+  double* A;     // n x n, row-major; the lower triangle holds A, then L
+  double* rhs;   // e, then y, then x
+  double* du;    // diag(U)
+  double* Linv;  // per 64-row block: the inverse of its diagonal block of L (64 x 64, row-major, lower)
   int n;
 };
 
 __device__ __forceinline__ Large large_of(const Batch& b, const WindowDesc& W) {
   const int n = 6 * W.ncf;
   double* A = b.A + W.a_off;
-  return Large{A, A + static_cast<int64_t>(n) * n, A + static_cast<int64_t>(n) * n + n, n};
+  const int64_t nn = static_cast<int64_t>(n) * n;
+  return Large{A, A + nn, A + nn + n, A + nn + 2 * n, n};
 }
 
 __device__ __forceinline__ double readlane_d(double x, int lane) {
@@ -138,17 +141,22 @@ __global__ __launch_bounds__(256) void k_lassemble(Batch b, int w) {
   if (blockIdx.x == 0 && threadIdx.x == 0) St->chol_ok = 1;
 }
 
-// Cholesky of diagonal block k (one workgroup): column by column in LDS.
+// Cholesky of diagonal block k in LDS by one workgroup of 256 threads, in panels of 4 columns: wavefront 0
+// factors the panel with its rows in registers (pivots and column entries broadcast with readlane) and writes the
+// 4 scaled columns to sCol; then every thread applies the rank-4 update to its cells of the trailing triangle
+// (loads before stores). Then the inverse X = L_kk^-1 (lane j of wavefront 0 solves column j of L X = I), and
+// the forward-substitution step y_k = X e_k.
 __global__ __launch_bounds__(256) void k_potrf(Batch b, int w, int k) {
-  __shared__ double s[kLB * kLds];
-  __shared__ double sinv;
-  __shared__ int fail;
+  constexpr int kP = 4;
+  __shared__ double sL[kLB * kLds];
+  __shared__ double sCol[kP * kLB];
+  __shared__ int sOk;
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
-  const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x;
-  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {
+  const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x, lane = tid & 63;
+  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {  // coalesced
     const int r = idx / kLB, c = idx % kLB;
     double v = 0.0;
     if (r < kb && c < kb) {
@@ -156,60 +164,141 @@ __global__ __launch_bounds__(256) void k_potrf(Batch b, int w, int k) {
     } else if (r == c) {
       v = 1.0;  // identity padding beyond the matrix
     }
-    s[r * kLds + c] = v;
+    sL[r * kLds + c] = v;
   }
-  if (tid == 0) fail = 0;
-  for (int c = 0; c < kLB; ++c) {
-    __syncthreads();
-    if (tid == 0) {
-      const double d = s[c * kLds + c];
-      if (!(d > 0.0)) fail = 1;
-      const double l = sqrt(fmax(d, 1e-300));
-      s[c * kLds + c] = l;
-      sinv = 1.0 / l;
+  if (tid == 0) sOk = 1;
+  __syncthreads();
+  for (int c0 = 0; c0 < kLB; c0 += kP) {
+    if (tid < kLB) {
+      double a[kP];
+#pragma unroll
+      for (int q = 0; q < kP; ++q) a[q] = sL[lane * kLds + c0 + q];
+      bool ok = true;
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        const int c = c0 + q;
+        const double d = readlane_d(a[q], c);
+        if (!(d > 0.0)) ok = false;
+        const double l = sqrt(fmax(d, 1e-300));
+        if (lane == c) a[q] = l;
+        else if (lane > c) a[q] = a[q] / l;
+#pragma unroll
+        for (int q2 = q + 1; q2 < kP; ++q2) {
+          const double l2 = readlane_d(a[q], c0 + q2);  // L[c0 + q2][c]
+          if (lane >= c0 + q2) a[q2] -= a[q] * l2;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        if (lane >= c0 + q) sL[lane * kLds + c0 + q] = a[q];
+        sCol[q * kLB + lane] = lane >= c0 + kP ? a[q] : 0.0;
+      }
+      if (!ok && lane == 0) sOk = 0;
     }
     __syncthreads();
-    if (tid > c && tid < kLB) s[tid * kLds + c] *= sinv;
+    double upd[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int cell = tid + 256 * t, r = cell >> 6, j = cell & 63;
+      double v = 0.0;
+      if (j >= c0 + kP && j <= r) {
+        v = sL[r * kLds + j];
+#pragma unroll
+        for (int q = 0; q < kP; ++q) v -= sCol[q * kLB + r] * sCol[q * kLB + j];
+      }
+      upd[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int cell = tid + 256 * t, r = cell >> 6, j = cell & 63;
+      if (j >= c0 + kP && j <= r) sL[r * kLds + j] = upd[t];
+    }
     __syncthreads();
-    for (int r = c + 1 + (tid >> 6); r < kLB; r += 4)
-      for (int j = c + 1 + (tid & 63); j <= r; j += 64) s[r * kLds + j] -= s[r * kLds + c] * s[j * kLds + c];
   }
-  __syncthreads();
+  const bool ok = sOk != 0;
   for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
     const int r = idx / kb, c = idx % kb;
-    if (c <= r) L.A[static_cast<int64_t>(k0 + r) * n + k0 + c] = s[r * kLds + c];
+    if (c <= r) L.A[static_cast<int64_t>(k0 + r) * n + k0 + c] = sL[r * kLds + c];
   }
-  if (tid == 0 && fail) St->chol_ok = 0;
+  __syncthreads();  // sL is overwritten with X below
+  if (tid >= kLB) {
+    if (tid == kLB && !ok) St->chol_ok = 0;
+    return;
+  }
+  // X = L^-1, column `lane`: X[i][j] = (delta_ij - sum_{m=j}^{i-1} L[i][m] X[m][j]) / L[i][i]
+  double x[kLB];
+#pragma unroll
+  for (int i = 0; i < kLB; ++i) {
+    double v = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int m = 0; m < i; ++m) v -= sL[i * kLds + m] * x[m];
+    x[i] = (i >= lane) ? v / sL[i * kLds + i] : 0.0;
+  }
+  // keep X (global, for the panel and the backward solve) and form y_k = X e_k through LDS
+  __shared__ double sE[kLB];
+  double* X = L.Linv + static_cast<int64_t>(k) * kLB * kLB;
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // wavefront 0 alone from here: its LDS reads are done
+#pragma unroll
+  for (int i = 0; i < kLB; ++i) {
+    X[i * kLB + lane] = x[i];
+    sL[i * kLds + lane] = x[i];
+  }
+  sE[lane] = lane < kb ? L.rhs[k0 + lane] : 0.0;
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  double y = 0.0;
+  for (int m = 0; m <= lane; ++m) y += sL[lane * kLds + m] * sE[m];
+  if (lane < kb) L.rhs[k0 + lane] = y;
+  if (lane == 0 && !ok) St->chol_ok = 0;
 }
 
-// Panel below diagonal block k: L_ik = A_ik L_kk^-T, one block row per workgroup.
+// Panel below diagonal block k: L_ik = A_ik L_kk^-T = A_ik X^T with the kept inverse X, one block row per
+// workgroup (each thread a 4 x 4 piece); then the block row's forward-substitution update e_i -= L_ik y_k.
 __global__ __launch_bounds__(256) void k_trsm(Batch b, int w, int k) {
-  __shared__ double sL[kLB * kLds];
-  __shared__ double sB[kLB * kLds];
+  __shared__ double sA[kLB * kLds];
+  __shared__ double sX[kLB * kLds];
+  __shared__ double sy[kLB];
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
   const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x;
   const int i0 = kLB * (k + 1 + blockIdx.x), ib = min(kLB, n - i0);
+  const double* X = L.Linv + static_cast<int64_t>(k) * kLB * kLB;
   for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {
     const int r = idx / kLB, c = idx % kLB;
-    sL[r * kLds + c] = (r < kb && c <= r) ? L.A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
-    sB[r * kLds + c] = (r < ib && c < kb) ? L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] : 0.0;
+    sA[r * kLds + c] = (r < ib && c < kb) ? L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] : 0.0;
+    sX[r * kLds + c] = X[idx];
   }
+  if (tid < kLB) sy[tid] = tid < kb ? L.rhs[k0 + tid] : 0.0;
+  __syncthreads();
+  const int r0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
+  double acc[4][4] = {};
+  for (int m = 0; m < kLB; ++m) {
+    double a[4], x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = sA[(r0 + q) * kLds + m];
+      x[q] = sX[(c0 + q) * kLds + m];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * x[q];
+  }
+  __syncthreads();  // sA is read by every thread before it is overwritten with L_ik
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + p, c = c0 + q;
+      sA[r * kLds + c] = (r < ib && c < kb) ? acc[p][q] : 0.0;
+      if (r < ib && c < kb) L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] = acc[p][q];
+    }
   __syncthreads();
   if (tid < ib) {
-    double* x = sB + tid * kLds;
-    for (int c = 0; c < kb; ++c) {
-      double a = x[c];
-      for (int m = 0; m < c; ++m) a -= x[m] * sL[c * kLds + m];
-      x[c] = a / sL[c * kLds + c];
-    }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < ib * kb; idx += blockDim.x) {
-    const int r = idx / kb, c = idx % kb;
-    L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] = sB[r * kLds + c];
+    double s = 0.0;
+    for (int m = 0; m < kb; ++m) s += sA[tid * kLds + m] * sy[m];
+    L.rhs[i0 + tid] -= s;
   }
 }
 
@@ -260,63 +349,35 @@ __global__ __launch_bounds__(256) void k_syrk(Batch b, int w, int k) {
     }
 }
 
-// L y = e, then L^T x = y, in one workgroup of 1024 threads: per 64-row block, the off-diagonal products are
-// summed by 16 thread slices, then wavefront 0 solves the diagonal block with register broadcasts.
-__global__ __launch_bounds__(1024) void k_ltrsv(Batch b, int w) {
-  extern __shared__ double sy[];  // n: y, then x
-  __shared__ double part[16 * kLB];
+// Backward substitution L^T x = y, one launch per 64-row block from the last (y from the factor kernels; rhs
+// holds w = y minus the products with the solved blocks). Launch for block bk, one workgroup per block j <= bk:
+// each computes x_bk = X_bk^T w_bk from the kept inverse (64 x 64, cheap to repeat); workgroup bk stores it and
+// every other workgroup j < bk subtracts L_bk,j^T x_bk from w_j. Block bk is final once its launch has run.
+__global__ __launch_bounds__(64) void k_bsolve(Batch b, int w, int bk) {
+  __shared__ double sw[kLB];
+  __shared__ double sx[kLB];
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
-  const int n = L.n, nb = nblocks(n), tid = threadIdx.x, lane = tid & 63, slice = tid >> 6;
-  for (int i = tid; i < n; i += blockDim.x) sy[i] = L.rhs[i];
+  const int n = L.n, lane = threadIdx.x, j = blockIdx.x;
+  const int b0 = kLB * bk, bb = min(kLB, n - b0);
+  sw[lane] = lane < bb ? L.rhs[b0 + lane] : 0.0;
   __syncthreads();
-  for (int blk = 0; blk < nb; ++blk) {
-    const int b0 = kLB * blk, bb = min(kLB, n - b0);
-    double s = 0.0;
-    if (lane < bb)
-      for (int c = slice; c < b0; c += 16) s += L.A[static_cast<int64_t>(b0 + lane) * n + c] * sy[c];
-    part[slice * kLB + lane] = s;
-    __syncthreads();
-    if (slice == 0) {
-      double v = 0.0;
-      if (lane < bb) {
-        v = sy[b0 + lane];
-        for (int q = 0; q < 16; ++q) v -= part[q * kLB + lane];
-      }
-      for (int c = 0; c < bb; ++c) {
-        const double yc = readlane_d(v, c) / L.A[static_cast<int64_t>(b0 + c) * n + b0 + c];
-        if (lane == c) v = yc;
-        else if (lane > c && lane < bb) v -= L.A[static_cast<int64_t>(b0 + lane) * n + b0 + c] * yc;
-      }
-      if (lane < bb) sy[b0 + lane] = v;
-    }
-    __syncthreads();
+  const double* X = L.Linv + static_cast<int64_t>(bk) * kLB * kLB;
+  double x = 0.0;
+  for (int m = lane; m < bb; ++m) x += X[m * kLB + lane] * sw[m];  // x_c = sum_{m >= c} X[m][c] w_m
+  if (j == bk) {
+    if (lane < bb) L.rhs[b0 + lane] = x;
+    return;
   }
-  for (int blk = nb - 1; blk >= 0; --blk) {
-    const int b0 = kLB * blk, bb = min(kLB, n - b0);
-    double s = 0.0;
-    if (lane < bb)
-      for (int r = b0 + kLB + slice; r < n; r += 16) s += L.A[static_cast<int64_t>(r) * n + b0 + lane] * sy[r];
-    part[slice * kLB + lane] = s;
-    __syncthreads();
-    if (slice == 0) {
-      double v = 0.0;
-      if (lane < bb) {
-        v = sy[b0 + lane];
-        for (int q = 0; q < 16; ++q) v -= part[q * kLB + lane];
-      }
-      for (int c = bb - 1; c >= 0; --c) {
-        const double xc = readlane_d(v, c) / L.A[static_cast<int64_t>(b0 + c) * n + b0 + c];
-        if (lane == c) v = xc;
-        else if (lane < c) v -= L.A[static_cast<int64_t>(b0 + c) * n + b0 + lane] * xc;
-      }
-      if (lane < bb) sy[b0 + lane] = v;
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += blockDim.x) L.rhs[i] = sy[i];
+  sx[lane] = lane < bb ? x : 0.0;
+  __syncthreads();
+  // w_j[c] -= sum_r L[b0 + r][j0 + c] x_r (rows of block bk, columns of block j < bk)
+  const int j0 = kLB * j;
+  double s = 0.0;
+  for (int r = 0; r < bb; ++r) s += L.A[static_cast<int64_t>(b0 + r) * n + j0 + lane] * sx[r];
+  L.rhs[j0 + lane] -= s;
 }
 
 // The pose step, max |dc|, dc^T diag(U) dc and the trial key frames of window w.
@@ -381,8 +442,11 @@ hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s) {
     }
     e = hipGetLastError();
   }
+  for (int bk = nb - 1; bk >= 0 && e == hipSuccess; --bk) {
+    hipLaunchKernelGGL(k_bsolve, dim3(bk + 1), dim3(64), 0, s, b, w, bk);
+    e = hipGetLastError();
+  }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ltrsv, dim3(1), dim3(1024), static_cast<size_t>(n) * sizeof(double), s, b, w);
   hipLaunchKernelGGL(k_lepilogue, dim3(1), dim3(256), 0, s, b, w);
   return hipGetLastError();
 }

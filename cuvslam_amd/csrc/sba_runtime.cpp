@@ -412,7 +412,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
       W.ue_off = ue_off;
       ue_off += 33 * static_cast<int64_t>(W.ncf);
       W.a_off = a_off;
-      a_off += static_cast<int64_t>(nn) * nn + 2 * nn;
+      a_off += static_cast<int64_t>(nn) * nn + 2 * nn + static_cast<int64_t>((nn + 63) / 64) * 64 * 64;
       b->mapped.push_back(w);
     } else if (W.factor_in_lds) {
       max_lds_packed = std::max<int>(max_lds_packed, static_cast<int>(packed));
