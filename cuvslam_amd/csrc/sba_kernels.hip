@@ -306,6 +306,15 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// Makes the LDS writes of some lanes of a wavefront visible to the later LDS reads of the other lanes: without
+// it the compiler may satisfy a lane's read before the stores that only other lanes execute.
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // A pair's record and first observation, loaded one tile ahead.
 struct PairPre {
   int4 rec;  // (first observation, observation end, key frame, point index in the tile)
@@ -668,9 +677,25 @@ __global__ __launch_bounds__(64) void k_reduce(Batch b) {
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// FP64 factor/solve of the reduced camera system, one workgroup per window.
+// FP64 factor/solve of the reduced camera system, one workgroup per window, the matrix held in registers.
+//
+// S + lambda diag(U) (n = 6 x free key frames <= 240) is cut into 16 x 16 tiles; A = U^T U is factored as the
+// upper tiles U_JI (J <= I), each tile in the registers of one wavefront in the FP64 matrix-core layout
+// (element (4v + lane / 16, lane % 16) in register v of the lane, i.e. 64 v + lane in row-major order).
+// Wavefront 0 holds the diagonal tiles, wavefronts 1..7 the others. Right-looking, per tile column K:
+//   panel     U_KI = X_KK A_KI with the inverse X_KK = U_KK^-T (4 MFMA 16x16x4), published to LDS, and the
+//             forward substitution e_I -= U_KI^T y_K;
+//   trailing  A_JI -= U_KJ^T U_KI for K < J <= I (4 MFMA each); wavefront 0 first updates (K+1, K+1) and
+//             factors it (one wavefront, pivots broadcast with readlane; X and y_{K+1} formed alongside), so
+//             the serial 16 x 16 factor overlaps the other wavefronts' updates. Two barriers per tile column.
+// Backward substitution U x = y per tile column: x_K = X_KK^T r_K, then r_J -= U_JK x_K by the tiles' owners.
+// (Golub & Van Loan, Matrix Computations, 4.2.4 block Cholesky and 3.1 triangular systems.)
 // ---------------------------------------------------------------------------------------------------------
 namespace {
+
+typedef double double4v __attribute__((ext_vector_type(4)));
+constexpr int kFT = 15;     // tile rows of the largest reduced system (240 = 15 x 16)
+constexpr int kSlots = 15;  // tiles per wavefront: wavefront 0 the <= 15 diagonal ones, 1..7 the <= 105 others
 
 __device__ __forceinline__ double readlane_d(double x, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
@@ -680,204 +705,329 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
 
 __device__ __forceinline__ int64_t pk(int i, int j) { return static_cast<int64_t>(i) * (i + 1) / 2 + j; }
 
+__device__ __forceinline__ double4v mfma64(double a, double b, double4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
+// c - P_J^T P_I for two 16 x 16 row-major tiles in LDS (A operand lane l: P_J[4q + l/16][l%16] = 64 q + l)
+__device__ __forceinline__ double4v tile_update(double4v c, const double* pj, const double* pi, int lane) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) c = mfma64(-pj[64 * q + lane], pi[64 * q + lane], c);
+  return c;
+}
+
+// A wavefront-uniform value the compiler may not hoist out of the enclosing loop: keeps the per-tile LDS
+// addresses from being precomputed for every tile slot at once (which would cost a register per slot).
+__device__ __forceinline__ int in_loop(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
+__device__ __forceinline__ double sum_rows4(double x) {  // over lanes l, l^16, l^32, l^48
+  x += __shfl_xor(x, 16);
+  x += __shfl_xor(x, 32);
+  return x;
+}
+
+__device__ __forceinline__ double sum_row16(double x) {  // over the 16 lanes of l's row group
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+// Cholesky of a symmetric 16 x 16 tile held by one wavefront in the FP64 matrix-core layout (cell
+// (4v + lane/16, lane%16) in t[v]), right-looking: per pivot c the current column c and row c of X go through
+// LDS (cb, xb: 16 doubles each) and every lane updates its four cells. On return t holds U = L^T on and above
+// the diagonal (cells below it are left unspecified), X^T = L^-T is written to xt (row-major) and 1 / L_ii to
+// rdiag. X = L^-1 is built by the same elementary column transforms, starting from the identity.
+// False on a non-positive pivot.
+__device__ bool factor_tile(double4v& t, double* cb, double* xb, double* xt, double* rdiag, int lane) {
+  const int li = lane >> 4, lj = lane & 15;
+  double4v x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) x[v] = 4 * v + li == lj ? 1.0 : 0.0;
+  bool ok = true;
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    if (lj == c) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cb[4 * v + li] = t[v];
+    }
+    if (li == (c & 3)) {
+      const int h = c >> 2;
+      xb[lj] = h == 0 ? x[0] : h == 1 ? x[1] : h == 2 ? x[2] : x[3];
+    }
+    wave_lds_sync();
+    const double d = cb[c];
+    if (!(d > 0.0)) ok = false;
+    // 1/sqrt(d) by the hardware estimate refined with one Newton-Raphson step
+    const double dd = fmax(d, 1e-300);
+    double il = __builtin_amdgcn_rsq(dd);
+    il = il * (1.5 - 0.5 * dd * il * il);
+    const double lc = cb[lj] * il;  // L[lj][c] = U[c][lj]
+    const double xc = xb[lj] * il;  // X[c][lj], final
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * v + li;
+      const double lr = cb[r] * il;  // L[r][c] for r > c
+      t[v] = (r > c && lj > c) ? t[v] - lr * lc : (r == c ? lc : t[v]);
+      x[v] = r > c ? x[v] - lr * xc : (r == c ? xc : x[v]);
+    }
+    if (lane == 0) rdiag[c] = il;
+    wave_lds_sync();  // every lane has read cb and xb before the next pivot overwrites them
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) xt[16 * lj + 4 * v + li] = x[v];
+  return ok;
+}
 
 }  // namespace
 
 __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_S, double* store_L) {
-  extern __shared__ double smd[];
+  static_assert(kFactorThreads == 512, "wavefront 0 plus 7 wavefronts of off-diagonal tiles");
+  __shared__ double vec[256];  // e, updated by the forward substitution
+  __shared__ double ys[256];   // y, then r of the backward substitution
+  __shared__ double xs[256];   // x
+  __shared__ double du[256];   // diag(U)
+  __shared__ double idg[256];  // 1 / L_ii
+  __shared__ double red[32];
+  __shared__ double cb[16], xb[16];   // pivot column and X row of the diagonal tile being factored
+  __shared__ double sXT[kFT * 256];   // X_KK^T of every tile column
+  __shared__ double sP[kFT * 256];    // panel tiles U_KI of the current tile column
+  __shared__ int flag;
   const int w = blockIdx.x;
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || W.mapped) return;  // mapped windows: sba_large.hip
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int warp = tid >> 6;
+  const int lane = tid & 63, li = lane >> 4, lj = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ncf = W.ncf;
   const int n = 6 * ncf;
-  const int nb = ncf * (ncf + 1) / 2;
+  const int T = W.nt;  // tile rows, 16 T >= n
   const float lam = St->lambda;
-  double* vec = smd;               // 256: right-hand side, then y, then x
-  double* du = smd + 256;          // 256: diag(U)
-  double* panel = smd + 512;       // 256 x kPanel
-  double* red = panel + 256 * kPanel;  // 32: reductions
-  int* flag = reinterpret_cast<int*>(red + 32);
-  double* idiag = red + 48;            // 256: 1 / L_ii
-  double* Lp = W.factor_in_lds ? (idiag + 256) : (b.A + W.a_off);
-  const int nt = W.nt;
-  const float* R = b.sred + W.sred_off;  // sum Z Z^T tiles, then U, e, diag U
-  const float* RU = R + 256 * (nt * (nt + 1) / 2);
+  const float* R = b.sred + W.sred_off;  // sum Z Z^T upper tiles (matrix-core lane order), then U, e, diag U
+  const float* RU = R + 256 * (T * (T + 1) / 2);
   const float* RE = RU + 21 * ncf;
   const float* RD = RE + 6 * ncf;
-  (void)nb;
 
-  // 1. assemble A = U - sum Z Z^T + lambda diag(U) (packed lower) and e
-  const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
-  for (int64_t idx = tid; idx < npk; idx += kFactorThreads) {
-    int i = static_cast<int>((sqrt(8.0 * static_cast<double>(idx) + 1.0) - 1.0) * 0.5);
-    while (pk(i + 1, 0) <= idx) ++i;
-    while (pk(i, 0) > idx) --i;
-    const int j = static_cast<int>(idx - pk(i, 0));
-    // S is symmetric: entry (j, i), j <= i, of upper tile (j / 16, i / 16), in matrix-core lane order
-    const int r = j & 15, c = i & 15;
-    double v = -static_cast<double>(R[256 * upper_tile(j >> 4, i >> 4, nt) + 64 * (r & 3) + 16 * (r >> 2) + c]);
-    if (i / 6 == j / 6) {
-      const int pose = i / 6, rr = j - 6 * pose, cc = i - 6 * pose;
-      v += static_cast<double>(RU[21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)]);
+  // tiles of this wavefront (wavefront-uniform)
+  int tJ[kSlots], tI[kSlots];
+  {
+    const int noff = T * (T - 1) / 2;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      if (wave == 0) {
+        tJ[s] = s < T ? s : -1;
+        tI[s] = s;
+      } else {
+        int m = 7 * s + wave - 1, J = 0;
+        if (m < noff) {
+          while (m >= T - 1 - J) {
+            m -= T - 1 - J;
+            ++J;
+          }
+          tJ[s] = J;
+          tI[s] = J + 1 + m;
+        } else {
+          tJ[s] = -1;
+          tI[s] = -1;
+        }
+      }
     }
-    if (i == j) v += static_cast<double>(lam) * static_cast<double>(RD[i]);
-    Lp[idx] = v;
   }
-  for (int i = tid; i < n; i += kFactorThreads) {
-    vec[i] = static_cast<double>(RE[i]);
-    du[i] = static_cast<double>(RD[i]);
+
+  // 1. assemble the tiles of A = U - sum Z Z^T + lambda diag(U); rows and columns past n are identity
+  double4v acc[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    acc[s] = double4v{0.0, 0.0, 0.0, 0.0};
+    if (tJ[s] >= 0) {
+      const int J = tJ[s], I = tI[s];
+      const float* tile = R + 256 * upper_tile(J, I, T);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rl = 4 * v + li, r = 16 * J + rl, c = 16 * I + lj;
+        double val = r == c ? 1.0 : 0.0;
+        if (r < n && c < n) {
+          // stored element (row, col) at 64 (row & 3) + 16 (row >> 2) + col; diagonal tiles read above the diagonal
+          const bool mir = J == I && rl > lj;
+          const int er = mir ? lj : rl, ec = mir ? rl : lj;
+          val = -static_cast<double>(tile[64 * (er & 3) + 16 * (er >> 2) + ec]);
+          const int lo = min(r, c), hi = max(r, c);
+          if (lo / 6 == hi / 6) {
+            const int pose = lo / 6, rr = lo - 6 * pose, cc = hi - 6 * pose;
+            val += static_cast<double>(RU[21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)]);
+          }
+          if (r == c) val += static_cast<double>(lam) * static_cast<double>(RD[r]);
+        }
+        acc[s][v] = val;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
-  if (tid == 0) flag[0] = 0;
-  __syncthreads();
+  for (int i = tid; i < 256; i += kFactorThreads) {
+    vec[i] = i < n ? static_cast<double>(RE[i]) : 0.0;
+    du[i] = i < n ? static_cast<double>(RD[i]) : 0.0;
+  }
+  if (tid == 0) flag = 0;
   if (dbg_S && w == 0) {
-    for (int64_t idx = tid; idx < static_cast<int64_t>(n) * n; idx += kFactorThreads) {
-      const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
-      dbg_S[idx] = i >= j ? Lp[pk(i, j)] : Lp[pk(j, i)];
-    }
-    for (int i = tid; i < n; i += kFactorThreads) dbg_S[static_cast<int64_t>(n) * n + i] = vec[i];
-    __syncthreads();
-  }
-
-  // 2. blocked right-looking Cholesky, forward substitution fused into the panel step
-  for (int k0 = 0; k0 < n; k0 += kPanel) {
-    const int kb = min(kPanel, n - k0);
-    if (warp == 0) {
-      double p[4][kPanel];
-      double rhs[4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int row = k0 + lane + 64 * s;
+    for (int s = 0; s < kSlots; ++s) {
+      if (tJ[s] >= 0) {
 #pragma unroll
-        for (int c = 0; c < kPanel; ++c) p[s][c] = (row < n && c < kb && row >= k0 + c) ? Lp[pk(row, k0 + c)] : 0.0;
-        rhs[s] = row < n ? vec[row] : 0.0;
-      }
-      bool ok = true;
-#pragma unroll
-      for (int c = 0; c < kPanel; ++c) {
-        if (c < kb) {
-          const double d = readlane_d(p[0][c], c);
-          if (!(d > 0.0)) ok = false;
-          // 1/sqrt(d) by the hardware estimate refined with one Newton-Raphson step, l = d / sqrt(d)
-          const double dd = fmax(d, 1e-300);
-          double il = __builtin_amdgcn_rsq(dd);
-          il = il * (1.5 - 0.5 * dd * il * il);
-          const double l = dd * il;
-          const double y = readlane_d(rhs[0], c) * il;
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const int row = k0 + lane + 64 * s;
-            if (row > k0 + c) {
-              p[s][c] *= il;
-              rhs[s] -= p[s][c] * y;
-            } else if (row == k0 + c) {
-              p[s][c] = l;
-              rhs[s] = y;
-              idiag[row] = il;
-            }
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * tJ[s] + 4 * v + li, c = 16 * tI[s] + lj;
+          if (r < n && c < n) {
+            dbg_S[static_cast<int64_t>(r) * n + c] = acc[s][v];
+            dbg_S[static_cast<int64_t>(c) * n + r] = acc[s][v];
           }
+        }
+      }
+    }
+    for (int i = tid; i < n; i += kFactorThreads) dbg_S[static_cast<int64_t>(n) * n + i] = static_cast<double>(RE[i]);
+  }
+  __syncthreads();
+
+  const bool keep = store_L && w == 0;
+  // the factor (packed lower L = U^T, then 1 / L_ii) kept for later solves with other right-hand sides
+  auto keep_tiles = [&]() {
 #pragma unroll
-          for (int c2 = c + 1; c2 < kPanel; ++c2) {
-            if (c2 < kb) {
-              const double l2 = readlane_d(p[0][c], c2);
+    for (int s = 0; s < kSlots; ++s) {
+      if (tJ[s] >= 0) {
 #pragma unroll
-              for (int s = 0; s < 4; ++s) {
-                const int row = k0 + lane + 64 * s;
-                if (row >= k0 + c2) p[s][c2] -= p[s][c] * l2;
-              }
+        for (int v = 0; v < 4; ++v) {
+          const int j = 16 * tJ[s] + 4 * v + li, i = 16 * tI[s] + lj;  // U[j][i] = L[i][j]
+          if (i < n && j <= i) store_L[pk(i, j)] = acc[s][v];
+        }
+      }
+    }
+  };
+  // 3. backward substitution U x = y: x_K = X_KK^T r_K (every wavefront), r_J -= U_JK x_K (owners of (J, K))
+  auto backward = [&](bool owner) {
+    for (int K = T - 1; K >= 0; --K) {
+      const double* xt = sXT + 256 * K;
+      double p = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) p += xt[16 * lj + 4 * li + kk] * ys[16 * K + 4 * li + kk];
+      p = sum_rows4(p);  // x_K[lane % 16]
+      if (!owner) {
+        if (lane < 16) xs[16 * K + lane] = p;
+      } else {
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+          if (tI[s] == K && tJ[s] >= 0) {
+            const int J = in_loop(tJ[s]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const double t = sum_row16(acc[s][v] * p);
+              if (lj == 0) ys[16 * J + 4 * v + li] -= t;
             }
           }
         }
       }
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int row = k0 + lane + 64 * s;
-        if (row < n) {
-#pragma unroll
-          for (int c = 0; c < kPanel; ++c) {
-            if (c < kb && row >= k0 + c) Lp[pk(row, k0 + c)] = p[s][c];
-            panel[c * 256 + (row - k0)] = c < kb ? p[s][c] : 0.0;
-          }
-          vec[row] = rhs[s];
-        }
-      }
-      if (!ok && lane == 0) flag[0] = 1;
+      __syncthreads();
     }
-    __syncthreads();
-    if (flag[0]) break;
-    // trailing update of rows/columns >= k0 + kb: a wave takes a 64-row strip and 4 columns; the strip's
-    // panel values are read lane-consecutive, the 4 columns' values are broadcasts (no bank conflicts)
-    const int t0 = k0 + kb;
-    const int tn = n - t0;
-    if (tn > 0) {
-      const int nstrip = (tn + 63) >> 6;
-      const int ngroup = (tn + 3) >> 2;
-      for (int item = warp; item < nstrip * ngroup; item += kFactorThreads / 64) {
-        const int strip = item / ngroup, grp = item - strip * ngroup;
-        const int i = t0 + 64 * strip + lane;
-        const int j0 = t0 + 4 * grp;
-        if (j0 > t0 + 64 * strip + 63) continue;  // the whole 64x4 block is above the diagonal
-        double li[kPanel];
-#pragma unroll
-        for (int c = 0; c < kPanel; ++c) li[c] = i < n ? panel[c * 256 + (i - k0)] : 0.0;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          const int j = j0 + jj;
-          if (j >= n) break;
-          double s2 = 0.0;
-#pragma unroll
-          for (int c = 0; c < kPanel; ++c) s2 += li[c] * panel[c * 256 + (j - k0)];
-          if (i < n && j <= i) Lp[pk(i, j)] -= s2;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  const bool ok = flag[0] == 0;
+  };
 
-  if (store_L && w == 0) {
-    // keep the factor (packed lower L, then 1 / L_ii) for later solves with other right-hand sides
-    for (int64_t idx = tid; idx < npk; idx += kFactorThreads) store_L[idx] = Lp[idx];
-    for (int i = tid; i < n; i += kFactorThreads) store_L[npk + i] = idiag[i];
+  // 2. factorization with the forward substitution; K = -1 only factors tile (0, 0). Wavefront 0 and the
+  //    others run their own copies of the loop (the same barriers), so each updates only its kind of tile.
+  if (wave == 0) {
+    for (int K = -1; K < T; ++K) {
+      if (flag) break;
+      if (K >= 0) {
+        const double* xt = sXT + 256 * K;
+        double yk = 0.0;  // y_K = X_KK e_K
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
+        yk = sum_rows4(yk);
+        if (lane < 16) ys[16 * K + lane] = yk;
+        __syncthreads();
+      }
+      // the next diagonal tile first: update, factor; then the remaining diagonal tiles
+      if (K + 1 < T) {
+        double4v dt = acc[0];
+#pragma unroll
+        for (int s = 1; s < kSlots; ++s)
+          if (s == K + 1) dt = acc[s];
+        if (K >= 0) dt = tile_update(dt, sP + 256 * (K + 1), sP + 256 * (K + 1), lane);
+        if (!factor_tile(dt, cb, xb, sXT + 256 * (K + 1), idg + 16 * (K + 1), lane) && lane == 0) flag = 1;
+        if (K < 0) acc[0] = dt;
+#pragma unroll
+        for (int s = 1; s < kSlots; ++s) {
+          if (s == K + 1) {
+            acc[s] = dt;
+          } else if (K >= 0 && s > K + 1 && s < T) {
+            acc[s] = tile_update(acc[s], sP + 256 * in_loop(s), sP + 256 * in_loop(s), lane);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (keep) keep_tiles();
+    else if (flag == 0) backward(false);
+  } else {
+    for (int K = -1; K < T; ++K) {
+      if (flag) break;
+      if (K >= 0) {
+        // panel U_KI = X_KK A_KI (the B operand of chunk q is register q of the tile itself), published to
+        // LDS, and e_I -= U_KI^T y_K
+        const double* xt = sXT + 256 * K;
+        double yk = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
+        yk = sum_rows4(yk);
+        double yv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) yv[v] = __shfl(yk, 4 * v + li);  // y_K[4v + lane/16]
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+          if (tJ[s] == K) {
+            const int I = in_loop(tI[s]);
+            double4v u = double4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) u = mfma64(xt[64 * q + lane], acc[s][q], u);
+            acc[s] = u;
+            double p = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              sP[256 * I + 64 * v + lane] = u[v];
+              p += u[v] * yv[v];
+            }
+            p = sum_rows4(p);
+            if (lane < 16) vec[16 * I + lane] -= p;
+          }
+        }
+        __syncthreads();
+        // trailing A_JI -= U_KJ^T U_KI, K < J < I
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+          if (tJ[s] > K) {
+            acc[s] = tile_update(acc[s], sP + 256 * in_loop(tJ[s]), sP + 256 * in_loop(tI[s]), lane);
+            __builtin_amdgcn_sched_barrier(0);  // one tile's operands live at a time
+          }
+      }
+      __syncthreads();
+    }
+    if (keep) keep_tiles();
+    else if (flag == 0) backward(true);
+  }
+  const bool ok = flag == 0;
+
+  if (keep) {
+    const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
+    for (int i = tid; i < n; i += kFactorThreads) store_L[npk + i] = idg[i];
     if (tid == 0) St->chol_ok = ok ? 1 : 0;
     return;
   }
 
   if (ok) {
-    // 3. backward substitution L^T x = y in one wavefront: y stays in registers (row k0 + lane + 64 s),
-    //    x_i = y_i / L_ii is broadcast from its lane, and every lane subtracts L_ik x_i from its rows k < i
-    if (warp == 0) {
-      double yv[4];
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2) yv[s2] = (lane + 64 * s2 < n) ? vec[lane + 64 * s2] : 0.0;
-#pragma unroll
-      for (int s2 = 3; s2 >= 0; --s2) {
-        const int lo = 64 * s2;
-        for (int ii = min(n - 1, lo + 63); ii >= lo; --ii) {
-          const double xi = readlane_d(yv[s2], ii - lo) * idiag[ii];
-          const int64_t rb = pk(ii, 0);
-#pragma unroll
-          for (int s3 = 0; s3 <= s2; ++s3) {
-            const int k = lane + 64 * s3;
-            if (k < ii) yv[s3] -= Lp[rb + k] * xi;
-            else if (k == ii) yv[s3] = xi;
-          }
-        }
-      }
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2)
-        if (lane + 64 * s2 < n) vec[lane + 64 * s2] = yv[s2];
-    }
-    __syncthreads();
     // 4. the pose step, its max-norm and dc^T diag(U) dc
     double sc = 0.0;
     float mx = 0.f;
     for (int i = tid; i < n; i += kFactorThreads) {
-      const double x = vec[i];
+      const double x = xs[i];
       b.dc[W.dc_off + i] = static_cast<float>(x);
       b.dc64[W.dc_off + i] = x;
       sc += x * x * du[i];
@@ -887,8 +1037,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     if (lane == 0) {
-      red[warp] = sc;
-      red[16 + warp] = static_cast<double>(mx);
+      red[wave] = sc;
+      red[16 + wave] = static_cast<double>(mx);
     }
     __syncthreads();
     if (tid == 0) {
@@ -901,7 +1051,7 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
       St->maxdc = static_cast<float>(m);
     }
     // 5. trial key frames: Exp(dc_j) * T_j, rotation re-orthonormalized, rounded to FP32
-    write_trial_poses(b, W, St->cur, vec, tid, kFactorThreads);
+    write_trial_poses(b, W, St->cur, xs, tid, kFactorThreads);
   }
   if (tid == 0) St->chol_ok = ok ? 1 : 0;
 }
@@ -1237,7 +1387,7 @@ hipError_t launch_reduce(const Batch& b, hipStream_t s) {
 }
 
 hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s, double* store_L) {
-  hipLaunchKernelGGL(k_factor, dim3(b.n_windows), dim3(kFactorThreads), static_cast<size_t>(b.factor_lds_words) * 4, s,
+  hipLaunchKernelGGL(k_factor, dim3(b.n_windows), dim3(kFactorThreads), 0, s,
                      b, dbg_S, store_L);
   return hipGetLastError();
 }
