@@ -680,9 +680,9 @@ __global__ __launch_bounds__(64) void k_reduce(Batch b) {
 // FP64 factor/solve of the reduced camera system, one workgroup per window, the matrix held in registers.
 //
 // S + lambda diag(U) (n = 6 x free key frames <= 240) is cut into 16 x 16 tiles; A = U^T U is factored as the
-// upper tiles U_JI (J <= I), each tile in the registers of one wavefront in the FP64 matrix-core layout
-// (element (4v + lane / 16, lane % 16) in register v of the lane, i.e. 64 v + lane in row-major order).
-// Wavefront 0 holds the diagonal tiles, wavefronts 1..7 the others. Right-looking, per tile column K:
+// upper tiles U_JI (J <= I) in the FP64 matrix-core layout (element (4v + lane / 16, lane % 16) in register v
+// of the lane, i.e. 64 v + lane in row-major order): wavefronts 1..7 keep the off-diagonal tiles in registers,
+// wavefront 0 the diagonal tiles in LDS. Right-looking, per tile column K:
 //   panel     U_KI = X_KK A_KI with the inverse X_KK = U_KK^-T (4 MFMA 16x16x4), published to LDS, and the
 //             forward substitution e_I -= U_KI^T y_K;
 //   trailing  A_JI -= U_KJ^T U_KI for K < J <= I (4 MFMA each); wavefront 0 first updates (K+1, K+1) and
@@ -695,7 +695,7 @@ namespace {
 
 typedef double double4v __attribute__((ext_vector_type(4)));
 constexpr int kFT = 15;     // tile rows of the largest reduced system (240 = 15 x 16)
-constexpr int kSlots = 15;  // tiles per wavefront: wavefront 0 the <= 15 diagonal ones, 1..7 the <= 105 others
+constexpr int kSlots = 15;  // register tiles per wavefront: 7 wavefronts hold the <= 105 off-diagonal tiles
 
 __device__ __forceinline__ double readlane_d(double x, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
@@ -736,49 +736,97 @@ __device__ __forceinline__ double sum_row16(double x) {  // over the 16 lanes of
 }
 
 // Cholesky of a symmetric 16 x 16 tile held by one wavefront in the FP64 matrix-core layout (cell
-// (4v + lane/16, lane%16) in t[v]), right-looking: per pivot c the current column c and row c of X go through
-// LDS (cb, xb: 16 doubles each) and every lane updates its four cells. On return t holds U = L^T on and above
-// the diagonal (cells below it are left unspecified), X^T = L^-T is written to xt (row-major) and 1 / L_ii to
-// rdiag. X = L^-1 is built by the same elementary column transforms, starting from the identity.
-// False on a non-positive pivot.
-__device__ bool factor_tile(double4v& t, double* cb, double* xb, double* xt, double* rdiag, int lane) {
-  const int li = lane >> 4, lj = lane & 15;
-  double4v x;
+// (4v + lane/16, lane%16) in t[v]). The tile goes through LDS (sT, row-major) to rows in lanes: lane i < 16 keeps
+// row i of L, pivots and column entries broadcast with readlane. Then X = L^-1 by forward substitution, lane j
+// solving column j against L read back from LDS. On return t holds U = L^T (zero below the diagonal), X^T is
+// written to xt (row-major) and 1 / L_ii to rdiag. False on a non-positive pivot.
+__device__ bool factor_tile(double4v& t, double* sT, double* xt, double* rdiag, int lane) {
+  const int i = lane & 15;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) x[v] = 4 * v + li == lj ? 1.0 : 0.0;
+  for (int v = 0; v < 4; ++v) sT[64 * v + lane] = t[v];
+  wave_lds_sync();
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = sT[16 * i + q];
+  wave_lds_sync();  // every row read before sT is rewritten
   bool ok = true;
-#pragma unroll 1
-  for (int c = 0; c < 16; ++c) {
-    if (lj == c) {
+  double ri = 0.0;  // 1 / L_ii of this lane's row
 #pragma unroll
-      for (int v = 0; v < 4; ++v) cb[4 * v + li] = t[v];
-    }
-    if (li == (c & 3)) {
-      const int h = c >> 2;
-      xb[lj] = h == 0 ? x[0] : h == 1 ? x[1] : h == 2 ? x[2] : x[3];
-    }
-    wave_lds_sync();
-    const double d = cb[c];
+  for (int c = 0; c < 16; ++c) {
+    const double d = readlane_d(a[c], c);
     if (!(d > 0.0)) ok = false;
-    // 1/sqrt(d) by the hardware estimate refined with one Newton-Raphson step
+    // 1/sqrt(d) by the hardware estimate refined with one Newton-Raphson step, L_cc = d / sqrt(d)
     const double dd = fmax(d, 1e-300);
     double il = __builtin_amdgcn_rsq(dd);
     il = il * (1.5 - 0.5 * dd * il * il);
-    const double lc = cb[lj] * il;  // L[lj][c] = U[c][lj]
-    const double xc = xb[lj] * il;  // X[c][lj], final
+    a[c] = i == c ? dd * il : (i > c ? a[c] * il : 0.0);
+    ri = i == c ? il : ri;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = 4 * v + li;
-      const double lr = cb[r] * il;  // L[r][c] for r > c
-      t[v] = (r > c && lj > c) ? t[v] - lr * lc : (r == c ? lc : t[v]);
-      x[v] = r > c ? x[v] - lr * xc : (r == c ? xc : x[v]);
+    for (int c2 = c + 1; c2 < 16; ++c2) {
+      const double l2 = readlane_d(a[c], c2);  // L[c2][c]
+      if (i >= c2) a[c2] -= a[c] * l2;
     }
-    if (lane == 0) rdiag[c] = il;
-    wave_lds_sync();  // every lane has read cb and xb before the next pivot overwrites them
+    __builtin_amdgcn_sched_barrier(0);  // one pivot's broadcasts live at a time
+  }
+  if (lane < 16) {
+    rdiag[lane] = ri;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sT[16 * q + i] = a[q];  // U[q][i] = L[i][q], zero for q > i
+  }
+  wave_lds_sync();
+  // X[r][j] = (delta_rj - sum_{m<r} L[r][m] X[m][j]) / L_rr for lane j, L[r][m] = U[m][r]; four partial sums
+  double x[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    double p0 = r == i ? 1.0 : 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+    for (int m = 0; m < r; m += 4) {
+      p0 -= sT[16 * m + r] * x[m];
+      if (m + 1 < r) p1 -= sT[16 * (m + 1) + r] * x[m + 1];
+      if (m + 2 < r) p2 -= sT[16 * (m + 2) + r] * x[m + 2];
+      if (m + 3 < r) p3 -= sT[16 * (m + 3) + r] * x[m + 3];
+    }
+    x[r] = ((p0 + p1) + (p2 + p3)) * rdiag[r];
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) xt[16 * lane + q] = x[q];  // X^T[lane][q]
   }
 #pragma unroll
-  for (int v = 0; v < 4; ++v) xt[16 * lj + 4 * v + li] = x[v];
+  for (int v = 0; v < 4; ++v) t[v] = sT[64 * v + lane];
   return ok;
+}
+
+// Sum over the 16 lanes of each row of 16 (DPP within rows; every lane gets its row's sum).
+__device__ __forceinline__ double dpp_f64(double x, int ctrl_sel) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  switch (ctrl_sel) {
+    case 0:
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 1:
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 2:
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xF, 0xF, false);  // row_ror:4
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xF, 0xF, false);
+      break;
+    default:
+      lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xF, 0xF, false);  // row_ror:8
+      hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xF, 0xF, false);
+      break;
+  }
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double sum_row16_dpp(double x) {
+  x += dpp_f64(x, 0);
+  x += dpp_f64(x, 1);
+  x += dpp_f64(x, 2);
+  x += dpp_f64(x, 3);
+  return x;
 }
 
 }  // namespace
@@ -791,7 +839,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   __shared__ double du[256];   // diag(U)
   __shared__ double idg[256];  // 1 / L_ii
   __shared__ double red[32];
-  __shared__ double cb[16], xb[16];   // pivot column and X row of the diagonal tile being factored
+  __shared__ double sT[256];          // the diagonal tile being factored
+  __shared__ double sDg[kFT * 256];   // diagonal tiles (wavefront 0), row-major
   __shared__ double sXT[kFT * 256];   // X_KK^T of every tile column
   __shared__ double sP[kFT * 256];    // panel tiles U_KI of the current tile column
   __shared__ int flag;
@@ -811,16 +860,113 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   const float* RE = RU + 21 * ncf;
   const float* RD = RE + 6 * ncf;
 
-  // tiles of this wavefront (wavefront-uniform)
-  int tJ[kSlots], tI[kSlots];
-  {
-    const int noff = T * (T - 1) / 2;
+  // element (4v + lane/16, lane%16) of upper tile (J, I) of A = U - sum Z Z^T + lambda diag(U); rows and columns
+  // past n are identity. Stored sum Z Z^T tiles hold element (row, col) at 64 (row & 3) + 16 (row >> 2) + col;
+  // diagonal tiles are read above their diagonal.
+  auto element = [&](int J, int I, int v) -> double {
+    // branch-free, every load in bounds, so that all the loads of a wavefront's tiles are in flight together
+    const int rl = 4 * v + li, r = 16 * J + rl, c = 16 * I + lj;
+    const bool inside = r < n && c < n;
+    const bool mir = J == I && rl > lj;
+    const int er = mir ? lj : rl, ec = mir ? rl : lj;
+    const float zz = R[256 * upper_tile(J, I, T) + 64 * (er & 3) + 16 * (er >> 2) + ec];
+    const int lo = min(r, c), hi = max(r, c), pose = lo / 6, rr = lo - 6 * pose, cc = hi - 6 * pose;
+    const bool blk = inside && cc < 6;
+    const float u = RU[blk ? 21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr) : 0];
+    const float dg = RD[min(r, n - 1)];
+    const double val = -static_cast<double>(zz) + (blk ? static_cast<double>(u) : 0.0) +
+                       (r == c ? static_cast<double>(lam) * static_cast<double>(dg) : 0.0);
+    return inside ? val : (r == c ? 1.0 : 0.0);
+  };
+  auto dbg_tile = [&](int J, int I, int v, double val) {
+    const int r = 16 * J + 4 * v + li, c = 16 * I + lj;
+    if (r < n && c < n) {
+      dbg_S[static_cast<int64_t>(r) * n + c] = val;
+      dbg_S[static_cast<int64_t>(c) * n + r] = val;
+    }
+  };
+  for (int i = tid; i < 256; i += kFactorThreads) {
+    vec[i] = i < n ? static_cast<double>(RE[i]) : 0.0;
+    du[i] = i < n ? static_cast<double>(RD[i]) : 0.0;
+  }
+  if (tid == 0) flag = 0;
+  if (dbg_S && w == 0)
+    for (int i = tid; i < n; i += kFactorThreads) dbg_S[static_cast<int64_t>(n) * n + i] = static_cast<double>(RE[i]);
+  const bool keep = store_L && w == 0;
+  const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
+  // the factor (packed lower L = U^T, then 1 / L_ii) kept for later solves with other right-hand sides
+  auto keep_tile = [&](int J, int I, int v, double val) {
+    const int j = 16 * J + 4 * v + li, i = 16 * I + lj;  // U[j][i] = L[i][j]
+    if (i < n && j <= i) store_L[pk(i, j)] = val;
+  };
+
+  // the diagonal tiles, kept in LDS, are assembled by all wavefronts
+  for (int J = wave; J < T; J += kFactorThreads / 64) {
+    double val[4];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-      if (wave == 0) {
-        tJ[s] = s < T ? s : -1;
-        tI[s] = s;
-      } else {
+    for (int v = 0; v < 4; ++v) val[v] = element(J, J, v);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sDg[256 * J + 64 * v + lane] = val[v];
+    if (dbg_S && w == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dbg_tile(J, J, v, val[v]);
+    }
+  }
+
+  if (wave == 0) {
+    // wavefront 0: the diagonal factors
+    __syncthreads();
+    // 2. factorization; K = -1 only factors tile (0, 0). The other wavefronts run their own copy of the loop
+    //    with the same barriers.
+    for (int K = -1; K < T; ++K) {
+      if (flag) break;
+      if (K >= 0) {
+        const double* xt = sXT + 256 * K;
+        double yk = 0.0;  // y_K = X_KK e_K
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
+        yk = sum_rows4(yk);
+        if (lane < 16) ys[16 * K + lane] = yk;
+        __syncthreads();
+      }
+      // the next diagonal tile first: update, factor; then the remaining diagonal tiles
+      if (K + 1 < T) {
+        double* tk = sDg + 256 * (K + 1);
+        double4v dt;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dt[v] = tk[64 * v + lane];
+        if (K >= 0) dt = tile_update(dt, sP + 256 * (K + 1), sP + 256 * (K + 1), lane);
+        if (!factor_tile(dt, sT, sXT + 256 * (K + 1), idg + 16 * (K + 1), lane) && lane == 0) flag = 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tk[64 * v + lane] = dt[v];
+      }
+      __syncthreads();
+    }
+    if (keep) {
+      for (int J = 0; J < T; ++J)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) keep_tile(J, J, v, sDg[256 * J + 64 * v + lane]);
+    } else if (flag == 0) {
+      // 3. backward substitution U x = y, tile column K from T - 1 down: x_K = X_KK^T r_K (here), then
+      //    r_J -= U_JK x_K (the owners of (J, K))
+      for (int K = T - 1; K >= 0; --K) {
+        const double* xt = sXT + 256 * K;
+        double p = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) p += xt[16 * lj + 4 * li + kk] * ys[16 * K + 4 * li + kk];
+        p = sum_rows4(p);
+        if (lane < 16) xs[16 * K + lane] = p;
+        __syncthreads();
+      }
+    }
+  } else {
+    // wavefronts 1..7: the off-diagonal tiles (J < I) in registers, tile m of the row-major order in slot
+    // (m / 7) of wavefront 1 + m % 7
+    int tJ[kSlots], tI[kSlots];
+    {
+      const int noff = T * (T - 1) / 2;
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) {
         int m = 7 * s + wave - 1, J = 0;
         if (m < noff) {
           while (m >= T - 1 - J) {
@@ -835,139 +981,23 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
         }
       }
     }
-  }
-
-  // 1. assemble the tiles of A = U - sum Z Z^T + lambda diag(U); rows and columns past n are identity
-  double4v acc[kSlots];
-#pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    acc[s] = double4v{0.0, 0.0, 0.0, 0.0};
-    if (tJ[s] >= 0) {
-      const int J = tJ[s], I = tI[s];
-      const float* tile = R + 256 * upper_tile(J, I, T);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int rl = 4 * v + li, r = 16 * J + rl, c = 16 * I + lj;
-        double val = r == c ? 1.0 : 0.0;
-        if (r < n && c < n) {
-          // stored element (row, col) at 64 (row & 3) + 16 (row >> 2) + col; diagonal tiles read above the diagonal
-          const bool mir = J == I && rl > lj;
-          const int er = mir ? lj : rl, ec = mir ? rl : lj;
-          val = -static_cast<double>(tile[64 * (er & 3) + 16 * (er >> 2) + ec]);
-          const int lo = min(r, c), hi = max(r, c);
-          if (lo / 6 == hi / 6) {
-            const int pose = lo / 6, rr = lo - 6 * pose, cc = hi - 6 * pose;
-            val += static_cast<double>(RU[21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)]);
-          }
-          if (r == c) val += static_cast<double>(lam) * static_cast<double>(RD[r]);
-        }
-        acc[s][v] = val;
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  for (int i = tid; i < 256; i += kFactorThreads) {
-    vec[i] = i < n ? static_cast<double>(RE[i]) : 0.0;
-    du[i] = i < n ? static_cast<double>(RD[i]) : 0.0;
-  }
-  if (tid == 0) flag = 0;
-  if (dbg_S && w == 0) {
+    double4v acc[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
+      acc[s] = double4v{0.0, 0.0, 0.0, 0.0};
       if (tJ[s] >= 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = 16 * tJ[s] + 4 * v + li, c = 16 * tI[s] + lj;
-          if (r < n && c < n) {
-            dbg_S[static_cast<int64_t>(r) * n + c] = acc[s][v];
-            dbg_S[static_cast<int64_t>(c) * n + r] = acc[s][v];
-          }
-        }
+        for (int v = 0; v < 4; ++v) acc[s][v] = element(tJ[s], tI[s], v);
       }
     }
-    for (int i = tid; i < n; i += kFactorThreads) dbg_S[static_cast<int64_t>(n) * n + i] = static_cast<double>(RE[i]);
-  }
-  __syncthreads();
-
-  const bool keep = store_L && w == 0;
-  // the factor (packed lower L = U^T, then 1 / L_ii) kept for later solves with other right-hand sides
-  auto keep_tiles = [&]() {
+    if (dbg_S && w == 0) {
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-      if (tJ[s] >= 0) {
+      for (int s = 0; s < kSlots; ++s)
+        if (tJ[s] >= 0)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int j = 16 * tJ[s] + 4 * v + li, i = 16 * tI[s] + lj;  // U[j][i] = L[i][j]
-          if (i < n && j <= i) store_L[pk(i, j)] = acc[s][v];
-        }
-      }
+          for (int v = 0; v < 4; ++v) dbg_tile(tJ[s], tI[s], v, acc[s][v]);
     }
-  };
-  // 3. backward substitution U x = y: x_K = X_KK^T r_K (every wavefront), r_J -= U_JK x_K (owners of (J, K))
-  auto backward = [&](bool owner) {
-    for (int K = T - 1; K >= 0; --K) {
-      const double* xt = sXT + 256 * K;
-      double p = 0.0;
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) p += xt[16 * lj + 4 * li + kk] * ys[16 * K + 4 * li + kk];
-      p = sum_rows4(p);  // x_K[lane % 16]
-      if (!owner) {
-        if (lane < 16) xs[16 * K + lane] = p;
-      } else {
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-          if (tI[s] == K && tJ[s] >= 0) {
-            const int J = in_loop(tJ[s]);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const double t = sum_row16(acc[s][v] * p);
-              if (lj == 0) ys[16 * J + 4 * v + li] -= t;
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  };
-
-  // 2. factorization with the forward substitution; K = -1 only factors tile (0, 0). Wavefront 0 and the
-  //    others run their own copies of the loop (the same barriers), so each updates only its kind of tile.
-  if (wave == 0) {
-    for (int K = -1; K < T; ++K) {
-      if (flag) break;
-      if (K >= 0) {
-        const double* xt = sXT + 256 * K;
-        double yk = 0.0;  // y_K = X_KK e_K
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
-        yk = sum_rows4(yk);
-        if (lane < 16) ys[16 * K + lane] = yk;
-        __syncthreads();
-      }
-      // the next diagonal tile first: update, factor; then the remaining diagonal tiles
-      if (K + 1 < T) {
-        double4v dt = acc[0];
-#pragma unroll
-        for (int s = 1; s < kSlots; ++s)
-          if (s == K + 1) dt = acc[s];
-        if (K >= 0) dt = tile_update(dt, sP + 256 * (K + 1), sP + 256 * (K + 1), lane);
-        if (!factor_tile(dt, cb, xb, sXT + 256 * (K + 1), idg + 16 * (K + 1), lane) && lane == 0) flag = 1;
-        if (K < 0) acc[0] = dt;
-#pragma unroll
-        for (int s = 1; s < kSlots; ++s) {
-          if (s == K + 1) {
-            acc[s] = dt;
-          } else if (K >= 0 && s > K + 1 && s < T) {
-            acc[s] = tile_update(acc[s], sP + 256 * in_loop(s), sP + 256 * in_loop(s), lane);
-            __builtin_amdgcn_sched_barrier(0);
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (keep) keep_tiles();
-    else if (flag == 0) backward(false);
-  } else {
+    __syncthreads();
     for (int K = -1; K < T; ++K) {
       if (flag) break;
       if (K >= 0) {
@@ -1000,7 +1030,17 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
           }
         }
         __syncthreads();
-        // trailing A_JI -= U_KJ^T U_KI, K < J < I
+        // trailing A_JI -= U_KJ^T U_KI, K < J < I; and the diagonal tiles past K + 1 (in LDS), J = K + 2 + m
+        // by wavefront 1 + m % 7, while wavefront 0 factors tile K + 1
+        for (int J = K + 1 + wave; J < T; J += 7) {
+          double* tj = sDg + 256 * J;
+          double4v t;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) t[v] = tj[64 * v + lane];
+          t = tile_update(t, sP + 256 * J, sP + 256 * J, lane);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tj[64 * v + lane] = t[v];
+        }
 #pragma unroll
         for (int s = 0; s < kSlots; ++s)
           if (tJ[s] > K) {
@@ -1010,13 +1050,37 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
       }
       __syncthreads();
     }
-    if (keep) keep_tiles();
-    else if (flag == 0) backward(true);
+    if (keep) {
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s)
+        if (tJ[s] >= 0)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) keep_tile(tJ[s], tI[s], v, acc[s][v]);
+    } else if (flag == 0) {
+      for (int K = T - 1; K >= 0; --K) {
+        const double* xt = sXT + 256 * K;
+        double p = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) p += xt[16 * lj + 4 * li + kk] * ys[16 * K + 4 * li + kk];
+        p = sum_rows4(p);  // x_K[lane % 16]
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+          if (tI[s] == K && tJ[s] >= 0) {
+            const int J = in_loop(tJ[s]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const double t = sum_row16_dpp(acc[s][v] * p);
+              if (lj == 0) ys[16 * J + 4 * v + li] -= t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
   const bool ok = flag == 0;
 
   if (keep) {
-    const int64_t npk = static_cast<int64_t>(n) * (n + 1) / 2;
     for (int i = tid; i < n; i += kFactorThreads) store_L[npk + i] = idg[i];
     if (tid == 0) St->chol_ok = ok ? 1 : 0;
     return;
