@@ -220,7 +220,7 @@ __device__ __forceinline__ void load_pose_lds(float* dst, const float4* src, int
 // point, zero where a key frame does not see the point), and the chunk's sum Z Z^T accumulates in
 // registers on the FP32 matrix cores (v_mfma_f32_16x16x4_f32) over the upper 16x16 tiles of S. Each wave owns
 // a fixed quarter of the tiles. Nothing is atomically added in global memory: a chunk writes its record once
-// and k_reduce sums the records of a window in chunk order.
+// and k_reduce sums the records of a window in a fixed order.
 // ---------------------------------------------------------------------------------------------------------
 namespace {
 
@@ -654,30 +654,37 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   }
 }
 
-// Sums the chunk records of each window in chunk order (deterministic; no atomics).
-__global__ __launch_bounds__(64) void k_reduce(Batch b) {
+// Sums the chunk records of each window (deterministic; no atomics): the four wavefronts of a workgroup take
+// consecutive quarters of the chunks for the same 64 record entries, each summed in chunk order with eight
+// loads in flight, and the quarters are added in a fixed order.
+__global__ __launch_bounds__(256) void k_reduce(Batch b) {
+  __shared__ float sq[3][64];
   const int w = blockIdx.y;
   const WindowDesc W = b.win[w];
   if (b.st[w].done || W.mapped) return;
-  const float* src = b.part + W.part_off;
-  float* dst = b.sred + W.sred_off;
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 64 + lane;
   const int64_t rec = W.rec;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W.rec; e += gridDim.x * blockDim.x) {
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int c = 0;
-    for (; c + 4 <= W.nchunks; c += 4) {
-      s0 += src[(c + 0) * rec + e];
-      s1 += src[(c + 1) * rec + e];
-      s2 += src[(c + 2) * rec + e];
-      s3 += src[(c + 3) * rec + e];
+  const int c0 = (W.nchunks * q) / 4, c1 = (W.nchunks * (q + 1)) / 4;
+  float s = 0.f;
+  if (e < rec) {
+    const float* src = b.part + W.part_off + e;
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += src[(c + k) * rec];
     }
-    for (; c < W.nchunks; ++c) s0 += src[c * rec + e];
-    dst[e] = (s0 + s1) + (s2 + s3);
+    for (; c < c1; ++c) a[0] += src[c * rec];
+    s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
+  if (q > 0) sq[q - 1][lane] = s;
+  __syncthreads();
+  if (q == 0 && e < rec) b.sred[W.sred_off + e] = ((s + sq[0][lane]) + sq[1][lane]) + sq[2][lane];
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// FP64 factor/solve of the reduced camera system, one workgroup per window, the matrix held in registers.
+// FP64 factor/solve of the reduced camera system, one workgroup per window.
 //
 // S + lambda diag(U) (n = 6 x free key frames <= 240) is cut into 16 x 16 tiles; A = U^T U is factored as the
 // upper tiles U_JI (J <= I) in the FP64 matrix-core layout (element (4v + lane / 16, lane % 16) in register v
@@ -1446,7 +1453,7 @@ hipError_t launch_schur(const Batch& b, hipStream_t s) {
 
 hipError_t launch_reduce(const Batch& b, hipStream_t s) {
   const int gx = (b.max_rec + 63) / 64;
-  hipLaunchKernelGGL(k_reduce, dim3(gx > 0 ? gx : 1, b.n_windows), dim3(64), 0, s, b);
+  hipLaunchKernelGGL(k_reduce, dim3(gx > 0 ? gx : 1, b.n_windows), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
