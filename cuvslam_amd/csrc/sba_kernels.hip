@@ -402,7 +402,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       tp = pre.rec.w & 255;
       jl = pre.rec.w >> 8;
       const int i = T.point_begin + tp;
-      j = pre.rec.z;
+      j = pre.rec.z & 255;
       pair_pf = i < pf_end;
       pair_cf = j < ncf;
       const float4 p = sPts[(t & 1) * kTilePoints + tp];
@@ -654,14 +654,29 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   }
 }
 
-// Sums the chunk records of each window (deterministic; no atomics): the four wavefronts of a workgroup take
-// consecutive quarters of the chunks for the same 64 record entries, each summed in chunk order with eight
-// loads in flight, and the quarters are added in a fixed order.
+// Sums the chunk records of each window (deterministic; no atomics). A window of many chunks: the four
+// wavefronts of a workgroup take consecutive quarters of the chunks for the same 64 record entries, each
+// summed in chunk order with eight loads in flight, and the quarters are added in a fixed order. A window of
+// few chunks: one entry per thread over all its chunks.
 __global__ __launch_bounds__(256) void k_reduce(Batch b) {
   __shared__ float sq[3][64];
   const int w = blockIdx.y;
   const WindowDesc W = b.win[w];
   if (b.st[w].done || W.mapped) return;
+  if (W.nchunks < 16) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= W.rec) return;
+    const float* src = b.part + W.part_off + e;
+    float a0 = 0.f, a1 = 0.f;
+    int c = 0;
+    for (; c + 2 <= W.nchunks; c += 2) {
+      a0 += src[c * static_cast<int64_t>(W.rec)];
+      a1 += src[(c + 1) * static_cast<int64_t>(W.rec)];
+    }
+    if (c < W.nchunks) a0 += src[c * static_cast<int64_t>(W.rec)];
+    b.sred[W.sred_off + e] = a0 + a1;
+    return;
+  }
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int64_t e = static_cast<int64_t>(blockIdx.x) * 64 + lane;
   const int64_t rec = W.rec;
@@ -1136,7 +1151,6 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   __shared__ float sTrial[12 * kMaxPoses];
   __shared__ float sCam[12 * kMaxCameras];
   __shared__ float sAcc[4 * kTilePoints];  // sum_j W_ij^T dc_j, then dp
-  __shared__ int sPP[kTilePairs];
   __shared__ float sDc[6 * kMaxPoses];
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];
@@ -1168,31 +1182,34 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
     const Tile T = b.btiles[t];
-    const int p0 = b.point_pair[T.point_begin];
-    const int npair = b.point_pair[T.point_end] - p0;
+    const int p0 = T.pair_begin;
+    const int npair = T.pair_end - p0;
     const int npts = T.point_end - T.point_begin;
-    __syncthreads();
-    if (tid < npts) {
-      const int i = T.point_begin + tid;
-      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
-      for (int k = a; k < e; ++k) sPP[k] = tid;
-      sAcc[4 * tid] = sAcc[4 * tid + 1] = sAcc[4 * tid + 2] = sAcc[4 * tid + 3] = 0.f;
+    // this thread's pair: record (first observation, end, key frame, point in the tile) and first observation
+    int4 rec = make_int4(0, 0, 0, 0);
+    float2 xy0 = make_float2(0.f, 0.f);
+    float4 inf0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid < npair) {
+      rec = b.pair_rec[p0 + tid];
+      xy0 = b.pair_xy[p0 + tid];
+      inf0 = b.pair_info[p0 + tid];
     }
+    __syncthreads();
+    if (tid < npts) sAcc[4 * tid] = sAcc[4 * tid + 1] = sAcc[4 * tid + 2] = sAcc[4 * tid + 3] = 0.f;
     __syncthreads();
     if (!INITIAL) {
       // (a) sum_j W_ij^T dc_j = sum_obs w J_p^T Lambda J_c dc_j
       if (tid < npair) {
-        const int pair = p0 + tid;
-        const int tp = sPP[tid];
+        const int tp = rec.z >> 8;  // point inside this back-substitution tile
         const int i = T.point_begin + tp;
-        const int j = b.pair_pose[pair];
+        const int j = rec.z & 255;
         if (i < pf_end && j < ncf) {
           const float4 p = pts[i];
           const float* d = sDc + 6 * j;
           float acc[3] = {0.f, 0.f, 0.f};
-          for (int o = b.pair_obs[pair]; o < b.pair_obs[pair + 1]; ++o) {
-            const float2 xy = b.obs_xy[o];
-            const float4 inf = b.obs_info[o];
+          for (int o = rec.x; o < rec.y; ++o) {
+            const float2 xy = o == rec.x ? xy0 : b.obs_xy[o];
+            const float4 inf = o == rec.x ? inf0 : b.obs_info[o];
             Lin L;
             if (!linearize(sPose + 12 * j, sCam + 12 * __float_as_int(inf.w), p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y,
                            inf.z, kind, delta, L))
@@ -1239,10 +1256,9 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
     }
     // (c) one thread per pair: trial cost in FP64, and w |J_c dc + J_p dp|^2_Lambda
     if (tid < npair) {
-      const int pair = p0 + tid;
-      const int tp = sPP[tid];
+      const int tp = rec.z >> 8;
       const int i = T.point_begin + tp;
-      const int j = b.pair_pose[pair];
+      const int j = rec.z & 255;
       const float4 p = pts[i];
       float dp[3] = {0.f, 0.f, 0.f}, dcj[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (!INITIAL) {
@@ -1259,9 +1275,9 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
       const double prx = Pt[0] * qx + Pt[1] * qy + Pt[2] * qz + Pt[3];
       const double pry = Pt[4] * qx + Pt[5] * qy + Pt[6] * qz + Pt[7];
       const double prz = Pt[8] * qx + Pt[9] * qy + Pt[10] * qz + Pt[11];
-      for (int o = b.pair_obs[pair]; o < b.pair_obs[pair + 1]; ++o) {
-        const float2 xy = b.obs_xy[o];
-        const float4 inf = b.obs_info[o];
+      for (int o = rec.x; o < rec.y; ++o) {
+        const float2 xy = o == rec.x ? xy0 : b.obs_xy[o];
+        const float4 inf = o == rec.x ? inf0 : b.obs_info[o];
         const float* C = sCam + 12 * __float_as_int(inf.w);
         const double pcx = C[0] * prx + C[1] * pry + C[2] * prz + C[3];
         const double pcy = C[4] * prx + C[5] * pry + C[6] * prz + C[7];

@@ -141,114 +141,131 @@ __global__ __launch_bounds__(256) void k_lassemble(Batch b, int w) {
   if (blockIdx.x == 0 && threadIdx.x == 0) St->chol_ok = 1;
 }
 
-// Cholesky of diagonal block k in LDS by one workgroup of 256 threads, in panels of 4 columns: wavefront 0
-// factors the panel with its rows in registers (pivots and column entries broadcast with readlane) and writes the
-// 4 scaled columns to sCol; then every thread applies the rank-4 update to its cells of the trailing triangle
-// (loads before stores). Then the inverse X = L_kk^-1 (lane j of wavefront 0 solves column j of L X = I), and
-// the forward-substitution step y_k = X e_k.
+// Cholesky of diagonal block k by one workgroup of 256 threads, with its inverse X = L_kk^-1 and the forward-
+// substitution step y_k = X e_k formed alongside (Golub & Van Loan 4.2 and 3.1, column-oriented). Thread
+// (wavefront v, lane c) keeps rows r = 4t + v (t < 16) of column c of both the block and X in registers. Per panel
+// of 4 columns: the panel columns and the panel rows of X go to LDS; wavefront 0 factors the panel (lane = row,
+// pivots and column entries broadcast with readlane) and applies it to the X rows and to e (lane = column); then
+// every thread applies the rank-4 update to its cells of the trailing block, of X and of e. LDS buffers
+// alternate between panels, so two barriers per panel suffice. Rows and columns past the matrix are identity.
 __global__ __launch_bounds__(256) void k_potrf(Batch b, int w, int k) {
-  constexpr int kP = 4;
-  __shared__ double sL[kLB * kLds];
-  __shared__ double sCol[kP * kLB];
-  __shared__ int sOk;
+  constexpr int kP = 4, kT = kLB / 4;  // panel width (= wavefronts per workgroup), rows per thread
+  __shared__ double sP[2][kLB][kP];    // panel columns, [row][q]
+  __shared__ double sX[2][kP][kLB];    // panel rows of X, [q][column]
+  __shared__ double sE[kLB];           // e_k, overwritten with y_k
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
   const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x, lane = tid & 63;
-  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {  // coalesced
-    const int r = idx / kLB, c = idx % kLB;
-    double v = 0.0;
-    if (r < kb && c < kb) {
-      if (c <= r) v = L.A[static_cast<int64_t>(k0 + r) * n + k0 + c];
-    } else if (r == c) {
-      v = 1.0;  // identity padding beyond the matrix
-    }
-    sL[r * kLds + c] = v;
-  }
-  if (tid == 0) sOk = 1;
-  __syncthreads();
-  for (int c0 = 0; c0 < kLB; c0 += kP) {
-    if (tid < kLB) {
-      double a[kP];
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double a[kT], x[kT];
 #pragma unroll
-      for (int q = 0; q < kP; ++q) a[q] = sL[lane * kLds + c0 + q];
-      bool ok = true;
+  for (int t = 0; t < kT; ++t) {
+    const int r = 4 * t + wv;
+    double v = 0.0;
+    if (r < kb && lane < kb) {
+      if (lane <= r) v = L.A[static_cast<int64_t>(k0 + r) * n + k0 + lane];
+    } else if (r == lane) {
+      v = 1.0;
+    }
+    a[t] = v;
+    x[t] = r == lane ? 1.0 : 0.0;
+  }
+  if (tid < kLB) sE[tid] = tid < kb ? L.rhs[k0 + tid] : 0.0;
+  bool ok = true;
+  for (int p = 0; p < kLB / kP; ++p) {
+    const int c0 = kP * p, buf = p & 1;
+    if (lane >= c0 && lane < c0 + kP) {  // rows above the diagonal hold zeros
+#pragma unroll
+      for (int t = 0; t < kT; ++t) sP[buf][4 * t + wv][lane - c0] = a[t];
+    }
+    {
+      double v = x[0];  // X row c0 + wv is x[p] of wavefront wv
+#pragma unroll
+      for (int t = 1; t < kT; ++t) v = t == p ? x[t] : v;
+      sX[buf][wv][lane] = v;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      double c[kP], rl[kP], lpp[kP][kP];
+#pragma unroll
+      for (int q = 0; q < kP; ++q) c[q] = lane >= c0 ? sP[buf][lane][q] : 0.0;
 #pragma unroll
       for (int q = 0; q < kP; ++q) {
-        const int c = c0 + q;
-        const double d = readlane_d(a[q], c);
+        const int cq = c0 + q;
+        const double d = readlane_d(c[q], cq);
         if (!(d > 0.0)) ok = false;
         const double l = sqrt(fmax(d, 1e-300));
-        if (lane == c) a[q] = l;
-        else if (lane > c) a[q] = a[q] / l;
+        rl[q] = 1.0 / l;
+        c[q] = lane == cq ? l : (lane > cq ? c[q] * rl[q] : 0.0);
 #pragma unroll
         for (int q2 = q + 1; q2 < kP; ++q2) {
-          const double l2 = readlane_d(a[q], c0 + q2);  // L[c0 + q2][c]
-          if (lane >= c0 + q2) a[q2] -= a[q] * l2;
+          lpp[q2][q] = readlane_d(c[q], c0 + q2);  // L[c0 + q2][c0 + q]
+          if (lane >= c0 + q2) c[q2] -= c[q] * lpp[q2][q];
         }
       }
 #pragma unroll
+      for (int q = 0; q < kP; ++q) sP[buf][lane][q] = c[q];  // zero above the diagonal
+      double xr[kP], y[kP];
+#pragma unroll
       for (int q = 0; q < kP; ++q) {
-        if (lane >= c0 + q) sL[lane * kLds + c0 + q] = a[q];
-        sCol[q * kLB + lane] = lane >= c0 + kP ? a[q] : 0.0;
+        double v = sX[buf][q][lane], u = sE[c0 + q];
+#pragma unroll
+        for (int q2 = 0; q2 < q; ++q2) {
+          v -= lpp[q][q2] * xr[q2];
+          u -= lpp[q][q2] * y[q2];
+        }
+        xr[q] = v * rl[q];
+        y[q] = u * rl[q];
+        sX[buf][q][lane] = xr[q];
       }
-      if (!ok && lane == 0) sOk = 0;
+      // e_r -= L[r][panel] y[panel] for the rows below the panel (lane = row)
+      double de = 0.0;
+#pragma unroll
+      for (int q = 0; q < kP; ++q) de += c[q] * y[q];
+      if (lane >= c0 + kP) sE[lane] -= de;
+      else if (lane >= c0) sE[lane] = lane == c0 ? y[0] : lane == c0 + 1 ? y[1] : lane == c0 + 2 ? y[2] : y[3];
     }
     __syncthreads();
-    double upd[16];
+    // all loads first: this thread's column of the panel, X's panel rows, and the panel entries of its rows
+    double lc[kP], xr[kP], lr[kT][kP];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int cell = tid + 256 * t, r = cell >> 6, j = cell & 63;
-      double v = 0.0;
-      if (j >= c0 + kP && j <= r) {
-        v = sL[r * kLds + j];
+    for (int q = 0; q < kP; ++q) {
+      lc[q] = sP[buf][lane][q];
+      xr[q] = sX[buf][q][lane];
+    }
 #pragma unroll
-        for (int q = 0; q < kP; ++q) v -= sCol[q * kLB + r] * sCol[q * kLB + j];
+    for (int t = 0; t < kT; ++t)
+#pragma unroll
+      for (int q = 0; q < kP; ++q) lr[t][q] = sP[buf][4 * t + wv][q];
+    // Branch-free over the rows: panel entries of rows above the panel are zero, so those rows do not change.
+    const bool below = lane >= c0 + kP, in_panel = lane >= c0 && !below;
+    const int qp = lane - c0;
+    const double xown = wv == 0 ? xr[0] : wv == 1 ? xr[1] : wv == 2 ? xr[2] : xr[3];
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+      const int r = 4 * t + wv;
+      double da = 0.0, dx = 0.0;
+#pragma unroll
+      for (int q = 0; q < kP; ++q) {
+        da += lr[t][q] * lc[q];
+        dx += lr[t][q] * xr[q];
       }
-      upd[t] = v;
+      const double lsel = sP[buf][r][qp & (kP - 1)];  // L[r][lane] for the panel's lanes
+      if (lane <= r) a[t] = below ? a[t] - da : (in_panel ? lsel : a[t]);
+      x[t] = (r >= c0 && r < c0 + kP) ? xown : x[t] - dx;  // row c0 + wv of X is final
     }
+  }
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int cell = tid + 256 * t, r = cell >> 6, j = cell & 63;
-      if (j >= c0 + kP && j <= r) sL[r * kLds + j] = upd[t];
-    }
-    __syncthreads();
+  for (int t = 0; t < kT; ++t) {
+    const int r = 4 * t + wv;
+    if (r < kb && lane <= r) L.A[static_cast<int64_t>(k0 + r) * n + k0 + lane] = a[t];
+    L.Linv[static_cast<int64_t>(k) * kLB * kLB + r * kLB + lane] = x[t];
   }
-  const bool ok = sOk != 0;
-  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx / kb, c = idx % kb;
-    if (c <= r) L.A[static_cast<int64_t>(k0 + r) * n + k0 + c] = sL[r * kLds + c];
-  }
-  __syncthreads();  // sL is overwritten with X below
-  if (tid >= kLB) {
-    if (tid == kLB && !ok) St->chol_ok = 0;
-    return;
-  }
-  // X = L^-1, column `lane`: X[i][j] = (delta_ij - sum_{m=j}^{i-1} L[i][m] X[m][j]) / L[i][i]
-  double x[kLB];
-#pragma unroll
-  for (int i = 0; i < kLB; ++i) {
-    double v = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-    for (int m = 0; m < i; ++m) v -= sL[i * kLds + m] * x[m];
-    x[i] = (i >= lane) ? v / sL[i * kLds + i] : 0.0;
-  }
-  // keep X (global, for the panel and the backward solve) and form y_k = X e_k through LDS
-  __shared__ double sE[kLB];
-  double* X = L.Linv + static_cast<int64_t>(k) * kLB * kLB;
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // wavefront 0 alone from here: its LDS reads are done
-#pragma unroll
-  for (int i = 0; i < kLB; ++i) {
-    X[i * kLB + lane] = x[i];
-    sL[i * kLds + lane] = x[i];
-  }
-  sE[lane] = lane < kb ? L.rhs[k0 + lane] : 0.0;
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  double y = 0.0;
-  for (int m = 0; m <= lane; ++m) y += sL[lane * kLds + m] * sE[m];
-  if (lane < kb) L.rhs[k0 + lane] = y;
-  if (lane == 0 && !ok) St->chol_ok = 0;
+  __syncthreads();
+  if (tid < kb) L.rhs[k0 + tid] = sE[tid];
+  if (tid == 0 && !ok) St->chol_ok = 0;
 }
 
 // Panel below diagonal block k: L_ik = A_ik L_kk^-T = A_ik X^T with the kept inverse X, one block row per

@@ -218,10 +218,11 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pb, pb + pairs, {0, 0, 0}});
     }
   };
-  const size_t first_tile = H.tiles.size();
+  const size_t first_tile = H.tiles.size(), first_btile = H.btiles.size();
   make_tiles(H.tiles, kTilePoints);
   make_tiles(H.btiles, kBlock);
-  // pair records: observation range, key frame, point index inside its Schur tile; first observation by pair
+  // pair records: observation range, key frame | point index inside its back-substitution tile << 8, point
+  // index inside its Schur tile | local key frame << 8; first observation by pair
   const size_t first_pair = H.pair_rec.size();
   H.pair_rec.resize(H.pair_pose.size());
   for (size_t t = first_tile; t < H.tiles.size(); ++t) {
@@ -236,6 +237,13 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
         const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
         H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - T.point_begin) | (jl << 8));
       }
+    }
+  }
+  for (size_t t = first_btile; t < H.btiles.size(); ++t) {
+    const Tile& T = H.btiles[t];
+    for (int i = T.point_begin; i < T.point_end; ++i) {
+      const int e = i + 1 < static_cast<int>(H.point_pair.size()) ? H.point_pair[i + 1] : static_cast<int>(H.pair_pose.size());
+      for (int q = H.point_pair[i]; q < e; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - T.point_begin) << 8);
     }
   }
   for (size_t q = first_pair; q < H.pair_rec.size(); ++q) {
