@@ -98,13 +98,15 @@ struct Batch {
   const Chunk* chunks;
   const Tile* btiles;    // back-substitution tiles (<= kTilePairs pairs, <= kBlock points)
   const Chunk* bchunks;
-  int32_t n_bchunks, pad1;
+  int32_t n_bchunks, red_blocks;  // red_blocks: workgroups of the record reduction
+  const int32_t* red_first;       // per window (+1): its first reduction workgroup
   const float2* obs_xy;
   const float4* obs_info;
   const int32_t* pair_obs;
   const int32_t* pair_pose;
-  const int4* pair_rec;      // per pair: (first observation, end of its observations, key frame, point index
-                             // inside its Schur tile | local free key frame index << 8, 255 when fixed)
+  const int4* pair_rec;      // per pair: (first observation, end of its observations, key frame | point index
+                             // inside its back-substitution tile << 8, point index inside its Schur tile |
+                             // local free key frame index << 8, 255 when fixed)
   const float2* pair_xy;     // per pair: its first observation (the obs streams themselves when every pair has
   const float4* pair_info;   // exactly one observation)
   const int32_t* point_pair;

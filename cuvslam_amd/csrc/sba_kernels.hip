@@ -660,11 +660,18 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
 // few chunks: one entry per thread over all its chunks.
 __global__ __launch_bounds__(256) void k_reduce(Batch b) {
   __shared__ float sq[3][64];
-  const int w = blockIdx.y;
+  // this workgroup's window: the last w with red_first[w] <= blockIdx.x
+  int lo = 0, hi = b.n_windows - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.red_first[mid] <= static_cast<int>(blockIdx.x)) lo = mid;
+    else hi = mid - 1;
+  }
+  const int w = lo, blk = blockIdx.x - b.red_first[w];
   const WindowDesc W = b.win[w];
   if (b.st[w].done || W.mapped) return;
-  if (W.nchunks < 16) {
-    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (W.nchunks < 64) {
+    const int64_t e = static_cast<int64_t>(blk) * 256 + threadIdx.x;
     if (e >= W.rec) return;
     const float* src = b.part + W.part_off + e;
     float a0 = 0.f, a1 = 0.f;
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(256) void k_reduce(Batch b) {
     return;
   }
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+  const int64_t e = static_cast<int64_t>(blk) * 64 + lane;
   const int64_t rec = W.rec;
   const int c0 = (W.nchunks * q) / 4, c1 = (W.nchunks * (q + 1)) / 4;
   float s = 0.f;
@@ -1197,6 +1204,10 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
     __syncthreads();
     if (tid < npts) sAcc[4 * tid] = sAcc[4 * tid + 1] = sAcc[4 * tid + 2] = sAcc[4 * tid + 3] = 0.f;
     __syncthreads();
+    // the first observation's J_c dc_j, J_p and weight, kept from (a) for the predicted reduction in (c):
+    // kept 0 = not linearized in (a), 1 = kept, 2 = behind the camera
+    int kept = 0;
+    float ku0 = 0.f, ku1 = 0.f, kw = 0.f, kjp[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     if (!INITIAL) {
       // (a) sum_j W_ij^T dc_j = sum_obs w J_p^T Lambda J_c dc_j
       if (tid < npair) {
@@ -1212,13 +1223,26 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
             const float4 inf = o == rec.x ? inf0 : b.obs_info[o];
             Lin L;
             if (!linearize(sPose + 12 * j, sCam + 12 * __float_as_int(inf.w), p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y,
-                           inf.z, kind, delta, L))
+                           inf.z, kind, delta, L)) {
+              if (o == rec.x) kept = 2;
               continue;
+            }
             float u0 = 0.f, u1 = 0.f;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
               u0 += L.jc[0][k] * d[k];
               u1 += L.jc[1][k] * d[k];
+            }
+            if (o == rec.x) {
+              kept = 1;
+              ku0 = u0;
+              ku1 = u1;
+              kw = L.w;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                kjp[0][k] = L.jp[0][k];
+                kjp[1][k] = L.jp[1][k];
+              }
             }
             const float lu0 = L.w * (inf.x * u0 + inf.y * u1), lu1 = L.w * (inf.y * u0 + inf.z * u1);
 #pragma unroll
@@ -1289,7 +1313,17 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
         } else {
           ++skipped;
         }
-        if (!INITIAL) {
+        if (!INITIAL && o == rec.x && kept != 0) {
+          if (kept == 1) {
+            float g0 = ku0, g1 = ku1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              g0 += kjp[0][k] * dp[k];
+              g1 += kjp[1][k] * dp[k];
+            }
+            hess += static_cast<double>(kw * (g0 * (inf.x * g0 + inf.y * g1) + g1 * (inf.y * g0 + inf.z * g1)));
+          }
+        } else if (!INITIAL) {
           Lin L;
           if (linearize(sPose + 12 * j, C, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L)) {
             float g0 = 0.f, g1 = 0.f;
@@ -1468,8 +1502,7 @@ hipError_t launch_schur(const Batch& b, hipStream_t s) {
 }
 
 hipError_t launch_reduce(const Batch& b, hipStream_t s) {
-  const int gx = (b.max_rec + 63) / 64;
-  hipLaunchKernelGGL(k_reduce, dim3(gx > 0 ? gx : 1, b.n_windows), dim3(256), 0, s, b);
+  if (b.red_blocks > 0) hipLaunchKernelGGL(k_reduce, dim3(b.red_blocks), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 

@@ -521,8 +521,17 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
   }
 
+  // record reduction work list: 64 entries per workgroup for windows of many chunks, 256 otherwise
+  std::vector<int32_t> red_first(n + 1, 0);
+  for (int w = 0; w < n; ++w) {
+    const WindowDesc& W = b->win[w];
+    const int per = W.nchunks < 64 ? 256 : 64;
+    red_first[w + 1] = red_first[w] + (W.mapped ? 0 : (W.rec + per - 1) / per);
+  }
+
   Batch& d = b->dev;
   d.n_windows = n;
+  d.red_blocks = red_first[n];
   d.n_chunks = static_cast<int32_t>(chunks.size());
   d.n_bchunks = static_cast<int32_t>(bchunks.size());
   d.n_points = static_cast<int32_t>(H.pts.size());
@@ -543,6 +552,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   UP(chunk_map, chunk_map);
   UP(H.btiles, btiles);
   UP(bchunks, bchunks);
+  UP(red_first, red_first);
   UP(H.obs_xy, obs_xy);
   UP(H.obs_info, obs_info);
   UP(H.pair_obs, pair_obs);
