@@ -87,7 +87,8 @@ typedef struct sba_options {
   int32_t fixed_iterations;       /* 1: run exactly max_iterations LM iterations (the step-size
                                      convergence exit is off; the zero-cost and infeasible guards stay) */
   int32_t use_graph;              /* 1: capture the LM iterations of a batch in one hipGraph */
-  int32_t chunk_pairs;            /* work per Schur workgroup in (point, key frame) pairs; 0 = autotuned */
+  int32_t chunk_pairs;            /* work per Schur workgroup in (point, key frame) pairs; 0 = built-in
+                                     heuristic (sba_autotune_chunks picks a value by timing) */
   int32_t reserved[7];
 } sba_options;
 
