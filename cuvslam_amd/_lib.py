@@ -83,6 +83,8 @@ SIGNATURES = {
     "sba_batch_info": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip, _ip]),
     "sba_batch_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, _fp, _ip]),
     "sba_batch_debug_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+    "sba_batch_winsorize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint8),
+                                      C.POINTER(C.c_int32)]),
     "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
     "sba_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
     "sba_autotune_chunks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32, _ip, _fp]),

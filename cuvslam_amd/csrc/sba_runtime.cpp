@@ -36,6 +36,9 @@ hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_large_zero(const Batch& b, int w, hipStream_t s);
 hipError_t launch_large_scatter(const Batch& b, int w, int nchunks, hipStream_t s);
 hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s);
+hipError_t launch_winsorize_select(const Batch& b, int w, int np, int kf, double* err, int32_t* pt_of,
+                                   int32_t* count, uint8_t* mark, int32_t* out, hipStream_t s);
+hipError_t launch_winsorize_apply(const Batch& b, int w, int np, int nc, const uint8_t* mark, hipStream_t s);
 hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_gather(const Batch& b, hipStream_t s);
 hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s);
@@ -738,6 +741,41 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
     if (launches_per_class) launches_per_class[k] = b->class_launches[k];
   }
   return SBA_OK;
+}
+
+int sba_batch_winsorize(sba_handle* h, sba_batch* b, int32_t window, int32_t key_frame, uint8_t* reset_mask,
+                        int32_t* n_reset) {
+  if (!h || !b || window < 0 || window >= b->n) return fail(h, SBA_E_INVALID, "bad sba_batch_winsorize arguments");
+  const WindowDesc& W = b->win[window];
+  if (key_frame < 0 || key_frame >= W.nc) return fail(h, SBA_E_INVALID, "key frame out of range");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  hipStream_t s = h->stream;
+  const size_t no = static_cast<size_t>(std::max(W.no, 1)), np = static_cast<size_t>(std::max(W.np, 1));
+  // workspace: squared errors (FP64) and point index per observation, counters, marks per point
+  const size_t off_pt = 8 * no, off_cnt = (off_pt + 4 * no + 255) & ~size_t(255), off_mark = off_cnt + 256;
+  char* ws = nullptr;
+  HIP_TRY(hipMalloc(&ws, off_mark + np));
+  auto done = [&](int rc) {
+    (void)hipFree(ws);
+    return rc;
+  };
+  int32_t out[4] = {0, 0, 0, 0};
+  int32_t* cnt = reinterpret_cast<int32_t*>(ws + off_cnt);
+  uint8_t* mark = reinterpret_cast<uint8_t*>(ws + off_mark);
+  if (hipMemsetAsync(ws + off_cnt, 0, 256, s) != hipSuccess || hipMemsetAsync(mark, 0, np, s) != hipSuccess ||
+      launch_winsorize_select(b->dev, window, W.np, key_frame, reinterpret_cast<double*>(ws),
+                              reinterpret_cast<int32_t*>(ws + off_pt), cnt, mark, cnt + 4, s) != hipSuccess ||
+      hipMemcpyAsync(out, cnt + 4, sizeof(out), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(fail(h, SBA_E_DEVICE, "winsorize launch failed"));
+  if (out[3]) return done(fail(h, SBA_E_CAPACITY, "more than 1024 observations share the winsorizing cut"));
+  if (launch_winsorize_apply(b->dev, window, W.np, W.nc, mark, s) != hipSuccess ||
+      (reset_mask && W.np > 0 && hipMemcpyAsync(reset_mask, mark, W.np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(fail(h, SBA_E_DEVICE, "winsorize apply failed"));
+  b->adj_ready = false;
+  if (n_reset) *n_reset = out[0];
+  return done(SBA_OK);
 }
 
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc, double* dp,

@@ -268,6 +268,20 @@ class DeviceBatch:
             raise SbaError(rc, self.solver.last_error())
         return S, e, dc, dp, sc, rc
 
+    def winsorize(self, key_frame: int, window: int = 0):
+        """Landmark winsorizing of a solved window (pipelines/winsorizer.cpp:23-73): resets the points whose
+        observations from ``key_frame`` are among the 5% largest reprojection errors (above 0.0025^2, at least 50
+        observations). The next ``run`` continues from this solve without them. Returns (reset mask (N_p,) bool,
+        number reset)."""
+        p = self.problems[window]
+        mask = np.zeros(p.points.shape[0], np.uint8)
+        n = C.c_int32()
+        rc = self.solver._L.sba_batch_winsorize(self.solver._h, self._b, window, key_frame,
+                                                mask.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(n))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.solver.last_error())
+        return mask.astype(bool), n.value
+
     def refine(self, newton_steps: int = 3, max_cg: int = 200, tol: float = 1e-12, gauss_newton_steps: int = 4):
         """FP64 refinement of the solved state of window 0 (the state ``adjoint`` then differentiates at):
         ``gauss_newton_steps`` steps with the factored Gauss-Newton system, then ``newton_steps`` full-Hessian

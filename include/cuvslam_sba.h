@@ -135,6 +135,19 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
 int sba_autotune_chunks(sba_handle* h, const sba_problem_view* problems, int32_t n, const int32_t* candidates,
                         int32_t n_candidates, int32_t repeats, int32_t* best_chunk_pairs, float* ms_per_candidate);
 
+/* Landmark winsorizing between two solves (replaces cuvslam::pipelines::winsorize, pipelines/winsorizer.cpp:23-73,
+ * as called between the two bundler runs of service_sba.h:317-339), on the solved state of `window` of a batch
+ * (run sba_batch_run first). The squared reprojection errors (normalized coordinates) of the observations made
+ * from key frame `key_frame` are taken, skipping points inside the near plane; with at least 50 of them, the
+ * int(5 n / 100) largest reset their points when above 0.0025^2 (ties at the cut by decreasing point index; the
+ * reference orders ties by landmark address). Reset points' observations then carry zero information in the
+ * batch, and the window's solved state becomes its initial state, so the next sba_batch_run is the reference's
+ * second run: it continues from the first solve without the reset landmarks.
+ *   reset_mask [optional, num_points of the window] 1 for reset points; n_reset [optional] their number.
+ * SBA_E_CAPACITY when more than 1024 observations share the cut error (nothing is changed then). */
+int sba_batch_winsorize(sba_handle* h, sba_batch* b, int32_t window, int32_t key_frame, uint8_t* reset_mask,
+                        int32_t* n_reset);
+
 /* One LM step of window 0 of a batch from its current state with damping lambda, for phase-level parity
  * tests (the reference tests its phases the same way: libs/cuda_modules/test/sba_test.cpp).
  * Outputs (any may be null):

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -1148,6 +1149,47 @@ void orc_grad_f64(const OrcProblem* p, const OrcOptions* o, const double* sp, co
 
 double orc_loss_f64(int32_t kind, double s, double d, int32_t derivative) {
   return derivative ? loss_drho<double>(kind, s, d) : loss_rho<double>(kind, s, d);
+}
+
+// Landmark winsorizing (pipelines/winsorizer.cpp:23-73): squared reprojection errors, in normalized
+// coordinates, of the observations made from key frame kf at the state (points [np][3], poses [nc][16]
+// column-major; the problem's own when null), skipping points inside the near plane (IsDepthInFront,
+// epipolar/near_plane.h:38). With at least 50 of them (winsorizer.cpp:62-64), the int(5 n / 100) largest of
+// the sorted (error, point, observation) triples reset their points when above 0.0025^2
+// (winsorizer.cpp:66-73; the reference orders equal errors by landmark address). reset [np] 0/1; returns the
+// number of reset points.
+int orc_winsorize_f64(const OrcProblem* p, int32_t kf, const double* pts, const double* poses, uint8_t* reset) {
+  std::fill(reset, reset + p->num_points, uint8_t(0));
+  double M[16];
+  for (int k = 0; k < 16; ++k) M[k] = poses ? poses[16 * kf + k] : static_cast<double>(p->rig_from_world[16 * kf + k]);
+  std::vector<std::tuple<double, int, int>> errs;
+  for (int o = 0; o < p->num_observations; ++o) {
+    if (p->pose_ids[o] != kf) continue;
+    const int i = p->point_ids[o];
+    double x[3];
+    for (int a = 0; a < 3; ++a) x[a] = pts ? pts[3 * i + a] : static_cast<double>(p->points[3 * i + a]);
+    double r[3], c[3];
+    for (int a = 0; a < 3; ++a) r[a] = M[a] * x[0] + M[4 + a] * x[1] + M[8 + a] * x[2] + M[12 + a];
+    const float* Cm = p->camera_from_rig + 16 * p->camera_ids[o];
+    for (int a = 0; a < 3; ++a)
+      c[a] = static_cast<double>(Cm[a]) * r[0] + static_cast<double>(Cm[4 + a]) * r[1] +
+             static_cast<double>(Cm[8 + a]) * r[2] + static_cast<double>(Cm[12 + a]);
+    if (!(c[2] > static_cast<double>(kHither))) continue;
+    const double u = c[0] / c[2] - p->observation_xys[2 * o], v = c[1] / c[2] - p->observation_xys[2 * o + 1];
+    errs.emplace_back(u * u + v * v, i, o);
+  }
+  const int n = static_cast<int>(errs.size());
+  if (n < 50) return 0;
+  std::sort(errs.begin(), errs.end());
+  const int cut = static_cast<int>(5.f * static_cast<float>(n) / 100.f);
+  const double min2 = static_cast<double>(0.0025f * 0.0025f);
+  for (int j = 0; j < cut; ++j) {
+    const auto& t = errs[n - 1 - j];
+    if (std::get<0>(t) > min2) reset[std::get<1>(t)] = 1;
+  }
+  int count = 0;
+  for (int i = 0; i < p->num_points; ++i) count += reset[i];
+  return count;
 }
 
 }  // extern "C"

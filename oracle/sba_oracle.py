@@ -69,6 +69,7 @@ def lib():
         L.orc_grad_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp, dp]
         L.orc_grad_f64.restype = None
         L.orc_solve_state_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp, C.POINTER(OrcResult)]
+        L.orc_winsorize_f64.argtypes = [PV, C.c_int32, dp, dp, C.POINTER(C.c_uint8)]
         L.orc_solve_state_f64.restype = C.c_int
         L.orc_loss_f64.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32]
         L.orc_loss_f64.restype = C.c_double
@@ -252,3 +253,13 @@ def solve_state(problem, points, poses, xy=None, info=None, trace=0, **opt):
     rc = L.orc_solve_state_f64(C.byref(v), C.byref(o), sp, sq, ox, oi, C.byref(r))
     return dict(status=rc, points=pts, poses=pq.reshape(n_c, 4, 4).transpose(0, 2, 1), iterations=r.iterations,
                 last_cost=r.last_cost)
+
+
+def winsorize(problem, key_frame, points=None, poses=None):
+    """Landmark winsorizing at a state (points (N,3), poses (N,4,4); the problem's own when None): (reset mask
+    (N_p,) bool, number reset)."""
+    v = problem.view()
+    sp, sq, keep = _state(points, poses)
+    mask = np.zeros(problem.points.shape[0], np.uint8)
+    n = lib().orc_winsorize_f64(C.byref(v), key_frame, sp, sq, mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return mask.astype(bool), n

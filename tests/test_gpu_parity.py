@@ -202,3 +202,39 @@ def test_large_window_in_a_batch_with_small_windows():
         alone = p.copy()
         assert s.solve(alone)
         assert abs(alone.last_cost_f64 - t.last_cost_f64) <= 1e-6 * alone.last_cost_f64
+
+
+def test_winsorize_matches_oracle_and_the_second_run_continues_without_the_reset_points():
+    # between the two bundler runs (service_sba.h:317-339): winsorize the solved window, then solve again
+    p, _, _ = small_scene(seed=12, n_points=1500, n_kf=8, max_iterations=8, outlier_rate=0.04)
+    s = ca.SchurComplementBundlerGpu(loss=HUBER, fixed_iterations=True)
+    b = s.batch([p])
+    b.run()
+    q = p.copy()
+    b.read([q])
+    state = (q.points.astype(np.float64), q.rig_from_world.astype(np.float64))
+    want0, n0 = O.winsorize(p, 0, *state)
+    mask0, m0 = b.winsorize(0)
+    assert m0 == n0 > 0
+    np.testing.assert_array_equal(mask0, want0)
+    # the second run starts from the first solve's state, without the reset points' observations
+    r = q.copy()
+    r.observation_infos[np.isin(r.point_ids, np.nonzero(mask0)[0])] = 0
+    ref = O.solve(r, "f64", loss=HUBER, fixed_iterations=1)
+    b.run()
+    t = p.copy()
+    b.read([t])
+    assert t.iterations == ref["iterations"]
+    assert abs(t.initial_cost_f64 - ref["initial_cost"]) / ref["initial_cost"] < 1e-7
+    assert abs(t.last_cost_f64 - ref["last_cost"]) / ref["last_cost"] < 1e-6
+
+
+def test_winsorize_needs_fifty_observations_and_checks_arguments(solver_huber):
+    p, _, _ = small_scene(seed=13, n_points=40, n_kf=4, max_iterations=3)
+    b = solver_huber.batch([p])
+    b.run()
+    mask, n = b.winsorize(0)
+    assert n == 0 and not mask.any()
+    with pytest.raises(ca.SbaError):
+        b.winsorize(4)
+    b.close()

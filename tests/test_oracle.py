@@ -352,3 +352,63 @@ def test_implicit_gradient_vs_finite_differences_of_converged_solves():
     assert rel(c.pick(duv), fd_uv) < 1e-4
     for k in range(4):
         assert abs(dK[k] - fd_K[k]) <= 1e-4 * abs(fd_K[k]), (k, dK[k], fd_K[k])
+
+
+# ---- landmark winsorizing (pipelines/winsorizer.cpp:23-73) ----
+
+def _reproj_sq(problem, kf):
+    """Squared reprojection errors of key frame kf's observations from plain matrix products: (error, point)."""
+    sel = np.nonzero(problem.pose_ids == kf)[0]
+    T = problem.rig_from_world[kf].astype(np.float64)
+    Cs = problem.camera_from_rig.astype(np.float64)
+    X = np.c_[problem.points[problem.point_ids[sel]].astype(np.float64), np.ones(sel.size)]
+    Pc = np.einsum("nij,nj->ni", Cs[problem.camera_ids[sel]], (T @ X.T).T)
+    keep = Pc[:, 2] > 0.1
+    proj = Pc[keep, :2] / Pc[keep, 2:3]
+    e = np.sum((proj - problem.observation_xys[sel[keep]].astype(np.float64)) ** 2, axis=1)
+    return e, problem.point_ids[sel[keep]], sel[keep]
+
+
+def test_winsorize_matches_a_sort_of_the_errors():
+    # the 5% largest of n errors, int(5 n / 100) of them, reset their points when above 0.0025^2
+    p, _, _ = ca.scene.generate(seed=3, num_points=1500, num_keyframes=8, num_fixed_keyframes=2)
+    for kf in range(3):
+        e, pt, ob = _reproj_sq(p, kf)
+        order = np.lexsort((ob, pt, e))  # ascending by error, then point, then observation
+        cut = int(np.float32(5.0) * np.float32(e.size) / np.float32(100.0))
+        top = order[e.size - cut:]
+        want = np.zeros(p.points.shape[0], bool)
+        want[pt[top[e[top] > np.float64(np.float32(0.0025) * np.float32(0.0025))]]] = True
+        mask, n = O.winsorize(p, kf)
+        assert e.size >= 50 and cut >= 1
+        np.testing.assert_array_equal(mask, want)
+        assert n == want.sum()
+
+
+def test_winsorize_keeps_exact_observations_and_catches_planted_outliers():
+    p, pts, poses = ca.scene.generate(seed=4, num_points=1200, num_keyframes=6, num_fixed_keyframes=2,
+                                      outlier_rate=0.0, pixel_noise=0.0)
+    # at the true state every error is ~0, below the minimum size: nothing is reset
+    mask, n = O.winsorize(p, 0, pts, poses)
+    assert n == 0 and not mask.any()
+    # plant exactly int(5 n / 100) gross errors among key frame 0's observations: exactly their points go
+    sel = np.nonzero(p.pose_ids == 0)[0]
+    e, _, _ = _reproj_sq(p, 0)
+    cut = int(np.float32(5.0) * np.float32(e.size) / np.float32(100.0))
+    first = {}
+    for o in sel:
+        first.setdefault(int(p.point_ids[o]), o)
+    planted = np.array(list(first.values())[:cut])
+    q = p.copy()
+    q.observation_xys[planted] += np.float32(0.05)
+    mask, n = O.winsorize(q, 0, pts, poses)
+    assert n == cut
+    np.testing.assert_array_equal(np.nonzero(mask)[0], np.sort(q.point_ids[planted]))
+
+
+def test_winsorize_needs_fifty_observations():
+    p, _, _ = ca.scene.generate(seed=5, num_points=40, num_keyframes=4, num_fixed_keyframes=2)
+    e, _, _ = _reproj_sq(p, 0)
+    assert e.size < 50
+    mask, n = O.winsorize(p, 0)
+    assert n == 0 and not mask.any()
