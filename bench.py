@@ -205,6 +205,20 @@ def main():
     extra["accepted"] = int(p0.accepted)
     extra["final_cost"] = p0.last_cost_f64
 
+    # end to end through the host boundary: host buffers in (sort, pack, upload), solve, results back to the host
+    # (PCIe-inclusive; not the headline value, whose inputs are already resident in HBM)
+    reps = 3 if args.workload != "c3" else 1
+    host_sets = [[q.copy() for q in pristine] for _ in range(reps)]
+    t0 = time.perf_counter()
+    for hosts in host_sets:
+        if len(hosts) > 1:
+            solver.solve_batch(hosts)
+        else:
+            solver.solve(hosts[0])
+    del host_sets
+    extra["end_to_end_ms_per_batch_host_buffers"] = round(1e3 * (time.perf_counter() - t0) / reps, 3)
+    log(f"[bench] end to end from host buffers: {extra['end_to_end_ms_per_batch_host_buffers']} ms per batch")
+
     if args.workload == "c3" and not args.no_plain:
         # the plain form: the same windows solved one window per launch sequence, one after another
         plain = [solver.batch([p]) for p in pristine]

@@ -8,9 +8,10 @@
 //                 the additions, and so the last bits of the sum, is not fixed for these windows)
 //   k_lassemble   A = U + lambda diag(U) - sum Z Z^T in FP64 (dense, row-major lower triangle), rhs e
 //   k_potrf / k_trsm / k_syrk   right-looking blocked Cholesky (Golub & Van Loan, Matrix Computations,
-//                 Algorithm 4.2.10) with 64 x 64 blocks: the diagonal block is factored in one wavefront with
-//                 its rows in registers, and its triangular inverse kept; the panel below it is then a product
-//                 with that inverse (one block per workgroup), the trailing update one block pair per workgroup.
+//                 Algorithm 4.2.10) with 64 x 64 blocks: the diagonal block is factored with the 16 x 16 FP64
+//                 tile scheme of the small-window factor (matrix cores for the tile updates) and its triangular
+//                 inverse kept; the panel below it is then a product with that inverse (one block per
+//                 workgroup), the trailing update one block pair per workgroup.
 //                 The forward substitution L y = e rides along: y_k = L_kk^-1 e_k in k_potrf, e_i -= L_ik y_k
 //                 in k_trsm
 //   k_bsolve      L^T x = y, one launch per 64-row block with the kept inverses
@@ -23,8 +24,10 @@
 #include <stdint.h>
 
 #include "sba_device.h"
+#include "sba_tiles.h"
 
 using namespace sba_dev;
+using namespace sba_tiles;
 
 namespace {
 
@@ -44,12 +47,6 @@ __device__ __forceinline__ Large large_of(const Batch& b, const WindowDesc& W) {
   double* A = b.A + W.a_off;
   const int64_t nn = static_cast<int64_t>(n) * n;
   return Large{A, A + nn, A + nn + n, A + nn + 2 * n, n};
-}
-
-__device__ __forceinline__ double readlane_d(double x, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
-  return __hiloint2double(hi, lo);
 }
 
 }  // namespace
@@ -141,131 +138,199 @@ __global__ __launch_bounds__(256) void k_lassemble(Batch b, int w) {
   if (blockIdx.x == 0 && threadIdx.x == 0) St->chol_ok = 1;
 }
 
-// Cholesky of diagonal block k by one workgroup of 256 threads, with its inverse X = L_kk^-1 and the forward-
-// substitution step y_k = X e_k formed alongside (Golub & Van Loan 4.2 and 3.1, column-oriented). Thread
-// (wavefront v, lane c) keeps rows r = 4t + v (t < 16) of column c of both the block and X in registers. Per panel
-// of 4 columns: the panel columns and the panel rows of X go to LDS; wavefront 0 factors the panel (lane = row,
-// pivots and column entries broadcast with readlane) and applies it to the X rows and to e (lane = column); then
-// every thread applies the rank-4 update to its cells of the trailing block, of X and of e. LDS buffers
-// alternate between panels, so two barriers per panel suffice. Rows and columns past the matrix are identity.
+// Cholesky of diagonal block k (64 x 64; rows and columns past the matrix are identity) by one workgroup of 256
+// threads, as four 16 x 16 tile columns: wavefront 0 factors the diagonal tiles (kept in LDS) one after another
+// (factor_tile), wavefronts 1..3 hold the six off-diagonal tiles U_JI (J < I, A = U^T U) in registers and apply
+// the panel step U_KI = X_KK A_KI and the trailing updates on the FP64 matrix cores; the forward-substitution
+// step y_k = L_kk^-1 e_k runs alongside. Then the block's inverse X = L_kk^-1 (for the panel GEMMs and the
+// backward solve), tile by tile: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ, one tile column J per wavefront 1..3.
 __global__ __launch_bounds__(256) void k_potrf(Batch b, int w, int k) {
-  constexpr int kP = 4, kT = kLB / 4;  // panel width (= wavefronts per workgroup), rows per thread
-  __shared__ double sP[2][kLB][kP];    // panel columns, [row][q]
-  __shared__ double sX[2][kP][kLB];    // panel rows of X, [q][column]
-  __shared__ double sE[kLB];           // e_k, overwritten with y_k
+  constexpr int T = kLB / 16;       // 4 tile columns
+  __shared__ double sDg[T * 256];   // diagonal tiles, row-major (U after factoring)
+  __shared__ double sXT[T * 256];   // X_KK^T per tile column (row-major)
+  __shared__ double sP[T * 256];    // panel tiles U_KI of the current tile column
+  __shared__ double sU[6 * 256];    // off-diagonal U tiles (row-major), by pair index
+  __shared__ double sX[6 * 256];    // off-diagonal X tiles X_IJ (row-major), by the pair index of (J, I)
+  __shared__ double sS[3 * 256];    // per-wavefront product scratch
+  __shared__ double sT[256];
+  __shared__ double vec[kLB], ys[kLB], idg[kLB];
+  __shared__ int flag;
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
-  const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x, lane = tid & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  double a[kT], x[kT];
+  const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x;
+  const int lane = tid & 63, li = lane >> 4, lj = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // pair index of tile (J, I), J < I: 0 (0,1) 1 (0,2) 2 (0,3) 3 (1,2) 4 (1,3) 5 (2,3)
+  auto pair_of = [](int J, int I) { return J == 0 ? I - 1 : (J == 1 ? I + 1 : 5); };
+  // element (4v + lane/16, lane%16) of upper tile (J, I) of the symmetric block, read from the lower triangle
+  auto element = [&](int J, int I, int v) -> double {
+    const int r = 16 * J + 4 * v + li, c = 16 * I + lj;
+    if (r >= kb || c >= kb) return r == c ? 1.0 : 0.0;
+    return L.A[static_cast<int64_t>(k0 + max(r, c)) * n + k0 + min(r, c)];
+  };
+  {
+    double val[4];
 #pragma unroll
-  for (int t = 0; t < kT; ++t) {
-    const int r = 4 * t + wv;
-    double v = 0.0;
-    if (r < kb && lane < kb) {
-      if (lane <= r) v = L.A[static_cast<int64_t>(k0 + r) * n + k0 + lane];
-    } else if (r == lane) {
-      v = 1.0;
-    }
-    a[t] = v;
-    x[t] = r == lane ? 1.0 : 0.0;
+    for (int v = 0; v < 4; ++v) val[v] = element(wave, wave, v);  // diagonal tile `wave`
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sDg[256 * wave + 64 * v + lane] = val[v];
   }
-  if (tid < kLB) sE[tid] = tid < kb ? L.rhs[k0 + tid] : 0.0;
-  bool ok = true;
-  for (int p = 0; p < kLB / kP; ++p) {
-    const int c0 = kP * p, buf = p & 1;
-    if (lane >= c0 && lane < c0 + kP) {  // rows above the diagonal hold zeros
+  if (tid < kLB) vec[tid] = tid < kb ? L.rhs[k0 + tid] : 0.0;
+  if (tid == 0) flag = 0;
+
+  if (wave == 0) {
+    __syncthreads();
+    for (int K = -1; K < T; ++K) {
+      if (flag) break;
+      if (K >= 0) {
+        const double* xt = sXT + 256 * K;
+        double yk = 0.0;  // y_K = X_KK e_K
 #pragma unroll
-      for (int t = 0; t < kT; ++t) sP[buf][4 * t + wv][lane - c0] = a[t];
+        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
+        yk = sum_rows4(yk);
+        if (lane < 16) ys[16 * K + lane] = yk;
+        __syncthreads();
+      }
+      if (K + 1 < T) {
+        double* tk = sDg + 256 * (K + 1);
+        double4v dt;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dt[v] = tk[64 * v + lane];
+        if (K >= 0) dt = tile_update(dt, sP + 256 * (K + 1), sP + 256 * (K + 1), lane);
+        if (!factor_tile(dt, sT, sXT + 256 * (K + 1), idg + 16 * (K + 1), lane) && lane == 0) flag = 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tk[64 * v + lane] = dt[v];
+      }
+      __syncthreads();
     }
-    {
-      double v = x[0];  // X row c0 + wv is x[p] of wavefront wv
+    __syncthreads();  // the off-diagonal U tiles are published
+    __syncthreads();  // the off-diagonal X tiles are formed
+  } else {
+    // two register tiles per wavefront: pair index m = 3 s + wave - 1
+    int tJ[2], tI[2];
 #pragma unroll
-      for (int t = 1; t < kT; ++t) v = t == p ? x[t] : v;
-      sX[buf][wv][lane] = v;
+    for (int s = 0; s < 2; ++s) {
+      const int m = 3 * s + wave - 1;
+      tJ[s] = m < 3 ? 0 : (m < 5 ? 1 : 2);
+      tI[s] = m < 3 ? m + 1 : (m < 5 ? m - 1 : 3);
+    }
+    double4v acc[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[s][v] = element(tJ[s], tI[s], v);
+    __syncthreads();
+    for (int K = -1; K < T; ++K) {
+      if (flag) break;
+      if (K >= 0) {
+        // panel U_KI = X_KK A_KI (the B operand of chunk q is register q of the tile), e_I -= U_KI^T y_K
+        const double* xt = sXT + 256 * K;
+        double yk = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) yk += xt[16 * (4 * li + kk) + lj] * vec[16 * K + 4 * li + kk];
+        yk = sum_rows4(yk);
+        double yv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) yv[v] = __shfl(yk, 4 * v + li);  // y_K[4v + lane/16]
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (tJ[s] == K) {
+            const int I = tI[s];
+            double4v u = double4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) u = mfma64(xt[64 * q + lane], acc[s][q], u);
+            acc[s] = u;
+            double p = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              sP[256 * I + 64 * v + lane] = u[v];
+              p += u[v] * yv[v];
+            }
+            p = sum_rows4(p);
+            if (lane < 16) vec[16 * I + lane] -= p;
+          }
+        }
+        __syncthreads();
+        // trailing: the diagonal tiles past K + 1 (LDS), then this wavefront's tiles (J > K)
+        for (int J = K + 1 + wave; J < T; J += 3) {
+          double* tj = sDg + 256 * J;
+          double4v t;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) t[v] = tj[64 * v + lane];
+          t = tile_update(t, sP + 256 * J, sP + 256 * J, lane);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tj[64 * v + lane] = t[v];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          if (tJ[s] > K) acc[s] = tile_update(acc[s], sP + 256 * tJ[s], sP + 256 * tI[s], lane);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sU[256 * pair_of(tJ[s], tI[s]) + 64 * v + lane] = acc[s][v];
+    __syncthreads();
+    if (flag == 0) {
+      // tile column J = wave - 1 of X: X_IJ = -X_II S, S = sum_{K=J}^{I-1} L_IK X_KJ, L_IK = U_KI^T
+      const int J = wave - 1;
+      double* S = sS + 256 * J;
+      double4v xo[3];
+#pragma unroll
+      for (int I = 1; I < T; ++I) {
+        if (I <= J) continue;
+        double4v sa = double4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int K = 0; K < T - 1; ++K) {
+          if (K < J || K >= I) continue;
+          const double* U = sU + 256 * pair_of(K, I);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            // A operand L_IK[l%16][4q + l/16] = U_KI[4q + l/16][l%16]; B operand X_KJ[4q + l/16][l%16]: from
+            // X_JJ^T in LDS, or register q of the tile X_KJ formed above
+            const double xb = K == J ? sXT[256 * J + 16 * lj + 4 * q + li] : xo[K - J - 1][q];
+            sa = mfma64(U[64 * q + lane], xb, sa);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) S[64 * v + lane] = sa[v];
+        wave_lds_sync();
+        // A operand X_II[l%16][4q + l/16] = X_II^T[4q + l/16][l%16]
+        double4v d = double4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d = mfma64(-sXT[256 * I + 64 * q + lane], S[64 * q + lane], d);
+        xo[I - J - 1] = d;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sX[256 * pair_of(J, I) + 64 * v + lane] = d[v];
+        wave_lds_sync();  // S is rewritten for the next I
+      }
     }
     __syncthreads();
-    if (wv == 0) {
-      double c[kP], rl[kP], lpp[kP][kP];
-#pragma unroll
-      for (int q = 0; q < kP; ++q) c[q] = lane >= c0 ? sP[buf][lane][q] : 0.0;
-#pragma unroll
-      for (int q = 0; q < kP; ++q) {
-        const int cq = c0 + q;
-        const double d = readlane_d(c[q], cq);
-        if (!(d > 0.0)) ok = false;
-        const double l = sqrt(fmax(d, 1e-300));
-        rl[q] = 1.0 / l;
-        c[q] = lane == cq ? l : (lane > cq ? c[q] * rl[q] : 0.0);
-#pragma unroll
-        for (int q2 = q + 1; q2 < kP; ++q2) {
-          lpp[q2][q] = readlane_d(c[q], c0 + q2);  // L[c0 + q2][c0 + q]
-          if (lane >= c0 + q2) c[q2] -= c[q] * lpp[q2][q];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kP; ++q) sP[buf][lane][q] = c[q];  // zero above the diagonal
-      double xr[kP], y[kP];
-#pragma unroll
-      for (int q = 0; q < kP; ++q) {
-        double v = sX[buf][q][lane], u = sE[c0 + q];
-#pragma unroll
-        for (int q2 = 0; q2 < q; ++q2) {
-          v -= lpp[q][q2] * xr[q2];
-          u -= lpp[q][q2] * y[q2];
-        }
-        xr[q] = v * rl[q];
-        y[q] = u * rl[q];
-        sX[buf][q][lane] = xr[q];
-      }
-      // e_r -= L[r][panel] y[panel] for the rows below the panel (lane = row)
-      double de = 0.0;
-#pragma unroll
-      for (int q = 0; q < kP; ++q) de += c[q] * y[q];
-      if (lane >= c0 + kP) sE[lane] -= de;
-      else if (lane >= c0) sE[lane] = lane == c0 ? y[0] : lane == c0 + 1 ? y[1] : lane == c0 + 2 ? y[2] : y[3];
-    }
-    __syncthreads();
-    // all loads first: this thread's column of the panel, X's panel rows, and the panel entries of its rows
-    double lc[kP], xr[kP], lr[kT][kP];
-#pragma unroll
-    for (int q = 0; q < kP; ++q) {
-      lc[q] = sP[buf][lane][q];
-      xr[q] = sX[buf][q][lane];
-    }
-#pragma unroll
-    for (int t = 0; t < kT; ++t)
-#pragma unroll
-      for (int q = 0; q < kP; ++q) lr[t][q] = sP[buf][4 * t + wv][q];
-    // Branch-free over the rows: panel entries of rows above the panel are zero, so those rows do not change.
-    const bool below = lane >= c0 + kP, in_panel = lane >= c0 && !below;
-    const int qp = lane - c0;
-    const double xown = wv == 0 ? xr[0] : wv == 1 ? xr[1] : wv == 2 ? xr[2] : xr[3];
-#pragma unroll
-    for (int t = 0; t < kT; ++t) {
-      const int r = 4 * t + wv;
-      double da = 0.0, dx = 0.0;
-#pragma unroll
-      for (int q = 0; q < kP; ++q) {
-        da += lr[t][q] * lc[q];
-        dx += lr[t][q] * xr[q];
-      }
-      const double lsel = sP[buf][r][qp & (kP - 1)];  // L[r][lane] for the panel's lanes
-      if (lane <= r) a[t] = below ? a[t] - da : (in_panel ? lsel : a[t]);
-      x[t] = (r >= c0 && r < c0 + kP) ? xown : x[t] - dx;  // row c0 + wv of X is final
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < kT; ++t) {
-    const int r = 4 * t + wv;
-    if (r < kb && lane <= r) L.A[static_cast<int64_t>(k0 + r) * n + k0 + lane] = a[t];
-    L.Linv[static_cast<int64_t>(k) * kLB * kLB + r * kLB + lane] = x[t];
   }
   __syncthreads();
-  if (tid < kb) L.rhs[k0 + tid] = sE[tid];
-  if (tid == 0 && !ok) St->chol_ok = 0;
+  if (flag != 0) {
+    if (tid == 0) St->chol_ok = 0;
+    return;
+  }
+  // L (lower) back to A, X = L_kk^-1 (64 x 64 row-major, zero above the diagonal) to Linv, y_k to rhs.
+  // U tile (J, I) element (r, c) is L[16 I + c][16 J + r]; X_II = (X_II^T)^T from sXT.
+  double* Xg = L.Linv + static_cast<int64_t>(k) * kLB * kLB;
+  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {
+    const int R = idx / kLB, Cc = idx % kLB;
+    const int I = R >> 4, J = Cc >> 4, a = R & 15, c = Cc & 15;
+    double x = 0.0;
+    if (J == I) x = sXT[256 * I + 16 * c + a];
+    else if (J < I) x = sX[256 * pair_of(J, I) + 16 * a + c];
+    Xg[idx] = x;
+    if (Cc <= R && R < kb) {
+      const double l = J == I ? sDg[256 * I + 16 * c + a] : sU[256 * pair_of(J, I) + 16 * c + a];
+      L.A[static_cast<int64_t>(k0 + R) * n + k0 + Cc] = l;
+    }
+  }
+  if (tid < kb) L.rhs[k0 + tid] = ys[tid];
 }
 
 // Panel below diagonal block k: L_ik = A_ik L_kk^-T = A_ik X^T with the kept inverse X, one block row per
