@@ -14,7 +14,7 @@
 //                 workgroup), the trailing update one block pair per workgroup.
 //                 The forward substitution L y = e rides along: y_k = L_kk^-1 e_k in k_potrf, e_i -= L_ik y_k
 //                 in k_trsm
-//   k_bsolve      L^T x = y, one launch per 64-row block with the kept inverses
+//   k_bsolve      L^T x = y by 64-row blocks with the kept inverses, in one workgroup
 //   k_lepilogue   the pose step, its max-norm and dc^T diag(U) dc, trial key frames
 // A non-positive pivot clears the window's chol_ok; the later kernels then return, and the LM decision raises
 // lambda (schur_complement_bundler_gpu.cpp:175-185), as for the small windows.
@@ -431,35 +431,65 @@ __global__ __launch_bounds__(256) void k_syrk(Batch b, int w, int k) {
     }
 }
 
-// Backward substitution L^T x = y, one launch per 64-row block from the last (y from the factor kernels; rhs
-// holds w = y minus the products with the solved blocks). Launch for block bk, one workgroup per block j <= bk:
-// each computes x_bk = X_bk^T w_bk from the kept inverse (64 x 64, cheap to repeat); workgroup bk stores it and
-// every other workgroup j < bk subtracts L_bk,j^T x_bk from w_j. Block bk is final once its launch has run.
-__global__ __launch_bounds__(64) void k_bsolve(Batch b, int w, int bk) {
-  __shared__ double sw[kLB];
-  __shared__ double sx[kLB];
+// Backward substitution L^T x = y in one workgroup of 1024 threads, 64-row blocks from the last: for block bk,
+// r = y_bk - sum_{i past the block} L[i][block bk]^T x_i (16 groups of 64 threads, each group one row at a time
+// with its 64 lanes on the block's 64 contiguous columns), then x_bk = X_bk^T r with the kept inverse. x stays
+// in LDS (n <= 6 x 255 = 1530) and goes to rhs at the end.
+__global__ __launch_bounds__(1024) void k_bsolve(Batch b, int w) {
+  constexpr int kG = 16;                        // row groups
+  __shared__ double sx[6 * kMaxPoses];           // y, overwritten by x block by block
+  __shared__ double part[kG * kLB];
+  __shared__ double sr[kLB];
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
-  const int n = L.n, lane = threadIdx.x, j = blockIdx.x;
-  const int b0 = kLB * bk, bb = min(kLB, n - b0);
-  sw[lane] = lane < bb ? L.rhs[b0 + lane] : 0.0;
+  const int n = L.n, tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+  const int nb = (n + kLB - 1) / kLB;
+  for (int i = tid; i < n; i += blockDim.x) sx[i] = L.rhs[i];
   __syncthreads();
-  const double* X = L.Linv + static_cast<int64_t>(bk) * kLB * kLB;
-  double x = 0.0;
-  for (int m = lane; m < bb; ++m) x += X[m * kLB + lane] * sw[m];  // x_c = sum_{m >= c} X[m][c] w_m
-  if (j == bk) {
-    if (lane < bb) L.rhs[b0 + lane] = x;
-    return;
+  for (int bk = nb - 1; bk >= 0; --bk) {
+    const int b0 = kLB * bk, bb = min(kLB, n - b0);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four rows in flight per thread
+    if (c < bb) {
+      const double* col = L.A + b0 + c;
+      int i = b0 + kLB + g;
+      for (; i + 3 * kG < n; i += 4 * kG) {
+        s0 += col[static_cast<int64_t>(i) * n] * sx[i];
+        s1 += col[static_cast<int64_t>(i + kG) * n] * sx[i + kG];
+        s2 += col[static_cast<int64_t>(i + 2 * kG) * n] * sx[i + 2 * kG];
+        s3 += col[static_cast<int64_t>(i + 3 * kG) * n] * sx[i + 3 * kG];
+      }
+      for (; i < n; i += kG) s0 += col[static_cast<int64_t>(i) * n] * sx[i];
+    }
+    part[g * kLB + c] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (tid < kLB) {
+      double r = 0.0;
+      if (c < bb) {
+        r = sx[b0 + c];
+#pragma unroll
+        for (int q = 0; q < kG; ++q) r -= part[q * kLB + c];
+      }
+      sr[c] = r;
+    }
+    __syncthreads();
+    // x_c = sum_{m >= c} X[m][c] r_m, the sum split over the 16 groups
+    const double* X = L.Linv + static_cast<int64_t>(bk) * kLB * kLB;
+    double xs = 0.0;
+    if (c < bb)
+      for (int m = c + g; m < bb; m += kG) xs += X[m * kLB + c] * sr[m];
+    part[g * kLB + c] = xs;
+    __syncthreads();
+    if (tid < bb) {
+      double x = 0.0;
+#pragma unroll
+      for (int q = 0; q < kG; ++q) x += part[q * kLB + c];
+      sx[b0 + c] = x;
+    }
+    __syncthreads();
   }
-  sx[lane] = lane < bb ? x : 0.0;
-  __syncthreads();
-  // w_j[c] -= sum_r L[b0 + r][j0 + c] x_r (rows of block bk, columns of block j < bk)
-  const int j0 = kLB * j;
-  double s = 0.0;
-  for (int r = 0; r < bb; ++r) s += L.A[static_cast<int64_t>(b0 + r) * n + j0 + lane] * sx[r];
-  L.rhs[j0 + lane] -= s;
+  for (int i = tid; i < n; i += blockDim.x) L.rhs[i] = sx[i];
 }
 
 // The pose step, max |dc|, dc^T diag(U) dc and the trial key frames of window w.
@@ -524,10 +554,9 @@ hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s) {
     }
     e = hipGetLastError();
   }
-  for (int bk = nb - 1; bk >= 0 && e == hipSuccess; --bk) {
-    hipLaunchKernelGGL(k_bsolve, dim3(bk + 1), dim3(64), 0, s, b, w, bk);
-    e = hipGetLastError();
-  }
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bsolve, dim3(1), dim3(1024), 0, s, b, w);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lepilogue, dim3(1), dim3(256), 0, s, b, w);
   return hipGetLastError();
