@@ -132,18 +132,26 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   W.robust = p.robustifier_scale;
   W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
-  // counting sort by point, then (key frame, camera) inside each point
+  // order (point, key frame, camera, input index): a stable counting sort by (key frame, camera) of the input
+  // order, then a stable counting sort by point
+  std::vector<int32_t> by_kc(no);
+  {
+    std::vector<int32_t> cnt(static_cast<size_t>(nc) * ncam + 1, 0);
+    for (int o = 0; o < no; ++o) ++cnt[p.pose_ids[o] * ncam + p.camera_ids[o] + 1];
+    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+    for (int o = 0; o < no; ++o) by_kc[cnt[p.pose_ids[o] * ncam + p.camera_ids[o]]++] = o;
+  }
   std::vector<int32_t> start(np + 1, 0);
   for (int o = 0; o < no; ++o) ++start[p.point_ids[o] + 1];
   for (int i = 0; i < np; ++i) start[i + 1] += start[i];
   std::vector<int32_t> order(no), fill(start.begin(), start.end() - 1);
-  for (int o = 0; o < no; ++o) order[fill[p.point_ids[o]]++] = o;
-  for (int i = 0; i < np; ++i)
-    std::sort(order.begin() + start[i], order.begin() + start[i + 1], [&](int a, int b) {
-      if (p.pose_ids[a] != p.pose_ids[b]) return p.pose_ids[a] < p.pose_ids[b];
-      if (p.camera_ids[a] != p.camera_ids[b]) return p.camera_ids[a] < p.camera_ids[b];
-      return a < b;
-    });
+  for (int k = 0; k < no; ++k) {
+    const int o = by_kc[k];
+    order[fill[p.point_ids[o]]++] = o;
+  }
+  H.obs_orig.reserve(H.obs_orig.size() + no);
+  H.obs_xy.reserve(H.obs_xy.size() + no);
+  H.obs_info.reserve(H.obs_info.size() + no);
   const int obs_base = static_cast<int>(H.obs_xy.size());
   for (int k = 0; k < no; ++k) {
     const int o = order[k];
@@ -158,6 +166,11 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
     std::memcpy(&inf.w, &cam, 4);
     H.obs_info.push_back(inf);
   }
+  H.pair_obs.reserve(H.pair_obs.size() + no);
+  H.pair_pose.reserve(H.pair_pose.size() + no);
+  H.point_pair.reserve(H.point_pair.size() + np + 1);
+  H.point_nfree.reserve(H.point_nfree.size() + np);
+  H.pts.reserve(H.pts.size() + np);
   for (int i = 0; i < np; ++i) {
     H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
     int nfree = 0;
@@ -249,10 +262,7 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       for (int q = H.point_pair[i]; q < e; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - T.point_begin) << 8);
     }
   }
-  for (size_t q = first_pair; q < H.pair_rec.size(); ++q) {
-    H.pair_xy.push_back(H.obs_xy[H.pair_rec[q].x]);
-    H.pair_info.push_back(H.obs_info[H.pair_rec[q].x]);
-  }
+  (void)first_pair;
   return SBA_OK;
 }
 
@@ -566,6 +576,12 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     d.pair_xy = d.obs_xy;
     d.pair_info = d.obs_info;
   } else {
+    H.pair_xy.resize(H.pair_rec.size());
+    H.pair_info.resize(H.pair_rec.size());
+    for (size_t q = 0; q < H.pair_rec.size(); ++q) {
+      H.pair_xy[q] = H.obs_xy[H.pair_rec[q].x];
+      H.pair_info[q] = H.obs_info[H.pair_rec[q].x];
+    }
     UP(H.pair_xy, pair_xy);
     UP(H.pair_info, pair_info);
   }
