@@ -13,6 +13,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -149,22 +150,35 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
     const int o = by_kc[k];
     order[fill[p.point_ids[o]]++] = o;
   }
-  H.obs_orig.reserve(H.obs_orig.size() + no);
-  H.obs_xy.reserve(H.obs_xy.size() + no);
-  H.obs_info.reserve(H.obs_info.size() + no);
   const int obs_base = static_cast<int>(H.obs_xy.size());
-  for (int k = 0; k < no; ++k) {
-    const int o = order[k];
-    H.obs_orig.push_back(o);
-    H.obs_xy.push_back(make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]));
-    const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
-    float4 inf;
-    inf.x = L[0];
-    inf.y = 0.5f * (L[1] + L[2]);
-    inf.z = L[3];
-    int cam = p.camera_ids[o];
-    std::memcpy(&inf.w, &cam, 4);
-    H.obs_info.push_back(inf);
+  H.obs_orig.resize(obs_base + no);
+  H.obs_xy.resize(obs_base + no);
+  H.obs_info.resize(obs_base + no);
+  // the gather into sorted order reads the input at random: split over host threads (disjoint output ranges)
+  auto gather = [&](int k_begin, int k_end) {
+    for (int k = k_begin; k < k_end; ++k) {
+      const int o = order[k];
+      H.obs_orig[obs_base + k] = o;
+      H.obs_xy[obs_base + k] = make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]);
+      const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
+      float4 inf;
+      inf.x = L[0];
+      inf.y = 0.5f * (L[1] + L[2]);
+      inf.z = L[3];
+      int cam = p.camera_ids[o];
+      std::memcpy(&inf.w, &cam, 4);
+      H.obs_info[obs_base + k] = inf;
+    }
+  };
+  const int nthreads = no < (1 << 16) ? 1 : static_cast<int>(std::min(8u, std::max(1u, std::thread::hardware_concurrency())));
+  if (nthreads == 1) {
+    gather(0, no);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back(gather, static_cast<int>(static_cast<int64_t>(no) * t / nthreads),
+                        static_cast<int>(static_cast<int64_t>(no) * (t + 1) / nthreads));
+    for (auto& th : pool) th.join();
   }
   H.pair_obs.reserve(H.pair_obs.size() + no);
   H.pair_pose.reserve(H.pair_pose.size() + no);
