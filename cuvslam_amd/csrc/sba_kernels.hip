@@ -23,6 +23,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "sba_device.h"
@@ -134,18 +135,24 @@ __device__ __forceinline__ void jrot(float a[3][3], float v[3][3], int p, int q)
 // rank 2 drops the smallest direction, rank <= 1 gives zero.
 //
 // Well-conditioned blocks take a short path: for a symmetric positive definite 3x3 with eigenvalues
-// l1 >= l2 >= l3, l3 / l1 >= det / trace^3, so det > 4e-6 trace^3 puts every eigenvalue above the rank
-// threshold; then V*^-1 = L^-T L^-1 with the Cholesky factor L, and G = L^-T has the same G G^T as the
-// eigen-decomposition (only the rank decision needs the eigenvalues).
+// l1 >= l2 >= l3, the sum of its principal 2x2 minors is m2 = l1 l2 + l1 l3 + l2 l3 >= l1 l2, so
+// l3 = det / (l1 l2) >= det / m2, and l1 <= trace; det > 4e-6 trace m2 therefore puts every eigenvalue above
+// the rank threshold (with a factor 4 to spare for rounding). The bound is close to l3 itself (m2 ~ l1 l2
+// when l3 is small), so only blocks near the threshold take the eigen-decomposition. Then
+// V*^-1 = L^-T L^-1 with the Cholesky factor L, and G = L^-T has the same G G^T as the eigen-decomposition
+// (only the rank decision needs the eigenvalues).
 __device__ __forceinline__ void point_factor(const float V[6], float lambda, float G[3][3]) {
   float a[3][3] = {{V[0] * (1.f + lambda), V[1], V[2]}, {V[1], V[3] * (1.f + lambda), V[4]},
                    {V[2], V[4], V[5] * (1.f + lambda)}};
   {
     const float tr = a[0][0] + a[1][1] + a[2][2];
-    const float det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[1][2]) -
-                      a[0][1] * (a[0][1] * a[2][2] - a[0][2] * a[1][2]) +
+    const float m00 = a[1][1] * a[2][2] - a[1][2] * a[1][2];
+    const float m11 = a[0][0] * a[2][2] - a[0][2] * a[0][2];
+    const float m22 = a[0][0] * a[1][1] - a[0][1] * a[0][1];
+    const float det = a[0][0] * m00 - a[0][1] * (a[0][1] * a[2][2] - a[0][2] * a[1][2]) +
                       a[0][2] * (a[0][1] * a[1][2] - a[0][2] * a[1][1]);
-    if (tr > 0.f && det > 4e-6f * tr * tr * tr && a[0][0] > 0.f) {
+    const float m2 = m00 + m11 + m22;
+    if (tr > 0.f && m2 > 0.f && det > 4e-6f * tr * m2 && a[0][0] > 0.f) {
       const float l00 = sqrtf(a[0][0]), i00 = 1.f / l00;
       const float l10 = a[0][1] * i00, l20 = a[0][2] * i00;
       const float d11 = a[1][1] - l10 * l10;
@@ -200,6 +207,14 @@ __device__ __forceinline__ T wave_sum(T x) {
 }
 
 __device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
+
+// Sum over the four lanes of a quad, (x0 + x1) + (x2 + x3) in every lane of it (the same bits in all four: the
+// two partial sums are added in either order).
+__device__ __forceinline__ float quad_sum(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // lane ^ 1
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // lane ^ 2
+  return x;
+}
 
 __device__ __forceinline__ void load_pose_lds(float* dst, const float4* src, int n3, int tid, int nthreads) {
   for (int k = tid; k < n3; k += nthreads) {
@@ -361,12 +376,21 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const bool u_owner = tid < kUGroups * nloc;
   const int uj = tid / kUGroups, u0 = kUGroup * (tid % kUGroups), u1 = min(u0 + kUGroup, kUVals);
   float accU[kUGroup] = {0.f, 0.f, 0.f, 0.f, 0.f};
-  // V owner: tile point vp
-  const int vp = tid - (kBlock - kTilePoints);
+  // V owners of tile point vp: the last wavefront, four lanes per point (lane vs takes every fourth pair) when
+  // the U owners leave it free; otherwise its last 16 lanes, one per point. The choice depends on the window
+  // only (a mapped window always takes one lane), so a point's sums do not depend on the chunking.
+  const bool v4 = !W.mapped && kUGroups * nloc <= kBlock - 64;
+  const int vp = v4 ? (tid - (kBlock - 64)) >> 2 : tid - (kBlock - kTilePoints);
+  const int vs = v4 ? tid & 3 : 0, vstep = v4 ? 4 : 1;
 
   for (int k = tid; k < kTilePoints * kMaxFreePoses; k += kBlock) sEnt[k] = -1;
-  // the first tile's pair data and points
+  // the first tile's pair data, points and point pair ranges
   Tile Tn = b.tiles[ch.tile_begin];
+  int pp_a = 0, pp_e = 0;  // pair range of point vp, loaded one tile ahead
+  if (vp >= 0 && vp < Tn.point_end - Tn.point_begin) {
+    pp_a = b.point_pair[Tn.point_begin + vp];
+    pp_e = b.point_pair[Tn.point_begin + vp + 1];
+  }
   PairPre pre{};
   if (tid < Tn.pair_end - Tn.pair_begin) {
     pre.rec = b.pair_rec[Tn.pair_begin + tid];
@@ -401,18 +425,19 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       const float4 p = sPts[(t & 1) * kTilePoints + tp];
       // the pair's terms go straight to its own scratch column (no other thread touches it): the first valid
       // observation stores, later ones (other cameras of a rig) add; a pair without one stores zeros
-      bool first = true;
-      auto put = [&](float* arr, int k, float v) {
-        float* q = arr + k * kScr + tid;
-        *q = first ? v : *q + v;
-      };
-      for (int o = pre.rec.x; o < pre.rec.y; ++o) {
+      // (store mode is a template argument, so that the common case of one observation per pair only writes)
+      auto obs = [&](int o, auto store) -> bool {
+        auto put = [&](float* arr, int k, float v) {
+          float* q = arr + k * kScr + tid;
+          if constexpr (decltype(store)::value) *q = v;
+          else *q += v;
+        };
         const float2 xy = o == pre.rec.x ? pre.xy : b.obs_xy[o];
         const float4 inf = o == pre.rec.x ? pre.inf : b.obs_info[o];
         const int cam = __float_as_int(inf.w);
         Lin L;
         if (!linearize(sPose + 12 * j, sCam + 12 * cam, p.x, p.y, p.z, xy.x, xy.y, inf.x, inf.y, inf.z, kind, delta, L))
-          continue;
+          return false;
         float LJc[2][6], LJp[2][3];
         const float wr0 = L.w * (inf.x * L.r0 + inf.y * L.r1);
         const float wr1 = L.w * (inf.y * L.r0 + inf.z * L.r1);
@@ -451,8 +476,12 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
           for (int k = 0; k < 3; ++k) put(sScrV, 6 + k, L.jp[0][k] * wr0 + L.jp[1][k] * wr1);
         }
-        first = false;
-      }
+        return true;
+      };
+      bool first = true;
+      int o = pre.rec.x;
+      for (; first && o < pre.rec.y; ++o) first = !obs(o, std::true_type{});
+      for (; o < pre.rec.y; ++o) obs(o, std::false_type{});
       if (first) {
         if (pair_cf) {
           for (int k = 0; k < kUVals; ++k) sScrU[k * kScr + tid] = 0.f;
@@ -468,32 +497,52 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     // (c) owners sum the pair terms: U and e_a of key frame uj into registers; V, e_b of point vp, then its
     // damped pseudo-inverse V*^+ = G G^T
     if (u_owner) {
-      int ent[kTilePoints];
+      // four tile points at a time: their table entries, then their 20 values, each batch of loads in flight
+      // together (values past u1 are read and dropped: rows u0 + v <= 29 lie inside the scratch, whose 9 V
+      // rows follow the 27 U rows)
+#pragma unroll 1
+      for (int q0 = 0; q0 < npts; q0 += 4) {
+        int en[4];
 #pragma unroll
-      for (int q = 0; q < kTilePoints; ++q) ent[q] = sEnt[q * kMaxFreePoses + uj];
+        for (int j = 0; j < 4; ++j) en[j] = sEnt[(q0 + j) * kMaxFreePoses + uj];
+        float x[4][kUGroup];
+        bool ok[4];
 #pragma unroll
-      for (int q = 0; q < kTilePoints; ++q) {
-        const bool ok = q < npts && ent[q] >= 0 && (ent[q] & ~511) == stamp;
-        const int col = ok ? (ent[q] & 511) : 0;
+        for (int j = 0; j < 4; ++j) {
+          ok[j] = q0 + j < npts && en[j] >= 0 && (en[j] & ~511) == stamp;
+          const int col = ok[j] ? (en[j] & 511) : 0;
 #pragma unroll
-        for (int v = 0; v < kUGroup; ++v)
-          if (u0 + v < u1) {
-            const float x = sScrU[(u0 + v) * kScr + col];
-            accU[v] += ok ? x : 0.f;
-          }
+          for (int v = 0; v < kUGroup; ++v) x[j][v] = sScrU[(u0 + v) * kScr + col];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int v = 0; v < kUGroup; ++v) accU[v] += ok[j] && u0 + v < u1 ? x[j][v] : 0.f;
       }
     }
     if (vp >= 0 && vp < npts && T.point_begin + vp < pf_end) {
       const int i = T.point_begin + vp;
-      const int a = b.point_pair[i] - p0, e = b.point_pair[i + 1] - p0;
-      float V[6] = {}, eb[3] = {};
-#pragma unroll 4
-      for (int k = a; k < e; ++k) {
+      const int a = pp_a - p0, e = pp_e - p0;
+      // pairs a + vs, a + vs + vstep, ... two at a time (18 loads in flight), then the four lanes of a point add
+      // their sums
+      float V[9] = {};
+      for (int k = a + vs; k < e; k += 2 * vstep) {
+        const int k2 = min(k + vstep, e - 1);
+        const bool two = k + vstep < e;
+        float x[9], y[9];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) V[m] += sScrV[m * kScr + k];
+        for (int m = 0; m < 9; ++m) {
+          x[m] = sScrV[m * kScr + k];
+          y[m] = sScrV[m * kScr + k2];
+        }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) eb[m] += sScrV[(6 + m) * kScr + k];
+        for (int m = 0; m < 9; ++m) V[m] += two ? x[m] + y[m] : x[m];
       }
+      if (v4) {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) V[m] = quad_sum(V[m]);
+      }
+      const float eb[3] = {V[6], V[7], V[8]};
       float G[3][3];
       point_factor(V, lam, G);
       float tv[3], Vi[6], gb[3];
@@ -507,18 +556,20 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       Vi[5] = G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2];
 #pragma unroll
       for (int k = 0; k < 3; ++k) gb[k] = G[k][0] * tv[0] + G[k][1] * tv[1] + G[k][2] * tv[2];
-      float4* ax = b.paux + 3 * i;
-      ax[0] = make_float4(Vi[0], Vi[1], Vi[2], Vi[3]);
-      ax[1] = make_float4(Vi[4], Vi[5], gb[0], gb[1]);
-      ax[2] = make_float4(gb[2], V[0], V[3], V[5]);
-      float* ac = sAcc + 12 * vp;
+      if (vs == 0) {
+        float4* ax = b.paux + 3 * i;
+        ax[0] = make_float4(Vi[0], Vi[1], Vi[2], Vi[3]);
+        ax[1] = make_float4(Vi[4], Vi[5], gb[0], gb[1]);
+        ax[2] = make_float4(gb[2], V[0], V[3], V[5]);
+        float* ac = sAcc + 12 * vp;
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ac[3 * r + c] = G[r][c];
-      ac[9] = tv[0];
-      ac[10] = tv[1];
-      ac[11] = tv[2];
+          for (int c = 0; c < 3; ++c) ac[3 * r + c] = G[r][c];
+        ac[9] = tv[0];
+        ac[10] = tv[1];
+        ac[11] = tv[2];
+      }
     }
     lds_barrier();
     // (z) zero the staged columns that the matrix cores read: kpad k-rows x 16 nt rows, 16-byte stores
@@ -561,20 +612,31 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       pre.inf = b.pair_info[Tn.pair_begin + tid];
     }
     if (tid < npts_next) pn = pts[Tn.point_begin + tid];
+    if (vp >= 0 && vp < npts_next) {
+      pp_a = b.point_pair[Tn.point_begin + vp];
+      pp_e = b.point_pair[Tn.point_begin + vp + 1];
+    }
     if (u_owner && u1 > 21) {
-      int ent[kTilePoints];
+      // as in (c); rows -1 .. 8 around the six e_c rows are read (the pair table lies before them, the W scratch
+      // after them) and those outside 21 <= u0 + v < u1 dropped
+#pragma unroll 1
+      for (int q0 = 0; q0 < npts; q0 += 4) {
+        int en[4];
 #pragma unroll
-      for (int q = 0; q < kTilePoints; ++q) ent[q] = sEnt[q * kMaxFreePoses + uj];
+        for (int j = 0; j < 4; ++j) en[j] = sEnt[(q0 + j) * kMaxFreePoses + uj];
+        float x[4][kUGroup];
+        bool ok[4];
 #pragma unroll
-      for (int q = 0; q < kTilePoints; ++q) {
-        const bool ok = q < npts && ent[q] >= 0 && (ent[q] & ~511) == stamp;
-        const int col = ok ? (ent[q] & 511) : 0;
+        for (int j = 0; j < 4; ++j) {
+          ok[j] = q0 + j < npts && en[j] >= 0 && (en[j] & ~511) == stamp;
+          const int col = ok[j] ? (en[j] & 511) : 0;
 #pragma unroll
-        for (int v = 0; v < kUGroup; ++v)
-          if (u0 + v >= 21 && u0 + v < u1) {
-            const float x = sEc[(u0 + v - 21) * kScr + col];
-            accU[v] += ok ? x : 0.f;
-          }
+          for (int v = 0; v < kUGroup; ++v) x[j][v] = sEc[(u0 + v - 21) * kScr + col];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int v = 0; v < kUGroup; ++v) accU[v] += ok[j] && u0 + v >= 21 && u0 + v < u1 ? x[j][v] : 0.f;
       }
     }
     switch (tid >> 6) {
