@@ -135,19 +135,25 @@ def main():
 
     work = RF.work(problems)
     # the dominant kernel and the roofline that bounds it
+    def roofline(kern):
+        bytes_, f32, f64 = work[{"k_schur": "schur", "k_factor": "factor", "k_backsub": "backsub"}[kern]]
+        t_launch = per_launch[kern] * 1e-3
+        if f64 > f32:
+            ach, peak, unit, bound = f64 / t_launch / 1e12, RF.FP64_PEAK_TFLOPS, "TFLOP/s", "mfma"
+        elif f32 / max(bytes_, 1) > RF.FP32_PEAK_TFLOPS * 1e3 / RF.HBM_PEAK_GBS:
+            ach, peak, unit, bound = f32 / t_launch / 1e12, RF.FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
+        else:
+            ach, peak, unit, bound = bytes_ / t_launch / 1e9, RF.HBM_PEAK_GBS, "GB/s", "hbm"
+        return {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": unit, "frac": round(ach / peak, 6),
+                "traffic": measured_traffic(args.workload, kern), "kernel": kern,
+                "launch_ms": round(per_launch[kern], 5),
+                "algorithmic": {"bytes": bytes_, "flops_f32": f32, "flops_f64": f64}}
+
     dom = max(["k_schur", "k_factor", "k_backsub"], key=lambda n: per_launch[n])
-    key = {"k_schur": "schur", "k_factor": "factor", "k_backsub": "backsub"}[dom]
-    bytes_, f32, f64 = work[key]
-    t_launch = per_launch[dom] * 1e-3
-    if f64 > f32:
-        ach, peak, unit, bound = f64 / t_launch / 1e12, RF.FP64_PEAK_TFLOPS, "TFLOP/s", "mfma"
-    elif f32 / max(bytes_, 1) > RF.FP32_PEAK_TFLOPS * 1e3 / RF.HBM_PEAK_GBS:
-        ach, peak, unit, bound = f32 / t_launch / 1e12, RF.FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
-    else:
-        ach, peak, unit, bound = bytes_ / t_launch / 1e9, RF.HBM_PEAK_GBS, "GB/s", "hbm"
-    roof = {"bound": bound, "achieved": round(ach, 4), "peak": peak, "unit": unit, "frac": round(ach / peak, 6),
-            "traffic": measured_traffic(args.workload, dom), "kernel": dom, "launch_ms": round(per_launch[dom], 5),
-            "algorithmic": {"bytes": bytes_, "flops_f32": f32, "flops_f64": f64}}
+    roof = roofline(dom)
+    # the same figures for each of the three main kernels
+    roof_all = {k: {f: v for f, v in roofline(k).items() if f in ("bound", "achieved", "unit", "frac", "launch_ms")}
+                for k in ("k_schur", "k_factor", "k_backsub")}
 
     # accuracy against the FP64 oracle (window 0)
     res = batch.read()
@@ -277,6 +283,7 @@ def main():
         "config": {"workload": workload, "windows": len(problems), "observations": n_obs,
                    "pairs": info["pairs"], "lm_iterations": 10, "chunks": info["chunks"], "autotune": tuned},
         "roofline": roof,
+        "roofline_by_kernel": roof_all,
         "cpu_baseline": cpu,
         "kernels_ms_per_launch": {k: round(v, 5) for k, v in per_launch.items()},
         "accuracy": extra,
