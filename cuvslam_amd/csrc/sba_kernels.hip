@@ -1082,7 +1082,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   __shared__ float sPose[12 * kMaxPoses];
   __shared__ float sTrial[12 * kMaxPoses];
   __shared__ float sCam[12 * kMaxCameras];
-  __shared__ float sAcc[4 * kTilePoints];  // sum_j W_ij^T dc_j, then dp
+  __shared__ float sAcc[4 * kBlock];  // per tile point (up to kBlock): sum_j W_ij^T dc_j, then dp
   __shared__ float sDc[6 * kMaxPoses];
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];

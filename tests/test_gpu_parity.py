@@ -146,6 +146,36 @@ def test_empty_problem_returns_false(solver_huber):
     assert not solver_huber.solve(q)
 
 
+def max_points_per_back_substitution_tile(p, max_pairs=256, max_points=256):
+    """Largest number of points in one back-substitution tile: consecutive points holding at most 256
+    (point, key frame) pairs and at most 256 points."""
+    key = np.unique(np.asarray(p.point_ids).astype(np.int64) * 4096 + np.asarray(p.pose_ids))
+    pairs = np.bincount(key // 4096, minlength=p.points.shape[0])
+    best, i, n = 0, 0, len(pairs)
+    while i < n:
+        b, k = i, 0
+        while i < n and i - b < max_points and (k + pairs[i] <= max_pairs or i == b):
+            k += pairs[i]
+            i += 1
+        best = max(best, i - b)
+    return best
+
+
+def test_back_substitution_tiles_of_many_points_match_oracle():
+    # points seen from two to five key frames: a back-substitution tile holds far more points than a Schur tile,
+    # and every point's camera coupling sum_j W_ij^T dc_j must reach its step
+    p, _, _ = small_scene(seed=41, n_points=800, n_kf=5, sample_observations=1, max_iterations=8)
+    assert max_points_per_back_substitution_tile(p) > 32
+    ref = O.solve(p, "f64", loss=HUBER)
+    q = p.copy()
+    assert ca.SchurComplementBundlerGpu(loss=HUBER).solve(q)
+    assert q.iterations == ref["iterations"]
+    assert q.accepted == ref["accepted"]
+    assert abs(q.last_cost_f64 - ref["last_cost"]) / ref["last_cost"] < 1e-6
+    c, _ = O.cost(q, loss=HUBER)
+    assert abs(c - ref["last_cost"]) / ref["last_cost"] < 1e-6
+
+
 def test_batch_matches_single(solver_huber):
     probs = [small_scene(seed=20 + k, n_points=200 + 100 * k, n_kf=5 + 3 * k)[0] for k in range(4)]
     singles = []
