@@ -206,8 +206,6 @@ __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
-__device__ __forceinline__ void lds_add(float* p, float v) { atomicAdd(p, v); }
-
 // Sum over the four lanes of a quad, (x0 + x1) + (x2 + x3) in every lane of it (the same bits in all four: the
 // two partial sums are added in either order).
 __device__ __forceinline__ float quad_sum(float x) {
@@ -1082,7 +1080,8 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   __shared__ float sPose[12 * kMaxPoses];
   __shared__ float sTrial[12 * kMaxPoses];
   __shared__ float sCam[12 * kMaxCameras];
-  __shared__ float sAcc[4 * kBlock];  // per tile point (up to kBlock): sum_j W_ij^T dc_j, then dp
+  __shared__ float sAcc[4 * kBlock];  // per tile point (up to kBlock): dp
+  __shared__ float sPair[3 * kBlock];  // per pair: its part of sum_j W_ij^T dc_j
   __shared__ float sDc[6 * kMaxPoses];
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];
@@ -1111,6 +1110,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   double cost = 0.0, hess = 0.0, scal = 0.0;
   int skipped = 0;
   float maxdp = 0.f;
+  __syncthreads();  // key frames, cameras and dc in shared memory
 
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
     const Tile T = b.btiles[t];
@@ -1126,23 +1126,23 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
       xy0 = b.pair_xy[p0 + tid];
       inf0 = b.pair_info[p0 + tid];
     }
-    __syncthreads();
-    if (tid < npts) sAcc[4 * tid] = sAcc[4 * tid + 1] = sAcc[4 * tid + 2] = sAcc[4 * tid + 3] = 0.f;
-    __syncthreads();
+    // No barrier here: this tile's (a) writes only sPair, which the previous tile last read before its (b)-(c)
+    // barrier, and (b) writes sAcc after the (a)-(b) barrier, which every thread reaches after its previous (c).
     // the first observation's J_c dc_j, J_p and weight, kept from (a) for the predicted reduction in (c):
     // kept 0 = not linearized in (a), 1 = kept, 2 = behind the camera
     int kept = 0;
     float ku0 = 0.f, ku1 = 0.f, kw = 0.f, kjp[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     if (!INITIAL) {
-      // (a) sum_j W_ij^T dc_j = sum_obs w J_p^T Lambda J_c dc_j
+      // (a) per pair: sum over its observations of w J_p^T Lambda J_c dc_j, into the pair's own slot (the point's
+      // sum over its pairs is taken in pair order in (b): no floating-point atomics)
       if (tid < npair) {
+        float acc[3] = {0.f, 0.f, 0.f};
         const int tp = rec.z >> 8;  // point inside this back-substitution tile
         const int i = T.point_begin + tp;
         const int j = rec.z & 255;
         if (i < pf_end && j < ncf) {
           const float4 p = pts[i];
           const float* d = sDc + 6 * j;
-          float acc[3] = {0.f, 0.f, 0.f};
           for (int o = rec.x; o < rec.y; ++o) {
             const float2 xy = o == rec.x ? xy0 : b.obs_xy[o];
             const float4 inf = o == rec.x ? inf0 : b.obs_info[o];
@@ -1173,10 +1173,10 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[k] += L.jp[0][k] * lu0 + L.jp[1][k] * lu1;
           }
-          lds_add(sAcc + 4 * tp, acc[0]);
-          lds_add(sAcc + 4 * tp + 1, acc[1]);
-          lds_add(sAcc + 4 * tp + 2, acc[2]);
         }
+        sPair[3 * tid] = acc[0];
+        sPair[3 * tid + 1] = acc[1];
+        sPair[3 * tid + 2] = acc[2];
       }
       __syncthreads();
       // (b) one thread per point: dp = V*^-1 e_b - V*^-1 (sum W^T dc); trial point
@@ -1187,7 +1187,21 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
         if (i < pf_end) {
           const float4 a0 = b.paux[3 * i], a1 = b.paux[3 * i + 1], a2 = b.paux[3 * i + 2];
           const float Vi[3][3] = {{a0.x, a0.y, a0.z}, {a0.y, a0.w, a1.x}, {a0.z, a1.x, a1.y}};
-          const float s[3] = {sAcc[4 * tid], sAcc[4 * tid + 1], sAcc[4 * tid + 2]};
+          // the point's pairs in order, two at a time
+          const int qa = b.point_pair[i] - p0, qe = b.point_pair[i + 1] - p0;
+          float s[3] = {0.f, 0.f, 0.f};
+          for (int q = qa; q < qe; q += 2) {
+            const int q2 = min(q + 1, qe - 1);
+            const bool two = q + 1 < qe;
+            float x[3], y[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              x[k] = sPair[3 * q + k];
+              y[k] = sPair[3 * q2 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s[k] += two ? x[k] + y[k] : x[k];
+          }
           const float gb[3] = {a1.z, a1.w, a2.x};
 #pragma unroll
           for (int k = 0; k < 3; ++k) dp[k] = gb[k] - (Vi[k][0] * s[0] + Vi[k][1] * s[1] + Vi[k][2] * s[2]);

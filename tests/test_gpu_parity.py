@@ -196,8 +196,8 @@ def test_every_window_of_an_uneven_batch_matches_oracle():
     # windows of 5 to 40 key frames, points seen from two key frames up to all of them, outliers up to 10%: each
     # window of the batch against its own FP64 oracle solve. Windows of few key frames whose points are seen from
     # two or three of them are weakly constrained and not converged after 8 iterations: there the FP32 path's
-    # last-bit differences (including the run-to-run order of atomic sums) grow to 1e-6 .. 7e-6 of the cost over
-    # the trajectory (measured over repeated runs, at most two windows of ten past 1e-6); most stay within 1e-6.
+    # last-bit differences grow to 1e-6 .. 7e-6 of the cost over the trajectory (measured over variants of the
+    # summation order, at most two windows of ten past 1e-6); most stay within 1e-6.
     probs = ca.scene.batched_windows(10, seed=5, max_iterations=8, pt_range=(300, 2500))
     qs = [p.copy() for p in probs]
     assert all(ca.SchurComplementBundlerGpu(loss=HUBER).solve_batch(qs))
