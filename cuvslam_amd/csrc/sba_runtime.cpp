@@ -100,6 +100,13 @@ struct HostPack {
   std::vector<Tile> tiles, btiles;
 };
 
+// Room for `extra` more elements with geometric growth: an exact reserve per window would copy every earlier window
+// again (quadratic in the batch).
+template <class V>
+void reserve_more(V& v, size_t extra) {
+  if (v.size() + extra > v.capacity()) v.reserve(std::max(v.size() + extra, 2 * v.capacity()));
+}
+
 // Validates one window and appends it to the packed streams.
 int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, WindowDesc& W) {
   const int np = p.num_points, nc = p.num_poses, no = p.num_observations, ncam = p.num_cameras;
@@ -180,11 +187,11 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
                         static_cast<int>(static_cast<int64_t>(no) * (t + 1) / nthreads));
     for (auto& th : pool) th.join();
   }
-  H.pair_obs.reserve(H.pair_obs.size() + no);
-  H.pair_pose.reserve(H.pair_pose.size() + no);
-  H.point_pair.reserve(H.point_pair.size() + np + 1);
-  H.point_nfree.reserve(H.point_nfree.size() + np);
-  H.pts.reserve(H.pts.size() + np);
+  reserve_more(H.pair_obs, no);
+  reserve_more(H.pair_pose, no);
+  reserve_more(H.point_pair, np + 1);
+  reserve_more(H.point_nfree, np);
+  reserve_more(H.pts, np);
   for (int i = 0; i < np; ++i) {
     H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
     int nfree = 0;
