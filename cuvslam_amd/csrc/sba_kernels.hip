@@ -1,10 +1,11 @@
 // HIP kernels of the MI355X sliding-window SBA (gfx950).
 //
-// One Levenberg-Marquardt iteration of every window of a batch is four launches:
+// One Levenberg-Marquardt iteration of every window of a batch is five launches:
 //   k_schur     per chunk of a window's points: linearize each (point, key frame) pair (residual, 2x6 pose and
 //               2x3 point Jacobians, robust weight), accumulate V_i, e_b, W_ij, U_j, e_a, damp and
-//               pseudo-invert V_i, and subtract the Schur pair products W_ij V_i^-1 W_ik^T from a shared-memory
-//               partial of the reduced camera system S, flushed once per chunk with FP32 atomics.
+//               pseudo-invert V_i, and accumulate the Schur pair products W_ij V_i^-1 W_ik^T on the matrix cores;
+//               each chunk writes its partial of the reduced camera system S once, as a record
+//   k_reduce    per window: the chunk records summed in a fixed order
 //               (reference phases: calc_jacobians_kernel sba_v1.cu:361-460, build_full_system_1/2 :486-631,
 //               reduced_system_stage_1/12/2/3 :1016-1204; CPU twin schur_complement_bundler_cpu.cpp:64-287)
 //   k_factor    per window: assemble S + lambda diag(U) in FP64, blocked right-looking Cholesky with the
