@@ -47,6 +47,7 @@ class BundleAdjustmentProblem:
     last_cost_f64: float = 0.0
     status: int = 0
     accepted: int = 0
+    final_lambda: float = 0.0               # the LM damping when the solve ended
     _keep: list = field(default_factory=list, repr=False)
 
     def copy(self) -> "BundleAdjustmentProblem":
@@ -107,6 +108,7 @@ def _apply(p: BundleAdjustmentProblem, r: _lib.Result, out_pts, out_poses) -> bo
         p.initial_cost_f64 = float(r.initial_cost_f64)
         p.last_cost_f64 = float(r.last_cost_f64)
         p.accepted = int(r.accepted)
+        p.final_lambda = float(r.lambda_)
     if r.status == _lib.SBA_OK:
         p.points = out_pts
         p.rig_from_world = out_poses.transpose(0, 2, 1).copy()

@@ -154,7 +154,8 @@ int sba_batch_winsorize(sba_handle* h, sba_batch* b, int32_t window, int32_t key
  *   S       [n*n] FP64 damped reduced camera system, n = 6 * free key frames (before factorization)
  *   e       [n]   reduced right-hand side
  *   dc      [n]   pose step;  dp [3 * free points] point step
- *   scalars [6]   trial cost, number skipped, hessian term, scaling term, max|dp|, max|dc|
+ *   scalars [8]   trial cost, number skipped, hessian term d^T H d, scaling term d^T diag(H) d, max|dp|, max|dc|,
+ *                 current cost, factorization succeeded (1) or not (0)
  * Returns SBA_OK, or SBA_E_INFEASIBLE when the factorization failed (S not positive definite). */
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc,
                          double* dp, double* scalars);

@@ -1037,7 +1037,8 @@ int orc_linearize_f64(const OrcProblem* p, const OrcOptions* o, double* r, doubl
 
 // One LM step at the input state with damping lambda (the phases of cpu.cpp:479-494):
 // S [n*n] damped reduced system, e [n], dc [n], dp [3 npf];
-// scalars: trial cost, skipped, delta^T H delta, delta^T diag(H) delta, max|dp|, max|dc|, current cost, ok.
+// scalars [9]: trial cost, skipped, delta^T H delta, delta^T diag(H) delta, max|dp|, max|dc|, current cost, ok,
+// predicted relative reduction (ComputePredictedRelativeReduction, cpu.cpp:385-433, at the current cost).
 int orc_step_f64(const OrcProblem* p, const OrcOptions* o, double lambda, double* S, double* e, double* dc,
                  double* dp, double* scalars) {
   Solver<double> s;
@@ -1058,7 +1059,7 @@ int orc_step_f64(const OrcProblem* p, const OrcOptions* o, double lambda, double
   const double current = s.evaluate_cost(nullptr, nullptr);
   const double trial = s.evaluate_cost(&u, &skipped);
   double hess = 0, scal = 0;
-  s.predicted(current > 0 ? current : 1.0, lambda, u, F, &hess, &scal);
+  const double pred = s.predicted(current > 0 ? current : 1.0, lambda, u, F, &hess, &scal);
   if (scalars) {
     scalars[0] = trial;
     scalars[1] = skipped;
@@ -1068,8 +1069,26 @@ int orc_step_f64(const OrcProblem* p, const OrcOptions* o, double lambda, double
     scalars[5] = Solver<double>::inf_norm(u.dc);
     scalars[6] = current;
     scalars[7] = u.ok ? 1.0 : 0.0;
+    scalars[8] = pred;
   }
   return u.ok ? 0 : 2;
+}
+
+// The predicted relative reduction of the step at the input state with damping lambda, divided by a given
+// current cost (cpu.cpp:385-433 and sba.cpp:688-727 divide by the caller's current cost; a vanished cost gives
+// a value that is not finite, which sba_test.cpp:1274-1317 pins).
+double orc_predicted_f64(const OrcProblem* p, const OrcOptions* o, double lambda, double current) {
+  Solver<double> s;
+  s.load(p, o);
+  Model<double> m;
+  s.update_model(m);
+  Full<double> F;
+  s.build_full(F, m);
+  Solver<double>::Reduced R;
+  s.build_reduced(R, F, lambda);
+  Solver<double>::Update u;
+  s.compute_update(u, R);
+  return s.predicted(current, lambda, u, F, nullptr, nullptr);
 }
 
 // Cost at the input state moved by (dc, dp) (either may be null for a zero step).

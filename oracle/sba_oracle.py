@@ -52,6 +52,8 @@ def lib():
         L.orc_linearize_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, dp, dp]
         L.orc_step_f64.argtypes = [PV, C.POINTER(OrcOptions), C.c_double, dp, dp, dp, dp, dp]
         L.orc_step_f64.restype = C.c_int
+        L.orc_predicted_f64.argtypes = [PV, C.POINTER(OrcOptions), C.c_double, C.c_double]
+        L.orc_predicted_f64.restype = C.c_double
         L.orc_cost_f64.argtypes = [PV, C.POINTER(OrcOptions), dp, dp, C.POINTER(C.c_int32)]
         L.orc_cost_f64.restype = C.c_double
         L.orc_chol_solve_f64.argtypes = [C.c_int32, dp, dp, dp]
@@ -121,15 +123,21 @@ def linearize(problem, **opt):
 
 
 def step(problem, lam, **opt):
-    """One LM step at the problem's state: (S, e, dc, dp, scalars, rc)."""
+    """One LM step at the problem's state: (S, e, dc, dp, scalars, rc). scalars: trial cost, skipped, d^T H d,
+    d^T diag(H) d, max|dp|, max|dc|, current cost, factorization ok, predicted relative reduction."""
     L = lib()
     v = problem.view()
     ncf = problem.rig_from_world.shape[0] - problem.num_fixed_key_frames
     npf = problem.points.shape[0] - problem.num_fixed_points
     n = 6 * ncf
-    S, e, dc, dp, sc = np.zeros((n, n)), np.zeros(n), np.zeros(n), np.zeros(3 * npf), np.zeros(8)
+    S, e, dc, dp, sc = np.zeros((n, n)), np.zeros(n), np.zeros(n), np.zeros(3 * npf), np.zeros(9)
     rc = L.orc_step_f64(C.byref(v), C.byref(options(**opt)), lam, _d(S), _d(e), _d(dc), _d(dp), _d(sc))
     return S, e, dc, dp, sc, rc
+
+
+def predicted(problem, lam, current, **opt):
+    """Predicted relative reduction of the step with damping lam at the problem's state, over `current`."""
+    return lib().orc_predicted_f64(C.byref(problem.view()), C.byref(options(**opt)), lam, current)
 
 
 def cost(problem, dc=None, dp=None, **opt):

@@ -7,7 +7,9 @@ every free parameter by FP64 central differences (a left-multiplied twist Exp(d)
 via scipy.linalg.expm; an additive step of each free point), the dense Gauss-Newton system
 H = J^T diag(w Lambda) J, g = J^T diag(w Lambda) r, the damped step (H + lambda diag(H)) d = g by
 numpy.linalg.solve, the reduced camera system of that damped system, the trial cost at the stepped state,
-and the predicted-reduction numerators d^T H d and d^T diag(H) d.
+the predicted-reduction numerators d^T H d and d^T diag(H) d, and the decrease of the quadratic model
+F(x + d) ~ F - 2 g^T d + d^T H d of the cost, 2 g^T d - d^T H d (F = sum rho(r^T L r) with no 1/2, so its
+gradient is -2 g and its Gauss-Newton Hessian 2 H).
 
 Run:  python tests/golden/make_fixtures.py
 """
@@ -137,7 +139,8 @@ def lm_step(P, kind, d, lams):
         e_red = g[:n] - Hd[:n, n:] @ np.linalg.solve(Hd[n:, n:], g[n:])
         trial = cost(P, *state_after(P, x), kind, d)
         out.append(dict(lam=lam, dc=x[:n].tolist(), dp=x[n:].tolist(), S=S.tolist(), e=e_red.tolist(),
-                        trial_cost=trial, hess=float(x @ H @ x), scal=float(x @ np.diag(np.diag(H)) @ x)))
+                        trial_cost=trial, hess=float(x @ H @ x), scal=float(x @ np.diag(np.diag(H)) @ x),
+                        model_reduction=float(2 * x @ g - x @ H @ x)))
     return F0, out
 
 
@@ -145,7 +148,7 @@ def main():
     cases = []
     for seed, (nposes, npts, nfp, nfpt), kind in [(1, (3, 12, 1, 0), 1), (2, (4, 10, 2, 1), 0), (3, (3, 8, 1, 2), 1)]:
         P = make_problem(seed, nposes, npts, nfp, nfpt)
-        F0, steps = lm_step(P, kind, 1.0, [1e-3, 1e-1])
+        F0, steps = lm_step(P, kind, 1.0, [1e-3, 1e-1, 1e3])
         cases.append(dict(
             seed=seed, loss=kind, delta=1.0, initial_cost=F0, steps=steps,
             points=P["points"].tolist(), xy=P["xy"].tolist(), info=P["info"].tolist(),

@@ -16,6 +16,7 @@ import pytest
 from scipy.linalg import expm
 
 import cuvslam_amd as ca
+from lm_cases import reference_style_problem
 from oracle import sba_oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -209,36 +210,6 @@ def test_cost_zero_at_exact_observations_and_infinite_inside_near_plane():
     q.points[:, 2] = np.linspace(0.3, 0.8, q.points.shape[0], dtype=np.float32)
     c, sk = O.cost(q, loss=1)
     assert sk == 0 and np.isfinite(c)
-
-
-def reference_style_problem(seed, num_points=200, num_poses=5):
-    """The reference test generator's scene (sba_test.cpp:46-128): two-camera rig with a 0.125 m baseline,
-    poses Exp(omega ~ U(+-0.25), t ~ U(+-2)), points xy ~ U(+-4), z ~ U(2, 16), every (pose, point, camera)
-    with camera depth > 1 observed, identity information, one fixed pose."""
-    rng = np.random.default_rng(seed)
-    poses = []
-    for _ in range(num_poses):
-        tw = np.concatenate([rng.uniform(-0.25, 0.25, 3), rng.uniform(-2, 2, 3)])
-        poses.append(O.exp(tw).astype(np.float32))
-    pts = np.stack([rng.uniform(-4, 4, num_points), rng.uniform(-4, 4, num_points), rng.uniform(2, 16, num_points)], 1)
-    cams = np.stack([np.eye(4), np.eye(4)]).astype(np.float32)
-    cams[1, 0, 3] = 0.125
-    xy, pid, qid, cid = [], [], [], []
-    for j in range(num_poses):
-        for i in range(num_points):
-            for c in range(2):
-                q = cams[c].astype(np.float64) @ poses[j].astype(np.float64) @ np.append(pts[i], 1)
-                if q[2] <= 1.0:
-                    continue
-                xy.append(q[:2] / q[2])
-                pid.append(i)
-                qid.append(j)
-                cid.append(c)
-    n = len(xy)
-    return ca.BundleAdjustmentProblem(pts.astype(np.float32), np.array(xy, np.float32),
-                                      np.tile(np.eye(2, dtype=np.float32), (n, 1, 1)), np.array(pid, np.int32),
-                                      np.array(qid, np.int32), np.array(cid, np.int8), np.stack(poses), cams,
-                                      num_fixed_points=0, num_fixed_key_frames=1, max_iterations=50, robustifier_scale=1.0)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
