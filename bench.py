@@ -1,13 +1,22 @@
-"""Benchmark: ms per SBA solve (10 LM iterations) on one MI355X, relative cost error vs the FP64 oracle.
+"""Benchmark: batched SBA solves/s (10 LM iterations per window) on one MI355X, with ms per solve of the single
+30-key-frame window and the relative cost error vs the FP64 oracle.
 
-A step is one whole solve of the window from its initial state, with the inputs resident in HBM: the
-hipGraph of one solve (initial cost, 10 LM iterations, gather) replayed on the solver's stream.
+A step is one whole solve of every window of the batch from its initial state, with the inputs resident in HBM:
+the hipGraph of one solve (initial cost, 10 LM iterations, gather) replayed on the solver's stream.
 
-Default workload: 30 key frames (2 fixed), 20,000 points, pinhole 450 px, N(0, 0.5 px) noise, 5% outliers,
-Huber 1 px, exactly 10 LM iterations. `--workload c3` runs the batched form (512 uneven windows per step).
-`--workload c4` times the differentiable solve (20 key frames, 10,000 points, Student-t): forward LM solve, FP64
-Newton refinement, and the implicit backward to pixel and intrinsics gradients, which it checks against the
-FP64 oracle's adjoint at the same state and against FP64 central finite differences of converged oracle solves.
+Default workload (`--workload c3`): 512 independent windows, per window key frames U[5,40] (2 fixed), points
+U[500,8000], outlier rate U[0,10]%, each point observed by U{2..key frames} of the key frames it projects in,
+Huber 1 px, exactly 10 LM iterations; the work granularity is autotuned before the timed region. The same
+windows are also solved one window per launch sequence (the plain form) for the batched/plain ratio, and the
+30-key-frame / 20,000-point window is timed and checked against the FP64 oracle as an extra.
+`--workload c2` times that single window alone, `--workload c5` the 200-key-frame multi-camera map, and
+`--workload c4` the differentiable solve (20 key frames, 10,000 points, Student-t): forward LM solve, FP64
+Newton refinement, and the implicit backward to pixel and intrinsics gradients, checked against the FP64
+oracle's adjoint at the same state and against FP64 central finite differences of converged oracle solves.
+
+The CPU baseline is the oracle's FP32 restatement of the reference CPU bundler (one thread per window, like the
+reference), timed on this host: `--cpu-warmups` untimed runs, then the median of `--cpu-runs` timed runs, on a
+fixed sample of the workload's windows, both on one core and with one window per core on up to 16 cores.
 """
 from __future__ import annotations
 
@@ -40,6 +49,63 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def cpu_cores():
+    """Host threads for the one-window-per-core baseline: the logical CPUs of this process, at most 16 (the CPU
+    share of one GPU on an 8-GPU node)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def time_cpu(fn, warmups, runs):
+    """Median, min and max seconds of fn() over `runs` timed runs after `warmups` untimed ones."""
+    for _ in range(warmups):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), float(np.min(ts)), float(np.max(ts))
+
+
+def cpu_baseline_windows(ca, problems, args):
+    """The FP32 restatement of SchurComplementBundlerCpu::solve on a fixed sample of the windows: (i) one core,
+    the sample's windows one after another; (ii) one window per core, `cores` windows at once."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import sba_oracle as O
+
+    def solve(p):
+        O.solve(p, "f32", loss=ca.LOSS_HUBER, rcs_solver=1, dense_schur=1, fixed_iterations=1)
+
+    rng = np.random.default_rng(12345)
+    n_single = min(args.cpu_sample, len(problems))
+    cores = cpu_cores()
+    n_multi = min(cores, len(problems))
+    single = [problems[i].copy() for i in sorted(rng.choice(len(problems), n_single, replace=False))]
+    multi = [problems[i].copy() for i in sorted(rng.choice(len(problems), n_multi, replace=False))]
+    t1 = time_cpu(lambda: [solve(p) for p in single], args.cpu_warmups, args.cpu_runs)
+    with ThreadPoolExecutor(max_workers=cores) as pool:  # ctypes releases the GIL inside the oracle
+        tn = time_cpu(lambda: list(pool.map(solve, multi)), args.cpu_warmups, args.cpu_runs)
+    obs1 = sum(p.num_observations for p in single)
+    obsn = sum(p.num_observations for p in multi)
+    method = f"{args.cpu_warmups} warm-up runs, median of {args.cpu_runs} timed runs"
+    return {
+        "value": round(n_multi / tn[0], 4), "unit": "solves/s", "cores": cores, "kind": "port",
+        "sample": (f"FP32 restatement of the reference CPU bundler (SchurComplementBundlerCpu::solve: dense Schur "
+                   f"product, pseudo-inverse of the reduced system), 10 LM iterations per window, one window per "
+                   f"thread on {cores} threads: {n_multi} windows ({obsn} observations) of the batch at once; "
+                   f"{method}; {cpu_model()}, {os.cpu_count()} logical CPUs"),
+        "seconds_median_min_max": [round(x, 3) for x in tn],
+        "single_core": {"value": round(n_single / t1[0], 4), "unit": "solves/s", "cores": 1,
+                        "sample": f"{n_single} windows ({obs1} observations) one after another on one thread; {method}",
+                        "seconds_median_min_max": [round(x, 3) for x in t1]},
+    }
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed counter passes (profiles/hbm_traffic_pmc.txt), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.txt")
@@ -59,12 +125,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--cpu-warmups", type=int, default=1)
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=4, help="windows of the one-core CPU baseline (c3)")
+    ap.add_argument("--no-extra-c2", action="store_true", help="c3: skip the single 30-key-frame window extra")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--chunk-pairs", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune-table", default=os.path.join(ROOT, "cuvslam_amd", "autotune_table.txt"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy kernel (k_copy_calibrate), the reference of HBM byte counters")
     args = ap.parse_args()
@@ -89,23 +160,27 @@ def main():
                     "clutter, 3% outliers, Huber 1 px; reduced system 1194 x 1194")
     else:
         problems = ca.scene.batched_windows(512, seed=3, max_iterations=10)
-        workload = "512 windows: key frames U[5,40], points U[500,8000], outliers U[0,10]%, Huber 1 px"
+        workload = ("512 windows in one batch: key frames U[5,40] (2 fixed), points U[500,8000], each point observed by "
+                    "U{2..key frames} of the key frames it projects in, outliers U[0,10]%, 0.5 px noise, Huber 1 px, "
+                    "10 LM iterations")
     pristine = [p.copy() for p in problems]
     n_obs = sum(p.num_observations for p in problems)
     log(f"[bench] generated {len(problems)} window(s), {n_obs} observations in {time.time() - t0:.1f}s")
 
-    chunk_pairs = args.chunk_pairs
+    chunk_pairs, bchunk_pairs = args.chunk_pairs, 0
     tuned = None
     if args.workload == "c3" and chunk_pairs == 0 and not args.no_autotune:
-        # work granularity of the batched launch, picked by the autotuner before the timed region
+        # work granularity of the batched launch, from the autotune table (tuned before the timed region and added
+        # to the table when this batch's shape class is not in it)
         t0 = time.time()
         tuner = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
-        cands = [750, 1500, 3000, 6000, 12000]
-        chunk_pairs, ms = tuner.autotune(problems, cands, repeats=1)
+        chunk_pairs, bchunk_pairs, hit = tuner.autotune_table(problems, args.autotune_table, repeats=1)
         tuner.close()
-        tuned = {"candidates": cands, "ms_per_solve_batch": [round(m, 3) for m in ms], "chosen": chunk_pairs}
-        log(f"[bench] autotuned chunk_pairs in {time.time() - t0:.1f}s: {tuned}")
-    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=chunk_pairs)
+        tuned = {"chunk_pairs": chunk_pairs, "bchunk_pairs": bchunk_pairs, "from_table": hit,
+                 "table": os.path.relpath(args.autotune_table, ROOT)}
+        log(f"[bench] autotune in {time.time() - t0:.1f}s: {tuned}")
+    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True, chunk_pairs=chunk_pairs,
+                                          bchunk_pairs=bchunk_pairs)
     t0 = time.time()
     batch = solver.batch(problems)
     info = batch.info()
@@ -247,7 +322,10 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import sba_oracle as O
         sample = pristine[0]
-        if args.workload == "c5":
+        if args.workload == "c3":
+            cpu = cpu_baseline_windows(ca, pristine, args)
+            log(f"[bench] CPU baseline: {cpu}")
+        elif args.workload == "c5":
             # the reference's dense Schur product is 1.7 TFLOP per iteration at this size: the baseline is the
             # FP32 restatement with the block-sparse Schur product, one LM iteration timed and scaled to ten
             one = sample.copy()
@@ -257,15 +335,22 @@ def main():
             t_cpu = 10 * (time.perf_counter() - t0)
             what = ("one LM iteration of the map with the FP32 restatement of the reference CPU bundler "
                     "(block-sparse Schur product, Cholesky solve of the reduced system), x10")
+            cpu = {"value": round(1.0 / t_cpu, 6), "unit": "solves/s", "cores": 1, "kind": "port",
+                   "sample": f"{what}, single thread, {cpu_model()}", "seconds": round(t_cpu, 3)}
         else:
-            t0 = time.perf_counter()
-            O.solve(sample, "f32", loss=ca.LOSS_HUBER, rcs_solver=1, dense_schur=1, fixed_iterations=1)
-            t_cpu = time.perf_counter() - t0
-            what = (f"one full solve (10 LM iterations) of window 0 ({sample.num_observations} observations) "
-                    f"with the FP32 restatement of the reference CPU bundler")
-        cpu = {"value": round(1.0 / t_cpu, 6), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"{what}, single thread, {cpu_model()}", "seconds": round(t_cpu, 3)}
-        log(f"[bench] CPU baseline {t_cpu:.2f}s per solve")
+            med = time_cpu(lambda: O.solve(sample, "f32", loss=ca.LOSS_HUBER, rcs_solver=1, dense_schur=1,
+                                           fixed_iterations=1), args.cpu_warmups, args.cpu_runs)
+            what = (f"full solve (10 LM iterations) of the window ({sample.num_observations} observations) with the "
+                    f"FP32 restatement of the reference CPU bundler, single thread, {args.cpu_warmups} warm-up runs, "
+                    f"median of {args.cpu_runs} timed runs, {cpu_model()}")
+            cpu = {"value": round(1.0 / med[0], 6), "unit": "solves/s", "cores": 1, "kind": "port",
+                   "sample": what, "seconds_median_min_max": [round(x, 3) for x in med]}
+        log(f"[bench] CPU baseline {cpu['value']} solves/s")
+
+    if args.workload == "c3" and not args.no_extra_c2:
+        extra["single_window_30kf"] = single_window_extra(ca, RF, args)
+    if args.workload in ("c2", "c3"):
+        extra["production_window_7kf"] = production_window_extra(ca, args)
 
     line = {
         "metric": "ms per BA solve (10 LM iters) + batched solves/s; rel. cost err vs FP64 oracle",
@@ -289,6 +374,76 @@ def main():
         "accuracy": extra,
     }
     print(json.dumps(line), flush=True)
+
+
+def single_window_extra(ca, RF, args):
+    """The 30-key-frame / 20,000-point window (528k observations) alone: ms per solve with inputs resident in HBM,
+    per-kernel launch times, and the relative cost error vs the FP64 oracle after exactly 10 LM iterations."""
+    p, _, _ = ca.scene.generate("c2", seed=2, num_fixed_keyframes=2, max_iterations=10)
+    pristine = p.copy()
+    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    batch = solver.batch([p])
+    for _ in range(args.warmup):
+        batch.run()
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    batch.sync()
+    ms_solve = 1e3 * (time.perf_counter() - t0) / args.steps
+    ms, launches = batch.kernel_times()
+    names = ["k_schur", "k_reduce", "k_factor", "k_backsub", "k_decide"]
+    per_launch = {n: round(m / max(l, 1), 5) for n, m, l in zip(names, ms, launches)}
+    batch.read()
+    out = {"workload": "30 key frames (2 fixed), 20000 points, 0.5 px noise, 5% outliers, Huber 1 px",
+           "observations": p.num_observations, "ms_per_solve": round(ms_solve, 4),
+           "kernels_ms_per_launch": per_launch, "iterations": int(p.iterations), "accepted": int(p.accepted)}
+    f64 = RF.work([pristine])["factor"][2]
+    out["k_factor_fp64_frac"] = round(f64 / (per_launch["k_factor"] * 1e-3) / 1e12 / RF.FP64_PEAK_TFLOPS, 6)
+    if not args.no_oracle:
+        from oracle import sba_oracle as O
+        ref = O.solve(pristine, "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
+        out["rel_cost_err_vs_fp64_oracle"] = abs(p.last_cost_f64 - ref["last_cost"]) / ref["last_cost"]
+    q = pristine.copy()
+    t0 = time.perf_counter()
+    solver.solve(q)
+    out["end_to_end_ms_host_buffers"] = round(1e3 * (time.perf_counter() - t0), 3)
+    batch.close()
+    solver.close()
+    log(f"[bench] single 30-key-frame window: {out}")
+    return out
+
+
+def production_window_extra(ca, args):
+    """The reference's production window (sba_config.h:27-46: 7 key frames of which 3 fixed, 7 LM iterations, the
+    Student-t loss with scale 0.5, the early exit on), 2,000 points: the drop-in solve from host buffers (pack,
+    upload, solve, read back through the handle's kept workspace) and the device-only solve, median of 50 runs."""
+    p, _, _ = ca.scene.generate(seed=7, num_points=2000, num_keyframes=7, num_fixed_keyframes=3, max_iterations=7)
+    p.robustifier_scale = 0.5
+    solver = ca.SchurComplementBundlerGpu(loss=ca.LOSS_STUDENT_T)
+    host = []
+    for k in range(55):
+        q = p.copy()
+        t0 = time.perf_counter()
+        solver.solve(q)
+        if k >= 5:
+            host.append(time.perf_counter() - t0)
+    batch = solver.batch([p])
+    dev = []
+    for k in range(55):
+        t0 = time.perf_counter()
+        batch.run()
+        batch.sync()
+        if k >= 5:
+            dev.append(time.perf_counter() - t0)
+    batch.close()
+    solver.close()
+    out = {"workload": "7 key frames (3 fixed), 2000 points, Student-t 0.5, 7 LM iterations, early exit on",
+           "observations": p.num_observations, "iterations": int(q.iterations),
+           "ms_host_buffers_median": round(1e3 * float(np.median(host)), 4),
+           "ms_device_resident_median": round(1e3 * float(np.median(dev)), 4)}
+    log(f"[bench] production window: {out}")
+    return out
 
 
 def run_c4(args, ca):

@@ -37,7 +37,8 @@ class Result(C.Structure):
 class Options(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("loss", C.c_int32), ("fixed_iterations", C.c_int32), ("use_graph", C.c_int32),
-        ("chunk_pairs", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("chunk_pairs", C.c_int32), ("pack_threads", C.c_int32), ("bchunk_pairs", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -86,8 +87,10 @@ SIGNATURES = {
     "sba_batch_winsorize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint8),
                                       C.POINTER(C.c_int32)]),
     "sba_debug_factor": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
+    "sba_batch_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _fp, _ip, _ip]),
     "sba_batch_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
     "sba_autotune_chunks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _ip, C.c_int32, C.c_int32, _ip, _fp]),
+    "sba_autotune_table": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, _ip, _ip, _ip]),
     "sba_loop_default_config": (None, [C.POINTER(LoopConfig)]),
     "sba_scene_generate_loop": (C.c_int, [C.POINTER(LoopConfig), C.POINTER(C.c_void_p)]),
     "sba_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp, _dp,

@@ -22,8 +22,7 @@
 //                            damped point-block pseudo-inverse) as upper 16x16 tiles in matrix-core lane
 //                            order, then U (21 upper entries per free key frame), e (6), diag U (6)
 //                   sred     the window's records summed in chunk order (deterministic)
-//                   A        FP64 Cholesky workspace (packed lower triangle) for windows whose factor
-//                            does not fit in LDS
+//                   A        FP64 Cholesky workspace of mapped windows (more than kMaxFreePoses free key frames)
 //                   dc       FP32 pose step (6 per free key frame)
 // Windows carry their offsets into every stream, so one launch covers all windows.
 #pragma once
@@ -53,26 +52,41 @@ struct WindowDesc {
   int32_t nchunks;   // Schur records (chunks) of the window
   int64_t part_off;  // floats into part: the window's first chunk record
   int64_t sred_off;  // floats into sred: the window's reduced record
-  int64_t a_off;     // doubles into A (only when the factor lives in global memory)
+  int64_t a_off;     // doubles into A (mapped windows)
   int32_t dc_off;    // floats into dc
   int32_t max_iterations;
   float robust;
-  int32_t factor_in_lds;
   // windows with more than kMaxFreePoses free key frames ("mapped"): each chunk stages its own <= kMaxFreePoses
   // free key frames; chunk records are scattered into a dense FP32 sum (big) and a dense FP64 factor in A
   int32_t mapped;
   int32_t chunk_begin;  // the window's first Schur chunk
   int64_t big_off;      // floats into big: sum Z Z^T, n x n row-major, upper triangle (n = 6 ncf)
   int64_t ue_off;       // floats into big_ue: U (21 ncf), e (6 ncf), diag U (6 ncf)
+  int32_t bchunk_begin, nbchunks;  // the window's back-substitution chunks (and their CostPart slots)
 };
 
 struct WindowState {
   double current_cost, initial_cost, last_cost;
-  double acc_cost, acc_hess, acc_scal_pt, scal_pose;
+  double scal_pose;  // d_c^T diag(U) d_c of the last step (the factor kernel)
   float lambda, maxdc;
-  uint32_t acc_maxdp;  // float bits of a non-negative max
-  int32_t acc_skipped;
   int32_t chol_ok, done, status, iterations, accepted, cur;
+};
+
+// One back-substitution chunk's share of a window's trial cost and predicted-reduction terms. Each chunk writes its
+// own slot; the decision kernel sums a window's slots in chunk order, so the totals do not depend on scheduling.
+struct CostPart {
+  double cost, hess, scal;  // FP64 trial cost, d^T H d and d^T diag(V) d of the chunk's observations and points
+  int32_t skipped;          // observations inside the near plane
+  float maxdp;              // max |dp| over the chunk's points
+};
+
+// One LM iteration of a window as the decision kernel saw it (the first kTrace iterations are kept).
+constexpr int kTrace = 64;
+struct TraceEntry {
+  double rho;         // gain ratio (0 when no ratio was formed)
+  double trial_cost;  // FP64 cost of the trial state (0 when the factorization failed)
+  float lambda;       // damping the iteration's step was solved with
+  int32_t flags;      // 1 accepted, 2 factorization failed, 4 a stop test ended the solve
   int32_t pad[2];
 };
 
@@ -130,9 +144,10 @@ struct Batch {
   float4* out_pts;
   float4* out_poses;
   float4* dbg_dp;    // optional: the point step of the last back-substitution (phase-level tests)
+  TraceEntry* trace;  // optional: kTrace entries per window
+  CostPart* bpart;    // one slot per back-substitution chunk
   // shared-memory layout of the Schur kernel (offsets in 4-byte words)
   int32_t lds_pose, lds_cam, lds_acc, lds_ent, lds_ec, lds_w, lds_pts, lds_zc, lds_words;
-  int32_t factor_lds_words;  // LDS words of the factor kernel
 };
 
 // Orthonormal polar factor of a near-rotation (Newton iteration X <- (X + X^-T) / 2, Higham 1986); for a

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include <type_traits>
@@ -1310,34 +1312,54 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
       sk += sRedI[k];
       m = fmaxf(m, sRedM[k]);
     }
-    unsafeAtomicAdd(&St->acc_cost, c);
-    if (!INITIAL) {
-      unsafeAtomicAdd(&St->acc_hess, h);
-      unsafeAtomicAdd(&St->acc_scal_pt, s);
-      atomicMax(&St->acc_maxdp, __float_as_uint(m));
-    }
-    atomicAdd(&St->acc_skipped, sk);
+    b.bpart[blockIdx.x] = CostPart{c, INITIAL ? 0.0 : h, INITIAL ? 0.0 : s, sk, INITIAL ? 0.f : m};
   }
 }
 
 // ---------------------------------------------------------------------------------------------------------
 // LM decision per window (schur_complement_bundler_gpu.cpp:147-253). INITIAL: the setup checks (:149-173).
 // ---------------------------------------------------------------------------------------------------------
+// One wavefront per window: lane l sums the window's chunk slots l, l + 64, ... in order, then a fixed butterfly
+// adds the lanes (the same bits on every run); lane 0 decides.
 template <bool INITIAL>
-__global__ void k_decide(Batch b) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= b.n_windows) return;
+__global__ __launch_bounds__(64) void k_decide(Batch b) {
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x;
   const WindowDesc W = b.win[w];
   WindowState& S = b.st[w];
   if (S.done) return;
-  const double cost = S.acc_skipped == W.no ? static_cast<double>(INFINITY) : S.acc_cost;
-  const double hess = S.acc_hess, scal_pt = S.acc_scal_pt;
-  const float maxdp = __uint_as_float(S.acc_maxdp);
-  S.acc_cost = 0.0;
-  S.acc_hess = 0.0;
-  S.acc_scal_pt = 0.0;
-  S.acc_skipped = 0;
-  S.acc_maxdp = 0u;
+  if (!INITIAL && !S.chol_ok) {  // no trial state was evaluated
+    if (lane == 0) {
+      S.iterations += 1;
+      TraceEntry* tr = b.trace && S.iterations <= kTrace ? b.trace + static_cast<int64_t>(w) * kTrace + S.iterations - 1
+                                                         : nullptr;
+      if (tr) *tr = TraceEntry{0.0, 0.0, S.lambda, 2, {0, 0}};
+      S.lambda *= 5.f;
+      if (S.iterations >= W.max_iterations) S.done = 1;
+    }
+    return;
+  }
+  double c = 0.0, h = 0.0, sp = 0.0;
+  int sk = 0;
+  float mdp = 0.f;
+  for (int k = lane; k < W.nbchunks; k += 64) {
+    const CostPart q = b.bpart[W.bchunk_begin + k];
+    c += q.cost;
+    h += q.hess;
+    sp += q.scal;
+    sk += q.skipped;
+    mdp = fmaxf(mdp, q.maxdp);
+  }
+  c = wave_sum(c);
+  h = wave_sum(h);
+  sp = wave_sum(sp);
+  sk = wave_sum(sk);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mdp = fmaxf(mdp, __shfl_xor(mdp, o));
+  if (lane != 0) return;
+  const double cost = sk == W.no ? static_cast<double>(INFINITY) : c;
+  const double hess = h, scal_pt = sp;
+  const float maxdp = mdp;
   if (INITIAL) {
     S.initial_cost = S.current_cost = S.last_cost = cost;
     S.lambda = 0.001f;
@@ -1354,28 +1376,30 @@ __global__ void k_decide(Batch b) {
   }
   S.iterations += 1;
   const bool last = S.iterations >= W.max_iterations;
-  if (!S.chol_ok) {
-    S.lambda *= 5.f;
-    if (last) S.done = 1;
-    return;
-  }
+  TraceEntry* tr = b.trace && S.iterations <= kTrace ? b.trace + static_cast<int64_t>(w) * kTrace + S.iterations - 1
+                                                     : nullptr;
+  if (tr) *tr = TraceEntry{0.0, cost, S.lambda, 0, {0, 0}};
   if (S.current_cost < S.initial_cost * static_cast<double>(kFltEps)) {
     S.done = 1;
+    if (tr) tr->flags = 4;
     return;
   }
   const double lam = static_cast<double>(S.lambda);
   const double pred = (hess + 2.0 * lam * (S.scal_pose + scal_pt)) / S.current_cost;
   if (!b.fixed_iterations && pred < static_cast<double>(kSqrtFltEps) && S.maxdc < kSqrtFltEps && maxdp < kSqrtFltEps) {
     S.done = 1;
+    if (tr) tr->flags = 4;
     return;
   }
   const double rho = (1.0 - cost / S.current_cost) / pred;
+  if (tr) tr->rho = rho;
   if (rho > 0.25) {
     if (rho > 0.75 && S.lambda * 0.125f > 0.f) S.lambda *= 0.125f;
     S.current_cost = cost;
     S.last_cost = cost;
     S.cur ^= 1;
     S.accepted += 1;
+    if (tr) tr->flags = 1;
   } else {
     S.lambda *= 5.f;
   }
@@ -1420,6 +1444,16 @@ __global__ void k_mfma_layout(const float* A, const float* B, float* D) {
   for (int v = 0; v < 4; ++v) D[(4 * (l / 16) + v) * 16 + l % 16] = acc[v];
 }
 
+// Clears a 256-byte aligned device range in 16-byte stores (a kernel rather than a memset node, so that a solve's
+// set-up can sit inside a caller's stream capture). The tail below 16 bytes is cleared one byte per thread.
+__global__ __launch_bounds__(256) void k_fill_zero(uint4* __restrict__ p, int64_t n16, uint8_t* __restrict__ tail,
+                                                   int32_t ntail) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t i = gid; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (gid < ntail) tail[gid] = 0;
+}
+
 // A plain streaming copy of a known size, the calibration of HBM byte counters.
 __global__ __launch_bounds__(256) void k_copy_calibrate(float4* __restrict__ dst, const float4* __restrict__ src, int64_t n) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -1461,11 +1495,10 @@ hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s) {
 }
 
 hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s) {
-  const int blocks = (b.n_windows + 63) / 64;
   if (initial)
-    hipLaunchKernelGGL(k_decide<true>, dim3(blocks), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_decide<true>, dim3(b.n_windows), dim3(64), 0, s, b);
   else
-    hipLaunchKernelGGL(k_decide<false>, dim3(blocks), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_decide<false>, dim3(b.n_windows), dim3(64), 0, s, b);
   return hipGetLastError();
 }
 
@@ -1478,6 +1511,15 @@ hipError_t launch_gather(const Batch& b, hipStream_t s) {
 // runtime's last-error slot so that a stale error is not reported by the first launch of a handle.
 hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_calibrate, dim3(2048), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+  const int64_t n16 = static_cast<int64_t>(bytes / 16);
+  const int32_t ntail = static_cast<int32_t>(bytes % 16);
+  const int64_t blocks = std::min<int64_t>(std::max<int64_t>((n16 + 255) / 256, 1), 4096);
+  hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, static_cast<uint4*>(p), n16,
+                     static_cast<uint8_t*>(p) + 16 * n16, ntail);
   return hipGetLastError();
 }
 

@@ -11,11 +11,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
 #include <type_traits>
 #include <vector>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "../../include/cuvslam_sba.h"
 #include "sba_device.h"
@@ -44,16 +48,51 @@ hipError_t launch_decide(const Batch& b, bool initial, hipStream_t s);
 hipError_t launch_gather(const Batch& b, hipStream_t s);
 hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s);
 hipError_t launch_mfma_layout(const float* A, const float* B, float* D, hipStream_t s);
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t set_kernel_attributes();
 }  // namespace sba_dev
 
 using namespace sba_dev;
+
+// The host side of a batch while it is packed. A handle keeps one and clears it per batch, so that its vectors keep
+// their capacity across calls: a repeated solve writes into memory that is already mapped (first-touch page faults
+// of fresh buffers cost as much as the packing itself).
+struct HostPack {
+  std::vector<float2> obs_xy;
+  std::vector<float4> obs_info;
+  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
+  std::vector<int4> pair_rec;
+  std::vector<float2> pair_xy;
+  std::vector<float4> pair_info;
+  std::vector<float4> pts, poses, cams;
+  std::vector<Tile> tiles, btiles;
+  // per-window scratch of the packing
+  std::vector<int32_t> cnt, start, order, npair, nfree, first_pair;
+  void clear() {
+    for (auto* v : {&pair_obs, &pair_pose, &point_pair, &point_nfree, &obs_orig}) v->clear();
+    obs_xy.clear();
+    obs_info.clear();
+    pair_rec.clear();
+    pair_xy.clear();
+    pair_info.clear();
+    pts.clear();
+    poses.clear();
+    cams.clear();
+    tiles.clear();
+    btiles.clear();
+  }
+};
 
 struct sba_handle {
   sba_options opt{};
   hipStream_t stream = nullptr;
   std::string err;
   bool attrs_set = false;
+  // device workspace of sba_solve, kept across calls and regrown to twice the need (the reference bundler keeps
+  // its buffers the same way, schur_complement_bundler_gpu.cpp:92-99)
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  HostPack pack;  // host packing buffers, kept across calls
 };
 
 struct sba_batch {
@@ -71,12 +110,21 @@ struct sba_batch {
   std::vector<int> mapped;      // windows with more than kMaxFreePoses free key frames
   double* adj = nullptr;        // adjoint workspace of window 0 (allocated on first use)
   bool adj_ready = false;       // the workspace holds the factor and the FP64 state of the current solution
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[8] = {};        // kernel-class timing (created on first use)
   float class_ms[5] = {0, 0, 0, 0, 0};
   int class_launches[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
+
+// A roctx range over a host phase of the solver (the reference traces the same phases,
+// schur_complement_bundler_gpu.cpp:116,176); free when no profiler is attached.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 int fail(sba_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -89,17 +137,6 @@ int fail(sba_handle* h, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(h, SBA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-struct HostPack {
-  std::vector<float2> obs_xy;
-  std::vector<float4> obs_info;
-  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
-  std::vector<int4> pair_rec;
-  std::vector<float2> pair_xy;
-  std::vector<float4> pair_info;
-  std::vector<float4> pts, poses, cams;
-  std::vector<Tile> tiles, btiles;
-};
-
 // Room for `extra` more elements with geometric growth: an exact reserve per window would copy every earlier window
 // again (quadratic in the batch).
 template <class V>
@@ -107,7 +144,29 @@ void reserve_more(V& v, size_t extra) {
   if (v.size() + extra > v.capacity()) v.reserve(std::max(v.size() + extra, 2 * v.capacity()));
 }
 
-// Validates one window and appends it to the packed streams.
+// Runs fn(t, begin, end) over [0, n) cut into `nthreads` contiguous ranges, range t on its own thread (range 0 on
+// the caller's).
+template <class F>
+void parallel_ranges(int nthreads, int64_t n, const F& fn) {
+  if (nthreads <= 1 || n < 2 * nthreads) {
+    fn(0, int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(fn, t, n * t / nthreads, n * (t + 1) / nthreads);
+  fn(0, int64_t(0), n / nthreads);
+  for (auto& th : pool) th.join();
+}
+
+// Host threads that pack one window: the option when set, else up to 16 for windows of 65536 observations or more.
+int pack_threads(const sba_handle* h, int no) {
+  if (h->opt.pack_threads > 0) return h->opt.pack_threads;
+  if (no < (1 << 16)) return 1;
+  return static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+}
+
+// Validates one window and appends it to the packed streams. The phases over observations and points run on
+// pack_threads() host threads; every output element has one writer, so the result does not depend on the count.
 int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, WindowDesc& W) {
   const int np = p.num_points, nc = p.num_poses, no = p.num_observations, ncam = p.num_cameras;
   if (np < 0 || nc < 0 || no < 0 || ncam < 0) return fail(h, SBA_E_INVALID, "negative size in window " + std::to_string(w));
@@ -119,11 +178,16 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   const int ncf = nc - p.num_fixed_key_frames;
   if (nc > kMaxPoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 256 key frames");
   if (ncam > kMaxCameras) return fail(h, SBA_E_CAPACITY, "more than 32 cameras");
-  for (int o = 0; o < no; ++o) {
-    if (p.point_ids[o] < 0 || p.point_ids[o] >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 ||
-        p.camera_ids[o] >= ncam)
-      return fail(h, SBA_E_INVALID, "observation index out of range in window " + std::to_string(w));
-  }
+  const int T = pack_threads(h, no);
+  std::vector<uint8_t> bad(T, 0);
+  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+    for (int64_t o = a; o < e; ++o)
+      if (p.point_ids[o] < 0 || p.point_ids[o] >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 ||
+          p.camera_ids[o] >= ncam)
+        bad[t] = 1;
+  });
+  if (std::find(bad.begin(), bad.end(), 1) != bad.end())
+    return fail(h, SBA_E_INVALID, "observation index out of range in window " + std::to_string(w));
   W = WindowDesc{};
   W.np = np;
   W.npf = np - p.num_fixed_points;
@@ -140,30 +204,60 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   W.robust = p.robustifier_scale;
   W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
-  // order (point, key frame, camera, input index): a stable counting sort by (key frame, camera) of the input
-  // order, then a stable counting sort by point
-  std::vector<int32_t> by_kc(no);
+  // order (point, key frame, camera, input index): a stable counting sort by point (per-thread histograms, then
+  // each thread scatters its own input range), then within each point a stable sort by (key frame, camera)
+  const int64_t stride = static_cast<int64_t>(np) + 1;
+  std::vector<int32_t>& cnt = H.cnt;
+  cnt.assign(static_cast<size_t>(T) * stride, 0);
+  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+    int32_t* c = cnt.data() + t * stride;
+    for (int64_t o = a; o < e; ++o) ++c[p.point_ids[o]];
+  });
+  std::vector<int32_t>& start = H.start;
+  start.assign(np + 1, 0);
   {
-    std::vector<int32_t> cnt(static_cast<size_t>(nc) * ncam + 1, 0);
-    for (int o = 0; o < no; ++o) ++cnt[p.pose_ids[o] * ncam + p.camera_ids[o] + 1];
-    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
-    for (int o = 0; o < no; ++o) by_kc[cnt[p.pose_ids[o] * ncam + p.camera_ids[o]]++] = o;
+    int32_t run = 0;
+    for (int i = 0; i < np; ++i) {
+      start[i] = run;
+      for (int t = 0; t < T; ++t) {
+        const int32_t c = cnt[t * stride + i];
+        cnt[t * stride + i] = run;  // thread t's next slot for point i
+        run += c;
+      }
+    }
+    start[np] = run;
   }
-  std::vector<int32_t> start(np + 1, 0);
-  for (int o = 0; o < no; ++o) ++start[p.point_ids[o] + 1];
-  for (int i = 0; i < np; ++i) start[i + 1] += start[i];
-  std::vector<int32_t> order(no), fill(start.begin(), start.end() - 1);
-  for (int k = 0; k < no; ++k) {
-    const int o = by_kc[k];
-    order[fill[p.point_ids[o]]++] = o;
-  }
+  std::vector<int32_t>& order = H.order;
+  order.resize(no);
+  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+    int32_t* c = cnt.data() + t * stride;
+    for (int64_t o = a; o < e; ++o) order[c[p.point_ids[o]]++] = static_cast<int32_t>(o);
+  });
+  auto kc = [&](int32_t o) { return p.pose_ids[o] * ncam + p.camera_ids[o]; };
+  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+    for (int64_t i = a; i < e; ++i) {
+      int32_t* seg = order.data() + start[i];
+      const int m = start[i + 1] - start[i];
+      if (m <= 48) {  // insertion sort: stable, linear on segments already in key-frame order
+        for (int x = 1; x < m; ++x) {
+          const int32_t v = seg[x];
+          const int kv = kc(v);
+          int y = x;
+          for (; y > 0 && kc(seg[y - 1]) > kv; --y) seg[y] = seg[y - 1];
+          seg[y] = v;
+        }
+      } else {
+        std::stable_sort(seg, seg + m, [&](int32_t u, int32_t v) { return kc(u) < kc(v); });
+      }
+    }
+  });
   const int obs_base = static_cast<int>(H.obs_xy.size());
   H.obs_orig.resize(obs_base + no);
   H.obs_xy.resize(obs_base + no);
   H.obs_info.resize(obs_base + no);
-  // the gather into sorted order reads the input at random: split over host threads (disjoint output ranges)
-  auto gather = [&](int k_begin, int k_end) {
-    for (int k = k_begin; k < k_end; ++k) {
+  // the gather into sorted order reads the input at random (disjoint output ranges per thread)
+  parallel_ranges(T, no, [&](int, int64_t a, int64_t e) {
+    for (int64_t k = a; k < e; ++k) {
       const int o = order[k];
       H.obs_orig[obs_base + k] = o;
       H.obs_xy[obs_base + k] = make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]);
@@ -176,38 +270,59 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       std::memcpy(&inf.w, &cam, 4);
       H.obs_info[obs_base + k] = inf;
     }
-  };
-  const int nthreads = no < (1 << 16) ? 1 : static_cast<int>(std::min(8u, std::max(1u, std::thread::hardware_concurrency())));
-  if (nthreads == 1) {
-    gather(0, no);
-  } else {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nthreads; ++t)
-      pool.emplace_back(gather, static_cast<int>(static_cast<int64_t>(no) * t / nthreads),
-                        static_cast<int>(static_cast<int64_t>(no) * (t + 1) / nthreads));
-    for (auto& th : pool) th.join();
-  }
-  reserve_more(H.pair_obs, no);
-  reserve_more(H.pair_pose, no);
+  });
+  // pairs: one per run of equal key frames in a point's segment; counted per point, then written at the point's
+  // offset (prefix over points)
+  std::vector<int32_t>&npair = H.npair, &nfree = H.nfree;
+  npair.resize(np);
+  nfree.resize(np);
+  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+    for (int64_t i = a; i < e; ++i) {
+      int c = 0, f = 0, last = -1;
+      for (int k = start[i]; k < start[i + 1]; ++k) {
+        const int pose = p.pose_ids[order[k]];
+        if (pose != last) {
+          ++c;
+          f += pose < ncf;
+          last = pose;
+        }
+      }
+      npair[i] = c;
+      nfree[i] = f;
+    }
+  });
+  const size_t pair_base = H.pair_pose.size(), point_base = H.point_pair.size();
+  std::vector<int32_t>& first_pair = H.first_pair;
+  first_pair.assign(np + 1, 0);
+  for (int i = 0; i < np; ++i) first_pair[i + 1] = first_pair[i] + npair[i];
+  reserve_more(H.pair_obs, static_cast<size_t>(first_pair[np]));
+  reserve_more(H.pair_pose, static_cast<size_t>(first_pair[np]));
+  H.pair_obs.resize(pair_base + first_pair[np]);
+  H.pair_pose.resize(pair_base + first_pair[np]);
   reserve_more(H.point_pair, np + 1);
   reserve_more(H.point_nfree, np);
   reserve_more(H.pts, np);
-  for (int i = 0; i < np; ++i) {
-    H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
-    int nfree = 0;
-    int last_pose = -1;
-    for (int k = start[i]; k < start[i + 1]; ++k) {
-      const int pose = p.pose_ids[order[k]];
-      if (pose != last_pose) {
-        H.pair_obs.push_back(obs_base + k);
-        H.pair_pose.push_back(pose);
-        if (pose < ncf) ++nfree;
-        last_pose = pose;
+  H.point_pair.resize(point_base + np);
+  H.point_nfree.resize(point_base + np);
+  H.pts.resize(point_base + np);
+  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+    for (int64_t i = a; i < e; ++i) {
+      size_t q = pair_base + first_pair[i];
+      H.point_pair[point_base + i] = static_cast<int32_t>(q);
+      H.point_nfree[point_base + i] = nfree[i];
+      H.pts[point_base + i] = make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f);
+      int last = -1;
+      for (int k = start[i]; k < start[i + 1]; ++k) {
+        const int pose = p.pose_ids[order[k]];
+        if (pose != last) {
+          H.pair_obs[q] = obs_base + k;
+          H.pair_pose[q] = pose;
+          ++q;
+          last = pose;
+        }
       }
     }
-    H.point_nfree.push_back(nfree);
-    H.pts.push_back(make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f));
-  }
+  });
   auto push_pose = [](std::vector<float4>& dst, const float* m) {
     for (int r = 0; r < 3; ++r) dst.push_back(make_float4(m[r], m[4 + r], m[8 + r], m[12 + r]));
   };
@@ -260,30 +375,34 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   make_tiles(H.btiles, kBlock);
   // pair records: observation range, key frame | point index inside its back-substitution tile << 8, point
   // index inside its Schur tile | local key frame << 8; first observation by pair
-  const size_t first_pair = H.pair_rec.size();
   H.pair_rec.resize(H.pair_pose.size());
-  for (size_t t = first_tile; t < H.tiles.size(); ++t) {
-    const Tile& T = H.tiles[t];
-    for (int i = T.point_begin; i < T.point_end; ++i) {
-      const int a = H.point_pair[i];
-      const int e = i + 1 < static_cast<int>(H.point_pair.size()) ? H.point_pair[i + 1] : static_cast<int>(H.pair_pose.size());
-      for (int q = a; q < e; ++q) {
-        const int ob = H.pair_obs[q];
-        const int oe = q + 1 < static_cast<int>(H.pair_obs.size()) ? H.pair_obs[q + 1] : static_cast<int>(H.obs_xy.size());
-        // local free key frame index: the key frame itself unless the window is mapped (set with its chunks)
-        const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
-        H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - T.point_begin) | (jl << 8));
+  const int npoint_all = static_cast<int>(H.point_pair.size());
+  const int npair_all = static_cast<int>(H.pair_pose.size()), nobs_all = static_cast<int>(H.obs_xy.size());
+  parallel_ranges(T, static_cast<int64_t>(H.tiles.size() - first_tile), [&](int, int64_t a, int64_t e) {
+    for (int64_t t = a; t < e; ++t) {
+      const Tile& Tl = H.tiles[first_tile + t];
+      for (int i = Tl.point_begin; i < Tl.point_end; ++i) {
+        const int qa = H.point_pair[i];
+        const int qe = i + 1 < npoint_all ? H.point_pair[i + 1] : npair_all;
+        for (int q = qa; q < qe; ++q) {
+          const int ob = H.pair_obs[q];
+          const int oe = q + 1 < npair_all ? H.pair_obs[q + 1] : nobs_all;
+          // local free key frame index: the key frame itself unless the window is mapped (set with its chunks)
+          const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
+          H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - Tl.point_begin) | (jl << 8));
+        }
       }
     }
-  }
-  for (size_t t = first_btile; t < H.btiles.size(); ++t) {
-    const Tile& T = H.btiles[t];
-    for (int i = T.point_begin; i < T.point_end; ++i) {
-      const int e = i + 1 < static_cast<int>(H.point_pair.size()) ? H.point_pair[i + 1] : static_cast<int>(H.pair_pose.size());
-      for (int q = H.point_pair[i]; q < e; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - T.point_begin) << 8);
+  });
+  parallel_ranges(T, static_cast<int64_t>(H.btiles.size() - first_btile), [&](int, int64_t a, int64_t e) {
+    for (int64_t t = a; t < e; ++t) {
+      const Tile& Tl = H.btiles[first_btile + t];
+      for (int i = Tl.point_begin; i < Tl.point_end; ++i) {
+        const int qe = i + 1 < npoint_all ? H.point_pair[i + 1] : npair_all;
+        for (int q = H.point_pair[i]; q < qe; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - Tl.point_begin) << 8);
+      }
     }
-  }
-  (void)first_pair;
+  });
   return SBA_OK;
 }
 
@@ -316,6 +435,61 @@ int free_batch(sba_batch* b) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : b->allocs) (void)hipFree(p);
   delete b;
+  return SBA_OK;
+}
+
+// Device memory of a batch: every stream at a 256-byte aligned offset of one block. The uploaded streams come
+// first, then the streams that start at zero, which one fill kernel clears (no memset node, so the sequence can
+// sit inside a caller's stream capture).
+struct Layout {
+  struct Item {
+    size_t off, bytes;
+    const void* host;
+    size_t host_bytes;
+    std::function<void(char*)> bind;
+  };
+  std::vector<Item> items;
+  size_t total = 0, zero_begin = 0;
+
+  void add(size_t bytes, const void* host, size_t host_bytes, std::function<void(char*)> bind) {
+    items.push_back(Item{total, bytes, host, host_bytes, std::move(bind)});
+    total = (total + std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+  }
+  template <class P, class T>
+  void upload(P& field, const std::vector<T>& v) {
+    add(std::max<size_t>(v.size(), 1) * sizeof(T), v.empty() ? nullptr : v.data(), v.size() * sizeof(T),
+        [&field](char* p) { field = reinterpret_cast<P>(p); });
+  }
+  void begin_zeroed() { zero_begin = total; }
+  template <class P>
+  void zeroed(P& field, size_t count) {
+    add(std::max<size_t>(count, 1) * sizeof(*field), nullptr, 0, [&field](char* p) { field = reinterpret_cast<P>(p); });
+  }
+};
+
+// Where a batch's device block comes from: its own allocation or the handle's workspace.
+using MemorySource = std::function<int(sba_batch* b, size_t bytes, char** base)>;
+
+// Binds the layout to `base`, uploads the host streams and clears the zeroed ones on the handle's stream, then
+// waits for the stream (the host streams may be released after this returns).
+int place(sba_handle* h, const Layout& L, char* base) {
+  TraceRange tr("sba upload");
+  for (const auto& it : L.items) it.bind(base + it.off);
+  for (const auto& it : L.items)
+    if (it.host_bytes) HIP_TRY(hipMemcpyAsync(base + it.off, it.host, it.host_bytes, hipMemcpyHostToDevice, h->stream));
+  if (L.total > L.zero_begin) HIP_TRY(launch_fill_zero(base + L.zero_begin, L.total - L.zero_begin, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return SBA_OK;
+}
+
+// Every stream the kernels dereference must be bound before the first launch (a null stream would fault the GPU).
+int check_streams(sba_handle* h, const Batch& d) {
+  const void* need[] = {d.win, d.st, d.tiles, d.chunks, d.btiles, d.bchunks, d.red_first, d.obs_xy, d.obs_info,
+                        d.pair_obs, d.pair_pose, d.pair_rec, d.pair_xy, d.pair_info, d.point_pair, d.point_nfree,
+                        d.pts0, d.pts1, d.pts_init, d.paux, d.poses0, d.poses1, d.poses_init, d.cams, d.chunk_map,
+                        d.big, d.big_ue, d.part, d.sred, d.A, d.dc, d.dc64, d.out_pts, d.out_poses, d.trace, d.bpart};
+  for (size_t k = 0; k < sizeof(need) / sizeof(need[0]); ++k)
+    if (!need[k]) return fail(h, SBA_E_DEVICE, "internal error: device stream " + std::to_string(k) + " is not bound");
   return SBA_OK;
 }
 
@@ -408,6 +582,9 @@ int sba_create(const sba_options* opt, sba_handle** out) {
 
 int sba_destroy(sba_handle* h) {
   if (!h) return SBA_OK;
+  (void)hipSetDevice(h->opt.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->ws) (void)hipFree(h->ws);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SBA_OK;
@@ -415,25 +592,30 @@ int sba_destroy(sba_handle* h) {
 
 const char* sba_last_error(const sba_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
-int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_batch** out) {
-  if (!h || !problems || n <= 0 || !out) return fail(h, SBA_E_INVALID, "bad arguments to sba_batch_create");
+}  // extern "C"
+
+namespace {
+
+// Packs and plans the windows on the host, takes the batch's device block from `memory`, uploads, and (graph)
+// records the whole solve as one hipGraph. On failure nothing is left allocated by the batch itself.
+int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph, const MemorySource& memory,
+                sba_batch** out) {
   *out = nullptr;
-  HIP_TRY(hipSetDevice(h->opt.device));
-  HostPack H;
-  sba_batch* b = new (std::nothrow) sba_batch;
+  TraceRange tr("sba pack and plan");
+  HostPack& H = h->pack;
+  H.clear();
+  std::unique_ptr<sba_batch, int (*)(sba_batch*)> owned(new (std::nothrow) sba_batch, free_batch);
+  sba_batch* b = owned.get();
   if (!b) return fail(h, SBA_E_INVALID, "out of host memory");
   b->n = n;
   b->win.resize(n);
   b->status_pre.assign(n, SBA_OK);
   int64_t sred_off = 0, a_off = 0, dc_off = 0, big_off = 0, ue_off = 0;
-  int max_nc = 1, max_ncam = 1, max_ncf = 1, max_lds_packed = 0, max_rec = 0;
+  int max_nc = 1, max_ncam = 1, max_ncf = 1, max_rec = 0;
+  int rc = SBA_OK;
   for (int w = 0; w < n; ++w) {
     const sba_problem_view& p = problems[w];
-    const int rc = pack_window(h, p, w, H, b->win[w]);
-    if (rc != SBA_OK) {
-      free_batch(b);
-      return rc;
-    }
+    if ((rc = pack_window(h, p, w, H, b->win[w])) != SBA_OK) return rc;
     WindowDesc& W = b->win[w];
     if (W.np == 0 || W.nc == 0 || W.no == 0) b->status_pre[w] = SBA_E_EMPTY;
     W.nt = W.mapped ? 0 : (6 * W.ncf + 15) / 16;
@@ -444,9 +626,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
     W.dc_off = static_cast<int32_t>(dc_off);
     dc_off += 6 * W.ncf;
     const int nn = 6 * W.ncf;
-    const int64_t packed = static_cast<int64_t>(nn) * (nn + 1) / 2;
-    const int64_t fixed_doubles = 512 + 256 * kPanel + 48 + 256;
-    W.factor_in_lds = !W.mapped && (fixed_doubles + packed) * 8 <= 160 * 1024 ? 1 : 0;
+    W.a_off = 0;
     if (W.mapped) {
       // dense FP32 sum of Z Z^T and U, e, diag U; FP64 A (n x n), rhs, diag U
       W.big_off = big_off;
@@ -456,12 +636,6 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
       W.a_off = a_off;
       a_off += static_cast<int64_t>(nn) * nn + 2 * nn + static_cast<int64_t>((nn + 63) / 64) * 64 * 64;
       b->mapped.push_back(w);
-    } else if (W.factor_in_lds) {
-      max_lds_packed = std::max<int>(max_lds_packed, static_cast<int>(packed));
-      W.a_off = 0;
-    } else {
-      W.a_off = a_off;
-      a_off += packed;
     }
     max_nc = std::max(max_nc, W.nc);
     max_ncam = std::max(max_ncam, W.ncam);
@@ -478,6 +652,7 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each; each owns one record
   int64_t target = h->opt.chunk_pairs;
   if (target <= 0) target = default_chunk_pairs(b->total_pairs);
+  const int64_t btarget = h->opt.bchunk_pairs > 0 ? h->opt.bchunk_pairs : target;
   // In a mapped window a chunk also stages at most kMaxFreePoses free key frames: its map lists them in
   // increasing order and its pairs carry their index in that list.
   std::vector<Chunk> chunks;
@@ -547,10 +722,13 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
       const int w = H.btiles[t].window;
       const size_t begin = t;
       int64_t pairs = 0;
-      while (t < H.btiles.size() && H.btiles[t].window == w && (t == begin || pairs < target)) {
+      while (t < H.btiles.size() && H.btiles[t].window == w && (t == begin || pairs < btarget)) {
         pairs += H.btiles[t].pair_end - H.btiles[t].pair_begin;
         ++t;
       }
+      WindowDesc& W = b->win[w];
+      if (W.nbchunks == 0) W.bchunk_begin = static_cast<int32_t>(bchunks.size());
+      W.nbchunks += 1;
       bchunks.push_back(Chunk{static_cast<int32_t>(begin), static_cast<int32_t>(t), 0, 0, 0, 0, w});
     }
   }
@@ -574,69 +752,55 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   d.loss = h->opt.loss;
   d.fixed_iterations = h->opt.fixed_iterations;
   d.max_rec = max_rec;
-  int rc = SBA_OK;
-#define UP(vec, field)                                                                   \
-  if ((rc = upload(h, b, vec, const_cast<std::remove_const_t<std::remove_pointer_t<decltype(d.field)>>**>(&d.field))) != SBA_OK) { \
-    free_batch(b);                                                                       \
-    return rc;                                                                           \
-  }
-  UP(b->win, win);
-  UP(H.tiles, tiles);
-  UP(chunks, chunks);
-  UP(chunk_map, chunk_map);
-  UP(H.btiles, btiles);
-  UP(bchunks, bchunks);
-  UP(red_first, red_first);
-  UP(H.obs_xy, obs_xy);
-  UP(H.obs_info, obs_info);
-  UP(H.pair_obs, pair_obs);
-  UP(H.pair_pose, pair_pose);
-  UP(H.pair_rec, pair_rec);
-  if (H.obs_xy.size() == H.pair_rec.size()) {
-    // every pair has exactly one observation: the observation streams are already in pair order
-    d.pair_xy = d.obs_xy;
-    d.pair_info = d.obs_info;
-  } else {
+  Layout L;
+  L.upload(d.win, b->win);
+  L.upload(d.tiles, H.tiles);
+  L.upload(d.chunks, chunks);
+  L.upload(d.chunk_map, chunk_map);
+  L.upload(d.btiles, H.btiles);
+  L.upload(d.bchunks, bchunks);
+  L.upload(d.red_first, red_first);
+  L.upload(d.obs_xy, H.obs_xy);
+  L.upload(d.obs_info, H.obs_info);
+  L.upload(d.pair_obs, H.pair_obs);
+  L.upload(d.pair_pose, H.pair_pose);
+  L.upload(d.pair_rec, H.pair_rec);
+  const bool pair_streams = H.obs_xy.size() != H.pair_rec.size();
+  if (pair_streams) {
+    // some pair has several observations: the first observation of every pair, indexed by pair
     H.pair_xy.resize(H.pair_rec.size());
     H.pair_info.resize(H.pair_rec.size());
     for (size_t q = 0; q < H.pair_rec.size(); ++q) {
       H.pair_xy[q] = H.obs_xy[H.pair_rec[q].x];
       H.pair_info[q] = H.obs_info[H.pair_rec[q].x];
     }
-    UP(H.pair_xy, pair_xy);
-    UP(H.pair_info, pair_info);
+    L.upload(d.pair_xy, H.pair_xy);
+    L.upload(d.pair_info, H.pair_info);
   }
-  UP(H.point_pair, point_pair);
-  UP(H.point_nfree, point_nfree);
-  UP(H.pts, pts_init);
-  UP(H.poses, poses_init);
-  UP(H.cams, cams);
-#undef UP
-  if ((rc = upload(h, b, H.obs_orig, &b->obs_orig)) != SBA_OK) {
-    free_batch(b);
-    return rc;
-  }
-#define AL(count, field)                                \
-  if ((rc = alloc(h, b, count, &d.field)) != SBA_OK) { \
-    free_batch(b);                                      \
-    return rc;                                          \
-  }
-  AL(static_cast<size_t>(n), st);
-  AL(H.pts.size(), pts0);
-  AL(H.pts.size(), pts1);
-  AL(3 * H.pts.size(), paux);
-  AL(H.poses.size(), poses0);
-  AL(H.poses.size(), poses1);
-  AL(static_cast<size_t>(part_off), part);
-  AL(static_cast<size_t>(sred_off), sred);
-  AL(static_cast<size_t>(a_off), A);
-  AL(static_cast<size_t>(big_off), big);
-  AL(static_cast<size_t>(ue_off), big_ue);
-  AL(static_cast<size_t>(dc_off), dc);
-  AL(static_cast<size_t>(dc_off), dc64);
-  AL(H.pts.size(), out_pts);
-  AL(H.poses.size(), out_poses);
-#undef AL
+  L.upload(d.point_pair, H.point_pair);
+  L.upload(d.point_nfree, H.point_nfree);
+  L.upload(d.pts_init, H.pts);
+  L.upload(d.poses_init, H.poses);
+  L.upload(d.cams, H.cams);
+  L.upload(b->obs_orig, H.obs_orig);
+  L.begin_zeroed();
+  L.zeroed(d.st, static_cast<size_t>(n));
+  L.zeroed(d.pts0, H.pts.size());
+  L.zeroed(d.pts1, H.pts.size());
+  L.zeroed(d.paux, 3 * H.pts.size());
+  L.zeroed(d.poses0, H.poses.size());
+  L.zeroed(d.poses1, H.poses.size());
+  L.zeroed(d.part, static_cast<size_t>(part_off));
+  L.zeroed(d.sred, static_cast<size_t>(sred_off));
+  L.zeroed(d.A, static_cast<size_t>(a_off));
+  L.zeroed(d.big, static_cast<size_t>(big_off));
+  L.zeroed(d.big_ue, static_cast<size_t>(ue_off));
+  L.zeroed(d.dc, static_cast<size_t>(dc_off));
+  L.zeroed(d.dc64, static_cast<size_t>(dc_off));
+  L.zeroed(d.out_pts, H.pts.size());
+  L.zeroed(d.out_poses, H.poses.size());
+  L.zeroed(d.trace, static_cast<size_t>(n) * kTrace);
+  L.zeroed(d.bpart, bchunks.size());
   d.dbg_dp = nullptr;
   // shared-memory layout of the Schur kernel, sized for the batch's largest window
   int off = 0;
@@ -659,31 +823,93 @@ int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n,
   d.lds_zc = off;
   off += kKt * kZcLd;
   d.lds_words = off;
-  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + 256 + max_lds_packed);
   if (static_cast<int64_t>(d.lds_words) * 4 + 64 > 160 * 1024)
-    return free_batch(b), fail(h, SBA_E_CAPACITY, "Schur partial does not fit in shared memory");
-  for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e));
-  HIP_TRY(hipMalloc(&b->dbg_S, sizeof(double) * (240 * 240 + 240)));
-  b->allocs.push_back(b->dbg_S);
-  if (h->opt.use_graph) {
+    return fail(h, SBA_E_CAPACITY, "Schur partial does not fit in shared memory");
+  char* base = nullptr;
+  if ((rc = memory(b, L.total, &base)) != SBA_OK) return rc;
+  if ((rc = place(h, L, base)) != SBA_OK) return rc;
+  if (!pair_streams) {
+    // every pair has exactly one observation: the observation streams are already in pair order
+    d.pair_xy = d.obs_xy;
+    d.pair_info = d.obs_info;
+  }
+  if ((rc = check_streams(h, d)) != SBA_OK) return rc;
+  if (graph) {
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const int erc = enqueue_solve(h, b, false);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(h->stream, &g);
     if (erc != SBA_OK || ce != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      free_batch(b);
       return fail(h, SBA_E_DEVICE, "graph capture failed");
     }
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
   }
-  *out = b;
+  *out = owned.release();
   return SBA_OK;
+}
+
+// One window on the handle's stream, its device block taken from `memory`: packed, uploaded, solved with eager
+// launches (a different window shape on every call would need a new graph each time) and read back in place.
+int solve_one(sba_handle* h, const sba_problem_view* problem, sba_result* result, const MemorySource& memory) {
+  if (!h || !problem || !result) return fail(h, SBA_E_INVALID, "bad arguments to sba_solve");
+  if (problem->num_points == 0 || problem->num_poses == 0 || problem->num_observations == 0) {
+    result->status = SBA_E_EMPTY;
+    result->iterations = 0;
+    return SBA_E_EMPTY;
+  }
+  HIP_TRY(hipSetDevice(h->opt.device));
+  TraceRange tr("SchurComplementBundlerGpu::solve");
+  sba_batch* b = nullptr;
+  int rc = build_batch(h, problem, 1, false, memory, &b);
+  if (rc != SBA_OK) return rc;
+  std::unique_ptr<sba_batch, int (*)(sba_batch*)> owned(b, free_batch);
+  if ((rc = enqueue_solve(h, b, false)) != SBA_OK) return rc;
+  if ((rc = sba_batch_read(h, b, result)) != SBA_OK) return rc;
+  return result->status;
+}
+
+// A device block of its own for each batch (sba_batch_create).
+int own_memory(sba_handle* h, sba_batch* b, size_t bytes, char** base) {
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, bytes));
+  b->allocs.push_back(p);
+  *base = static_cast<char*>(p);
+  return SBA_OK;
+}
+
+// The handle's workspace, regrown to twice the need when a batch does not fit (sba_solve).
+int workspace_memory(sba_handle* h, size_t bytes, char** base) {
+  if (bytes > h->ws_bytes) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->ws) HIP_TRY(hipFree(h->ws));
+    h->ws = nullptr;
+    h->ws_bytes = 0;
+    const size_t grown = std::max(bytes, 2 * h->ws_bytes);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, grown));
+    h->ws = static_cast<char*>(p);
+    h->ws_bytes = grown;
+  }
+  *base = h->ws;
+  return SBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sba_batch_create(sba_handle* h, const sba_problem_view* problems, int32_t n, sba_batch** out) {
+  if (!h || !problems || n <= 0 || !out) return fail(h, SBA_E_INVALID, "bad arguments to sba_batch_create");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  return build_batch(h, problems, n, h->opt.use_graph != 0,
+                     [h](sba_batch* b, size_t bytes, char** base) { return own_memory(h, b, bytes, base); }, out);
 }
 
 int sba_batch_run(sba_handle* h, sba_batch* b) {
   if (!h || !b) return SBA_E_INVALID;
+  TraceRange tr("sba_batch_run");
   HIP_TRY(hipSetDevice(h->opt.device));
   b->adj_ready = false;
   if (b->gexec) {
@@ -701,6 +927,7 @@ int sba_batch_sync(sba_handle* h) {
 
 int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results) {
   if (!h || !b || !results) return SBA_E_INVALID;
+  TraceRange tr("sba_batch_read");
   HIP_TRY(hipStreamSynchronize(h->stream));
   std::vector<WindowState> st(b->n);
   HIP_TRY(hipMemcpy(st.data(), b->dev.st, sizeof(WindowState) * b->n, hipMemcpyDeviceToHost));
@@ -749,6 +976,27 @@ int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results) {
   return SBA_OK;
 }
 
+int sba_batch_trace(sba_handle* h, sba_batch* b, int32_t window, int32_t capacity, double* rho, double* trial_cost,
+                    float* lambda, int32_t* flags, int32_t* n) {
+  if (!h || !b || window < 0 || window >= b->n || capacity < 0) return fail(h, SBA_E_INVALID, "bad sba_batch_trace arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  WindowState st;
+  std::vector<TraceEntry> t(kTrace);
+  HIP_TRY(hipMemcpy(&st, b->dev.st + window, sizeof(st), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(t.data(), b->dev.trace + static_cast<int64_t>(window) * kTrace, sizeof(TraceEntry) * kTrace,
+                    hipMemcpyDeviceToHost));
+  const int m = std::min({st.iterations, kTrace, capacity});
+  for (int k = 0; k < m; ++k) {
+    if (rho) rho[k] = t[k].rho;
+    if (trial_cost) trial_cost[k] = t[k].trial_cost;
+    if (lambda) lambda[k] = t[k].lambda;
+    if (flags) flags[k] = t[k].flags;
+  }
+  if (n) *n = m;
+  return SBA_OK;
+}
+
 int sba_batch_destroy(sba_handle* h, sba_batch* b) {
   if (h) (void)hipSetDevice(h->opt.device);
   return free_batch(b);
@@ -770,6 +1018,8 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
     b->class_ms[k] = 0.f;
     b->class_launches[k] = 0;
   }
+  for (auto& e : b->ev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
   const int rc = enqueue_solve(h, b, true);
   if (rc != SBA_OK) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -824,6 +1074,10 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
   Batch d = b->dev;
   const WindowDesc& W = b->win[0];
   hipStream_t s = h->stream;
+  if (!b->dbg_S) {
+    HIP_TRY(hipMalloc(&b->dbg_S, sizeof(double) * (240 * 240 + 240)));
+    b->allocs.push_back(b->dbg_S);
+  }
   float4* dpbuf = nullptr;
   HIP_TRY(hipMalloc(&dpbuf, sizeof(float4) * std::max(1, W.np)));
   d.dbg_dp = dpbuf;
@@ -848,6 +1102,20 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
       break;
     }
     WindowState st1;
+    std::vector<CostPart> parts(std::max(1, W.nbchunks));
+    if (W.nbchunks && hipMemcpy(parts.data(), d.bpart + W.bchunk_begin, sizeof(CostPart) * W.nbchunks,
+                                hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(h, SBA_E_DEVICE, "readback failed");
+      break;
+    }
+    CostPart tot{0.0, 0.0, 0.0, 0, 0.f};
+    for (int k = 0; k < W.nbchunks; ++k) {
+      tot.cost += parts[k].cost;
+      tot.hess += parts[k].hess;
+      tot.scal += parts[k].scal;
+      tot.skipped += parts[k].skipped;
+      tot.maxdp = std::max(tot.maxdp, parts[k].maxdp);
+    }
     const int n = 6 * W.ncf;
     std::vector<double> Sbuf(static_cast<size_t>(n) * n + n), dcv(n);
     std::vector<float4> dpv(std::max(1, W.np));
@@ -868,11 +1136,11 @@ int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, 
         dp[3 * i + 2] = dpv[i].z;
       }
     if (scalars) {
-      scalars[0] = st1.acc_skipped == W.no ? INFINITY : st1.acc_cost;
-      scalars[1] = st1.acc_skipped;
-      scalars[2] = st1.acc_hess;
-      scalars[3] = st1.scal_pose + st1.acc_scal_pt;
-      scalars[4] = __builtin_bit_cast(float, st1.acc_maxdp);
+      scalars[0] = tot.skipped == W.no ? INFINITY : tot.cost;
+      scalars[1] = tot.skipped;
+      scalars[2] = tot.hess;
+      scalars[3] = st1.scal_pose + tot.scal;
+      scalars[4] = tot.maxdp;
       scalars[5] = st1.maxdc;
       scalars[6] = current;
       scalars[7] = st1.chol_ok;
@@ -914,8 +1182,6 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
   W.ncam = 1;
   W.no = 1;
   W.max_iterations = 1;
-  const int64_t packed = static_cast<int64_t>(n) * (n + 1) / 2;
-  W.factor_in_lds = (512 + 256 * kPanel + 48 + 256 + packed) * 8 <= 160 * 1024 ? 1 : 0;
   std::vector<float4> poses(3 * ncf);
   for (int j = 0; j < ncf; ++j) {
     poses[3 * j] = make_float4(1, 0, 0, 0);
@@ -932,11 +1198,10 @@ int sba_debug_factor(sba_handle* h, int32_t n, const double* A, const double* bv
       (rc = upload(h, b, stv, &d.st)) != SBA_OK || (rc = upload(h, b, rec, &d.sred)) != SBA_OK ||
       (rc = upload(h, b, poses, &d.poses0)) != SBA_OK || (rc = upload(h, b, poses, &d.poses1)) != SBA_OK ||
       (rc = alloc(h, b, static_cast<size_t>(n), &d.dc)) != SBA_OK || (rc = alloc(h, b, static_cast<size_t>(n), &d.dc64)) != SBA_OK ||
-      (rc = alloc(h, b, static_cast<size_t>(W.factor_in_lds ? 1 : packed), &d.A)) != SBA_OK) {
+      (rc = alloc(h, b, 1, &d.A)) != SBA_OK) {
     free_batch(b);
     return rc;
   }
-  d.factor_lds_words = 2 * (512 + 256 * kPanel + 48 + 256 + (W.factor_in_lds ? static_cast<int>(packed) : 0));
   if (launch_factor(d, nullptr, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
     free_batch(b);
     return fail(h, SBA_E_DEVICE, "factor launch failed");
@@ -1242,6 +1507,99 @@ int sba_autotune_chunks(sba_handle* h, const sba_problem_view* problems, int32_t
   return rc;
 }
 
+int sba_autotune_table(sba_handle* h, const char* path, const sba_problem_view* problems, int32_t n, int32_t repeats,
+                       int32_t* chunk_pairs, int32_t* bchunk_pairs, int32_t* from_table) {
+  if (!h || !path || !problems || n <= 0 || !chunk_pairs || !bchunk_pairs || repeats < 1)
+    return fail(h, SBA_E_INVALID, "bad sba_autotune_table arguments");
+  HIP_TRY(hipSetDevice(h->opt.device));
+  // the batch's shape class: device architecture, loss, window count, total observations and pairs to two
+  // significant digits, largest free key-frame count
+  hipDeviceProp_t prop{};
+  HIP_TRY(hipGetDeviceProperties(&prop, h->opt.device));
+  int64_t no = 0, maxf = 0;
+  for (int w = 0; w < n; ++w) {
+    no += problems[w].num_observations;
+    maxf = std::max<int64_t>(maxf, problems[w].num_poses - problems[w].num_fixed_key_frames);
+  }
+  auto two_digits = [](int64_t v) {
+    int64_t scale = 1;
+    while (v >= 100) {
+      v /= 10;
+      scale *= 10;
+    }
+    return v * scale;
+  };
+  char key[256];
+  std::snprintf(key, sizeof(key), "%s loss%d windows%d obs%lld maxfree%lld", prop.gcnArchName, h->opt.loss, n,
+                static_cast<long long>(two_digits(no)), static_cast<long long>(maxf));
+  for (char* c = key; *c; ++c)
+    if (*c == ':' ) *c = '_';
+  if (from_table) *from_table = 0;
+  if (FILE* f = std::fopen(path, "r")) {
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+      char k[256];
+      int c = 0, bc = 0;
+      // "<key fields...> | <chunk_pairs> <bchunk_pairs>"
+      const char* bar = std::strchr(line, '|');
+      if (!bar || static_cast<size_t>(bar - line) >= sizeof(k)) continue;
+      std::memcpy(k, line, bar - line);
+      k[bar - line] = 0;
+      while (bar - line > 0 && k[std::strlen(k) - 1] == ' ') k[std::strlen(k) - 1] = 0;
+      if (std::strcmp(k, key) == 0 && std::sscanf(bar + 1, "%d %d", &c, &bc) == 2 && c > 0 && bc > 0) {
+        *chunk_pairs = c;
+        *bchunk_pairs = bc;
+        if (from_table) *from_table = 1;
+        std::fclose(f);
+        return SBA_OK;
+      }
+    }
+    std::fclose(f);
+  }
+  // not in the table: tune the Schur chunks, then the back-substitution chunks at the chosen Schur size
+  int32_t best = 0;
+  int rc = sba_autotune_chunks(h, problems, n, nullptr, 0, repeats, &best, nullptr);
+  if (rc != SBA_OK) return rc;
+  const int32_t saved_c = h->opt.chunk_pairs, saved_b = h->opt.bchunk_pairs;
+  h->opt.chunk_pairs = best;
+  int32_t best_b = best;
+  float best_ms = 0.f;
+  for (int32_t m : {1, 2, 4, 8}) {
+    h->opt.bchunk_pairs = best * m;
+    sba_batch* b = nullptr;
+    if ((rc = sba_batch_create(h, problems, n, &b)) != SBA_OK) break;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess && sba_batch_run(h, b) == SBA_OK &&
+        hipEventRecord(e0, h->stream) == hipSuccess) {
+      for (int r = 0; r < repeats; ++r) (void)sba_batch_run(h, b);
+      if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        rc = fail(h, SBA_E_DEVICE, "autotune timing failed");
+    } else {
+      rc = fail(h, SBA_E_DEVICE, "autotune launch failed");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    sba_batch_destroy(h, b);
+    if (rc != SBA_OK) break;
+    if (m == 1 || ms < best_ms) {
+      best_ms = ms;
+      best_b = best * m;
+    }
+  }
+  h->opt.chunk_pairs = saved_c;
+  h->opt.bchunk_pairs = saved_b;
+  if (rc != SBA_OK) return rc;
+  *chunk_pairs = best;
+  *bchunk_pairs = best_b;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%s | %d %d %.4f\n", key, best, best_b, best_ms / repeats);
+    std::fclose(f);
+  }
+  return SBA_OK;
+}
+
 int sba_batch_adjoint(sba_handle* h, sba_batch* b, const double* g, double* dxy, double* dinfo, int32_t max_cg,
                       double tol, int32_t* cg_iterations, double* a_out) {
   if (!h || !b || !g || !dxy || !dinfo || max_cg < 1) return fail(h, SBA_E_INVALID, "bad sba_batch_adjoint arguments");
@@ -1276,15 +1634,9 @@ int sba_solve_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, 
 }
 
 int sba_solve(sba_handle* h, const sba_problem_view* problem, sba_result* result) {
-  if (!h || !problem || !result) return fail(h, SBA_E_INVALID, "bad arguments to sba_solve");
-  if (problem->num_points == 0 || problem->num_poses == 0 || problem->num_observations == 0) {
-    result->status = SBA_E_EMPTY;
-    result->iterations = 0;
-    return SBA_E_EMPTY;
-  }
-  const int rc = sba_solve_batch(h, problem, 1, result);
-  if (rc != SBA_OK) return rc;
-  return result->status;
+  return solve_one(h, problem, result, [h](sba_batch*, size_t bytes, char** base) {
+    return workspace_memory(h, bytes, base);
+  });
 }
 
 }  // extern "C"

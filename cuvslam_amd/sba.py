@@ -8,6 +8,7 @@ the solve ran, False for an empty problem or an infeasible start (every observat
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -120,12 +121,13 @@ class SchurComplementBundlerGpu:
     """cuvslam::sba::SchurComplementBundlerGpu on MI355X. ``loss``: LOSS_STUDENT_T (the reference's) or LOSS_HUBER."""
 
     def __init__(self, loss: int = LOSS_STUDENT_T, device: int = 0, fixed_iterations: bool = False,
-                 use_graph: bool = True, chunk_pairs: int = 0):
+                 use_graph: bool = True, chunk_pairs: int = 0, bchunk_pairs: int = 0, pack_threads: int = 0):
         L = _lib.lib()
         opt = _lib.Options()
         L.sba_default_options(C.byref(opt))
         opt.device, opt.loss = device, loss
         opt.fixed_iterations, opt.use_graph, opt.chunk_pairs = int(fixed_iterations), int(use_graph), chunk_pairs
+        opt.bchunk_pairs, opt.pack_threads = bchunk_pairs, pack_threads
         h = C.c_void_p()
         rc = L.sba_create(C.byref(opt), C.byref(h))
         if rc != SBA_OK:
@@ -157,6 +159,17 @@ class SchurComplementBundlerGpu:
         if rc != SBA_OK:
             raise SbaError(rc, self.last_error())
         return int(best.value), [float(x) for x in ms[:len(cand)]]
+
+    def autotune_table(self, problems: Sequence[BundleAdjustmentProblem], path: str, repeats: int = 1):
+        """Chunk sizes for ``problems`` from the autotune table at ``path`` (tuned and appended when the batch's shape
+        class is not there): (chunk_pairs, bchunk_pairs, taken from the table)."""
+        views = (_lib.ProblemView * len(problems))(*[p.view() for p in problems])
+        c, bc, hit = C.c_int32(), C.c_int32(), C.c_int32()
+        rc = self._L.sba_autotune_table(self._h, os.fsencode(path), views, len(problems), repeats, C.byref(c),
+                                        C.byref(bc), C.byref(hit))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.last_error())
+        return int(c.value), int(bc.value), bool(hit.value)
 
     def debug_copy(self, nbytes: int, repeats: int = 3) -> float:
         """Device-to-device copy of nbytes by a plain kernel (the calibration of HBM byte counters); ms per copy."""
@@ -269,6 +282,23 @@ class DeviceBatch:
         if rc not in (SBA_OK, _lib.SBA_E_INFEASIBLE):
             raise SbaError(rc, self.solver.last_error())
         return S, e, dc, dp, sc, rc
+
+    def trace(self, window: int = 0):
+        """The LM iterations of ``window`` in the last run: dict of arrays rho, trial_cost, lambda, flags (1 accepted,
+        2 factorization failed, 4 stopped), one entry per iteration (the first 64)."""
+        cap = 64
+        rho, cost = np.zeros(cap), np.zeros(cap)
+        lam = np.zeros(cap, np.float32)
+        flags = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        rc = self.solver._L.sba_batch_trace(self.solver._h, self._b, window, cap, d(rho), d(cost),
+                                            lam.ctypes.data_as(C.POINTER(C.c_float)),
+                                            flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n))
+        if rc != SBA_OK:
+            raise SbaError(rc, self.solver.last_error())
+        k = n.value
+        return {"rho": rho[:k], "trial_cost": cost[:k], "lambda": lam[:k], "flags": flags[:k]}
 
     def winsorize(self, key_frame: int, window: int = 0):
         """Landmark winsorizing of a solved window (pipelines/winsorizer.cpp:23-73): resets the points whose

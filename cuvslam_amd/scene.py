@@ -15,9 +15,9 @@ from . import _lib
 from .sba import BundleAdjustmentProblem
 
 PRESETS = {
-    # 10 key frames, 2,000 points (about 15k observations)
+    # 10 key frames, 2,000 points (17.8k observations with every point seen wherever it projects in-image)
     "c1": dict(num_keyframes=10, num_points=2000),
-    # 30 key frames, 20,000 points (about 300k observations)
+    # 30 key frames, 20,000 points (528k observations)
     "c2": dict(num_keyframes=30, num_points=20000),
     # 20 key frames, 10,000 points
     "c4": dict(num_keyframes=20, num_points=10000),

@@ -89,7 +89,10 @@ typedef struct sba_options {
   int32_t use_graph;              /* 1: capture the LM iterations of a batch in one hipGraph */
   int32_t chunk_pairs;            /* work per Schur workgroup in (point, key frame) pairs; 0 = built-in
                                      heuristic (sba_autotune_chunks picks a value by timing) */
-  int32_t reserved[7];
+  int32_t pack_threads;           /* host threads that sort and pack one window; 0 = up to 16 for windows of 65536
+                                     observations or more, else 1 (the packed streams do not depend on it) */
+  int32_t bchunk_pairs;           /* work per back-substitution workgroup in pairs; 0 = chunk_pairs */
+  int32_t reserved[5];
 } sba_options;
 
 typedef struct sba_handle sba_handle;
@@ -135,6 +138,15 @@ int sba_batch_kernel_times(sba_handle* h, sba_batch* b, float* ms_per_class, int
 int sba_autotune_chunks(sba_handle* h, const sba_problem_view* problems, int32_t n, const int32_t* candidates,
                         int32_t n_candidates, int32_t repeats, int32_t* best_chunk_pairs, float* ms_per_candidate);
 
+/* The autotuner with a table: looks up the batch's shape class (device architecture, loss, window count, total
+ * observations to two significant digits, largest free key-frame count) in the text file `path`; when it is not
+ * there, tunes the Schur chunk size (sba_autotune_chunks over its default candidates), then the back-substitution
+ * chunk size (1, 2, 4 and 8 times the Schur size), and appends the result. *from_table [optional] is 1 when the
+ * sizes came from the file. Set the results as sba_options.chunk_pairs and bchunk_pairs of a handle. No reference
+ * counterpart. */
+int sba_autotune_table(sba_handle* h, const char* path, const sba_problem_view* problems, int32_t n, int32_t repeats,
+                       int32_t* chunk_pairs, int32_t* bchunk_pairs, int32_t* from_table);
+
 /* Landmark winsorizing between two solves (replaces cuvslam::pipelines::winsorize, pipelines/winsorizer.cpp:23-73,
  * as called between the two bundler runs of service_sba.h:317-339), on the solved state of `window` of a batch
  * (run sba_batch_run first). The squared reprojection errors (normalized coordinates) of the observations made
@@ -159,6 +171,14 @@ int sba_batch_winsorize(sba_handle* h, sba_batch* b, int32_t window, int32_t key
  * Returns SBA_OK, or SBA_E_INFEASIBLE when the factorization failed (S not positive definite). */
 int sba_batch_debug_step(sba_handle* h, sba_batch* b, double lambda, double* S, double* e, double* dc,
                          double* dp, double* scalars);
+
+/* The LM iterations of `window` in the last run of a batch, as the device decided them (the first 64 are kept), for
+ * parity tests of the damping control (schur_complement_bundler_gpu.cpp:175-253): per iteration the gain ratio
+ * (0 when none was formed), the FP64 trial cost (0 when the factorization failed), the damping the step was solved
+ * with, and flags (1 accepted, 2 factorization failed, 4 a stop test ended the solve). *n receives the number of
+ * iterations written (at most capacity). Any output may be null. */
+int sba_batch_trace(sba_handle* h, sba_batch* b, int32_t window, int32_t capacity, double* rho, double* trial_cost,
+                    float* lambda, int32_t* flags, int32_t* n);
 
 /* FP64 refinement of the solved state of window 0 of a batch, for the differentiable path: the FP32 LM solve
  * stops at its own rounding floor, and an implicit gradient taken there carries that residual gradient. The FP32

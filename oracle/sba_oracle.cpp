@@ -44,7 +44,17 @@ struct OrcOptions {
   int32_t dense_schur;       // 1: dense temp * point_pose_block product as the CPU bundler does
   int32_t fixed_iterations;  // 1: no step-size convergence exit
   int32_t reorthonormalize;  // 1: UpdateState re-orthonormalizes rotations (cpu.cpp:377-380)
-  int32_t pad[3];
+  // Rounding model of an FP32 reduced system (FP64 solver only; a sensitivity probe, not a reference behaviour):
+  //   0 none; 1 the assembled S and e rounded to FP32; 2 the camera blocks U, e_a and the Schur sums
+  //   sum W V*^-1 W^T, sum W V*^-1 e_b each rounded to FP32 before S = U - sum W V*^-1 W^T (+ damping) is formed
+  //   in FP64, the order in which the MI355X kernels combine their FP32 accumulators.
+  int32_t round_reduced_f32;
+  // Branch probe (FP64 solver only): at this LM iteration (1-based; 0 = never) the damping decision takes the
+  // other side of the threshold its gain ratio is nearest to -- accept/reject when rho is nearer 0.25, keep/shrink
+  // lambda when it is nearer 0.75. Gives the FP64 trajectory that a solver whose rho lands on the other side of a
+  // threshold (through rounding) follows.
+  int32_t flip_iteration;
+  int32_t pad;
 };
 
 struct OrcResult {
@@ -528,7 +538,22 @@ struct Solver {
         }
       }
     }
+    if (opt.round_reduced_f32 == 2) {
+      // S currently holds U - sum W V*^-1 W^T and e holds e_a - sum W V*^-1 e_b: split off the Schur sums, round
+      // both parts to FP32, and combine again in FP64
+      auto r32 = [](T x) { return static_cast<T>(static_cast<float>(x)); };
+      std::vector<T> U(static_cast<size_t>(n) * n, T(0));
+      for (int c = 0; c < ncf; ++c)
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) U[(6 * c + a) * n + 6 * c + b] = F.U[36 * c + 6 * a + b];
+      for (size_t k = 0; k < U.size(); ++k) R.S[k] = r32(U[k]) - r32(U[k] - R.S[k]);
+      for (int k = 0; k < n; ++k) R.e[k] = r32(F.ea[k]) - r32(F.ea[k] - R.e[k]);
+    }
     for (int i = 0; i < n; ++i) R.S[static_cast<size_t>(i) * n + i] += lambda * F.U[36 * (i / 6) + 7 * (i % 6)];
+    if (opt.round_reduced_f32 == 1) {
+      for (auto& x : R.S) x = static_cast<T>(static_cast<float>(x));
+      for (auto& x : R.e) x = static_cast<T>(static_cast<float>(x));
+    }
   }
 
   struct Update {
@@ -694,8 +719,15 @@ struct Solver {
             flags |= 4;
           } else {
             rho = (T(1) - cost / current) / pred;
-            if (rho > T(0.25f)) {
-              if (rho > T(0.75f) && lambda * T(0.125f) > T(0)) lambda *= T(0.125f);
+            bool accept = rho > T(0.25f), shrink = rho > T(0.75f);
+            if (it == opt.flip_iteration) {
+              if (std::fabs(rho - T(0.75f)) < std::fabs(rho - T(0.25f)))
+                shrink = !shrink;
+              else
+                accept = !accept;
+            }
+            if (accept) {
+              if (shrink && lambda * T(0.125f) > T(0)) lambda *= T(0.125f);
               current = cost;
               res->last_cost = static_cast<double>(cost);
               res->accepted += 1;
@@ -1010,6 +1042,626 @@ int solve_impl(const OrcProblem* p, const OrcOptions* o, OrcResult* r, const dou
   return status;
 }
 
+
+// ---------------------------------------------------------------------------------------------------------
+// MonoSBASolver (libs/sba/mono_sba_solver.{h,cpp}): Hartley & Zisserman 2nd ed. Algorithm A6.4 with cameras
+// parameterized by the twist a of camera_from_world, the Huber loss with per-track information matrices
+// estimated from the track's residuals, multiplicative damping (1 + lambda) of the U and V diagonals, the x10 / /10
+// damping schedule, the mean robust residual as the step test, and a pose covariance from the last free frame's
+// block of the reduced system. Restated step for step, templated on the scalar (float: the reference's arithmetic).
+// ---------------------------------------------------------------------------------------------------------
+
+constexpr float kMonoLambdaMax = 1.0e+7f;  // mono_sba_solver.cpp:26
+constexpr float kMonoHuber = 0.5f;         // mono_sba_solver.cpp:214
+
+// Log of a pose (math/twist.cpp:44-67): the rotation axis is the right singular vector of R - I of its smallest
+// singular value (an eigenvector of (R - I)^T (R - I)), the angle atan2(rvec . axis, trace(R) - 1).
+template <class T>
+void twist_log(const Pose<T>& P, T tw[6]) {
+  T a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = P.R[i][j] - (i == j ? T(1) : T(0));
+  T ata[9], ev[3], V[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) ata[i * 3 + j] = a[0][i] * a[0][j] + a[1][i] * a[1][j] + a[2][i] * a[2][j];
+  sym_jacobi<T>(3, ata, ev, V);
+  int k = 0;
+  for (int c = 1; c < 3; ++c)
+    if (ev[c] < ev[k]) k = c;
+  T w[3] = {V[0 * 3 + k], V[1 * 3 + k], V[2 * 3 + k]};
+  const T nw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  for (auto& x : w) x /= nw;
+  const T rvec[3] = {a[2][1] - a[1][2], a[0][2] - a[2][0], a[1][0] - a[0][1]};
+  const T wmag = std::atan2(rvec[0] * w[0] + rvec[1] * w[1] + rvec[2] * w[2], a[0][0] + a[1][1] + a[2][2] + T(2));
+  const T h = wmag / T(2);
+  const T* t = P.t;
+  const T wxt[3] = {w[1] * t[2] - w[2] * t[1], w[2] * t[0] - w[0] * t[2], w[0] * t[1] - w[1] * t[0]};
+  const T wxwxt[3] = {w[1] * wxt[2] - w[2] * wxt[1], w[2] * wxt[0] - w[0] * wxt[2], w[0] * wxt[1] - w[1] * wxt[0]};
+  const T coef = std::fabs(wmag) < T(kFltEps) ? T(0) : (T(1) - h / std::tan(h));
+  for (int i = 0; i < 3; ++i) {
+    tw[i] = wmag * w[i];
+    tw[3 + i] = t[i] + coef * wxwxt[i] - h * wxt[i];
+  }
+}
+
+// VectorTwist::operator+ (math/twist.h:67-73): Log(Exp(rhs) Exp(lhs)).
+template <class T>
+void twist_add(const T lhs[6], const T rhs[6], T out[6]) {
+  twist_log<T>(compose(twist_exp<T>(rhs), twist_exp<T>(lhs)), out);
+}
+
+template <class T>
+T avoid_zero(T a) {  // common/types.h:219-221
+  return std::fabs(a) > T(kFltEps) ? a : std::copysign(T(kFltEps), a);
+}
+
+// Eigen's computeInverseAndDetWithCheck for 2x2 and 3x3 (row-major): the inverse is written only when |det| > thr.
+template <class T>
+bool inverse_with_check2(const T m[4], T out[4], T thr) {
+  const T det = m[0] * m[3] - m[1] * m[2];
+  if (!(std::fabs(det) > thr)) return false;
+  out[0] = m[3] / det;
+  out[1] = -m[1] / det;
+  out[2] = -m[2] / det;
+  out[3] = m[0] / det;
+  return true;
+}
+template <class T>
+bool inverse_with_check3(const T m[9], T out[9], T thr) {
+  const T c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+  const T det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+  if (!(std::fabs(det) > thr)) return false;
+  const T c10 = m[2] * m[7] - m[1] * m[8], c11 = m[0] * m[8] - m[2] * m[6], c12 = m[1] * m[6] - m[0] * m[7];
+  const T c20 = m[1] * m[5] - m[2] * m[4], c21 = m[2] * m[3] - m[0] * m[5], c22 = m[0] * m[4] - m[1] * m[3];
+  const T cof[9] = {c00, c01, c02, c10, c11, c12, c20, c21, c22};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[i * 3 + j] = cof[j * 3 + i] / det;  // adjugate / det
+  return true;
+}
+
+// StatisticalBase (common/statistic.h:80-106): running mean and n * variance.
+template <class T>
+struct RunStat {
+  T mean = 0, nvar = 0;
+  int count = 0;
+  void add(T v) {
+    ++count;
+    const T d = v - mean;
+    mean += d / T(count);
+    nvar += d * d * T(count - 1) / T(count);
+  }
+  T variance() const { return nvar / T(count); }
+};
+
+struct OrcMonoProblem {
+  int32_t num_frames, num_tracks, num_obs;
+  const int32_t* constrained;  // [num_frames]
+  const float* frame_poses;    // [num_frames][16] column-major world_from_camera (the addFrame argument)
+  const float* points;         // [num_tracks][3]
+  const int32_t* obs_frame;    // [num_obs]
+  const int32_t* obs_track;    // [num_obs]
+  const float* obs_xy;         // [num_obs][2]
+  int32_t max_iterations;
+  float min_residual, min_residual_speed;
+};
+
+struct OrcMonoResult {
+  double* frame_poses;  // out [num_frames][16] world_from_camera
+  double* points;       // out [num_tracks][3]
+  double covariance[36];
+  double residual, residual_initial, lambda;
+  int32_t iterations, failed, feasible, pad;
+};
+
+template <class T>
+struct MonoSolver {
+  struct Frame {
+    bool constrained;
+    T a[6], aD[6], da[6], errA[6], eps[6], Uo[36], Ua[36];
+  };
+  struct Track {
+    bool skip = false, removed = false;
+    T b[3], bD[3], db[3], errB[3], V[9], invVa[9], info[4];
+    std::vector<int> frames, obs;  // frames observing it in frame order, and the observation of each
+  };
+  struct Obs {
+    T coord[2], err[2], w, A[12], B[6], W[18], Y[18];
+  };
+  int max_iterations;
+  T min_residual, min_residual_speed;
+  T residual = -1, residual_initial = 0, lambda = T(0.001f);
+  int iteration = -1;
+  bool failed = false;
+  std::vector<Frame> frames;
+  std::vector<Track> tracks;
+  std::vector<Obs> obs;
+  T prep[36] = {}, cov[36] = {};
+
+  void load(const OrcMonoProblem* p) {
+    max_iterations = p->max_iterations;
+    min_residual = static_cast<T>(p->min_residual);
+    min_residual_speed = static_cast<T>(p->min_residual_speed);
+    frames.resize(p->num_frames);
+    for (int f = 0; f < p->num_frames; ++f) {
+      Frame& F = frames[f];
+      F.constrained = p->constrained[f] != 0;
+      const Pose<T> wc = load_pose<T>(p->frame_poses + 16 * f);
+      Pose<T> cw;  // the inverse: a maps world coordinates to frame coordinates (mono_sba_solver.cpp:45)
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) cw.R[i][j] = wc.R[j][i];
+        cw.t[i] = -(wc.R[0][i] * wc.t[0] + wc.R[1][i] * wc.t[1] + wc.R[2][i] * wc.t[2]);
+      }
+      twist_log<T>(cw, F.a);
+      std::fill(F.da, F.da + 6, T(0));
+      std::fill(F.errA, F.errA + 6, T(0));
+      if (F.constrained) std::copy(F.a, F.a + 6, F.aD);
+    }
+    tracks.resize(p->num_tracks);
+    for (int t = 0; t < p->num_tracks; ++t) {
+      Track& K = tracks[t];
+      for (int d = 0; d < 3; ++d) K.b[d] = static_cast<T>(p->points[3 * t + d]);
+      std::fill(K.db, K.db + 3, T(0));
+      K.info[0] = K.info[3] = T(1);
+      K.info[1] = K.info[2] = T(0);
+    }
+    obs.resize(p->num_obs);
+    // frames are added in order, each listing its tracks: a track's frames are in frame order
+    std::vector<int> order(p->num_obs);
+    for (int o = 0; o < p->num_obs; ++o) order[o] = o;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return p->obs_frame[x] < p->obs_frame[y]; });
+    for (int o : order) {
+      Obs& O = obs[o];
+      O.coord[0] = static_cast<T>(p->obs_xy[2 * o]);
+      O.coord[1] = static_cast<T>(p->obs_xy[2 * o + 1]);
+      O.err[0] = O.err[1] = T(0);
+      O.w = T(1);
+      tracks[p->obs_track[o]].frames.push_back(p->obs_frame[o]);
+      tracks[p->obs_track[o]].obs.push_back(o);
+    }
+  }
+
+  // project world point b through the camera with twist a (camera_from_world)
+  static void project(const T a[6], const T b[3], T out[2]) {
+    const Pose<T> M = twist_exp<T>(a);
+    T q[3];
+    apply(M, b, q);
+    const T z = avoid_zero(q[2]);
+    out[0] = q[0] / z;
+    out[1] = q[1] / z;
+  }
+
+  void filter_hither() {  // mono_sba_solver.cpp:346-361
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (int f : K.frames) {
+        const Pose<T> M = twist_exp<T>(frames[f].a);
+        T q[3];
+        apply(M, K.b, q);
+        if (!(q[2] > T(kHither))) {
+          K.skip = true;
+          break;
+        }
+      }
+    }
+  }
+
+  void update_aDbD() {  // mono_sba_solver.cpp:770-786
+    for (Frame& F : frames)
+      if (!F.constrained) twist_add<T>(F.a, F.da, F.aD);
+    for (Track& K : tracks)
+      if (!K.skip)
+        for (int d = 0; d < 3; ++d) K.bD[d] = K.b[d] + K.db[d];
+  }
+
+  void update() {  // mono_sba_solver.cpp:788-810
+    for (Frame& F : frames)
+      if (!F.constrained) std::copy(F.aD, F.aD + 6, F.a);
+    for (Track& K : tracks) {
+      if (K.removed) continue;
+      if (K.skip)
+        K.skip = false;
+      else
+        std::copy(K.bD, K.bD + 3, K.b);
+    }
+  }
+
+  static T huber(T s) { return loss_rho<T>(1, s, T(kMonoHuber)); }
+  static T dhuber(T s) { return loss_drho<T>(1, s, T(kMonoHuber)); }
+  static T quad(const T e[2], const T L[4]) { return e[0] * (L[0] * e[0] + L[1] * e[1]) + e[1] * (L[2] * e[0] + L[3] * e[1]); }
+
+  T calc_residual() {  // mono_sba_solver.cpp:743-768
+    int total = 0;
+    T r = 0;
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        Obs& O = obs[K.obs[k]];
+        T proj[2];
+        project(frames[K.frames[k]].aD, K.bD, proj);
+        O.err[0] = O.coord[0] - proj[0];
+        O.err[1] = O.coord[1] - proj[1];
+        r += huber(quad(O.err, K.info));
+        ++total;
+      }
+    }
+    return total != 0 ? r / T(total) : T(std::numeric_limits<float>::max());
+  }
+
+  // mono_sba_solver.cpp:363-432: the covariance of each track's residuals over its frames (variances floored at
+  // 1e-5; with more than three samples the cross term, then an LDLT with the floor on D, reassembled without the
+  // pivot permutation as the reference does), inverted when its determinant exceeds FLT_EPSILON, else the diagonal
+  void calc_information() {
+    const T thresh = T(1e-5f);
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      RunStat<T> sx, sy, sxy;
+      for (int o : K.obs) {
+        sx.add(obs[o].err[0]);
+        sy.add(obs[o].err[1]);
+        sxy.add(obs[o].err[0] * obs[o].err[1]);
+      }
+      T c[4] = {std::max(sx.variance(), thresh), 0, 0, std::max(sy.variance(), thresh)};
+      c[0] = sx.variance() < thresh ? thresh : sx.variance();
+      c[3] = sy.variance() < thresh ? thresh : sy.variance();
+      if (sx.count > 3) {
+        c[1] = c[2] = sxy.mean - sx.mean * sy.mean;
+        // Eigen::LDLT with pivoting: the larger diagonal first (the first on a tie)
+        const bool swap = std::fabs(c[3]) > std::fabs(c[0]);
+        const T p00 = swap ? c[3] : c[0], p11 = swap ? c[0] : c[3], p01 = c[1];
+        const T d0 = p00, l = p01 / p00, d1 = p11 - l * p01;
+        const T e0 = std::max(d0, thresh), e1 = std::max(d1, thresh);
+        c[0] = e0;
+        c[1] = c[2] = l * e0;
+        c[3] = l * l * e0 + e1;
+      }
+      if (!inverse_with_check2<T>(c, K.info, T(kFltEps))) {
+        K.info[0] = T(1) / c[0];
+        K.info[3] = T(1) / c[3];
+        K.info[1] = K.info[2] = T(0);
+      }
+    }
+  }
+
+  void compute_weights() {  // mono_sba_solver.cpp:435-444 (every track, skipped or not)
+    for (Track& K : tracks)
+      for (int o : K.obs) obs[o].w = dhuber(quad(obs[o].err, K.info));
+  }
+
+  void calc_derivatives() {  // mono_sba_solver.cpp:704-741, math/twist.h:152-206
+    std::vector<Pose<T>> M(frames.size());
+    std::vector<Pose<T>> Rn(frames.size());
+    for (size_t f = 0; f < frames.size(); ++f) {
+      M[f] = twist_exp<T>(frames[f].a);
+      nearest_rotation<T>(M[f].R, Rn[f].R);
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        const int f = K.frames[k];
+        Obs& O = obs[K.obs[k]];
+        T q[3];
+        apply(M[f], K.b, q);
+        const T iz = T(1) / avoid_zero(q[2]);
+        const T x = q[0] * iz, y = q[1] * iz;
+        for (int j = 0; j < 3; ++j) {
+          O.B[j] = (Rn[f].R[0][j] - Rn[f].R[2][j] * x) * iz;
+          O.B[3 + j] = (Rn[f].R[1][j] - Rn[f].R[2][j] * y) * iz;
+        }
+        if (!frames[f].constrained) {
+          // -am: the reference's camera derivatives with the sign of HZ's convention
+          const T am[12] = {x * y, -x * x - T(1), y, -iz, T(0), x * iz, y * y + T(1), -(x * y), -x, T(0), -iz, y * iz};
+          for (int e = 0; e < 12; ++e) O.A[e] = -am[e];
+        }
+      }
+    }
+  }
+
+  // tmp = A^T (w info) (6x2), then accumulations (mono_sba_solver.cpp:586-678)
+  void calc_UVW() {
+    for (Frame& F : frames) {
+      if (F.constrained) continue;
+      std::fill(F.Uo, F.Uo + 36, T(0));
+      std::fill(F.errA, F.errA + 6, T(0));
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        Frame& F = frames[K.frames[k]];
+        if (F.constrained) continue;
+        const Obs& O = obs[K.obs[k]];
+        T tmp[6][2];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 2; ++c) tmp[r][c] = O.A[r] * (O.w * K.info[c]) + O.A[6 + r] * (O.w * K.info[2 + c]);
+        for (int r = 0; r < 6; ++r) {
+          for (int c = 0; c < 6; ++c) F.Uo[r * 6 + c] += tmp[r][0] * O.A[c] + tmp[r][1] * O.A[6 + c];
+          F.errA[r] += tmp[r][0] * O.err[0] + tmp[r][1] * O.err[1];
+        }
+      }
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      std::fill(K.V, K.V + 9, T(0));
+      std::fill(K.errB, K.errB + 3, T(0));
+      for (int o : K.obs) {
+        const Obs& O = obs[o];
+        T tmp[3][2];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 2; ++c) tmp[r][c] = O.B[r] * (O.w * K.info[c]) + O.B[3 + r] * (O.w * K.info[2 + c]);
+        for (int r = 0; r < 3; ++r) {
+          for (int c = 0; c < 3; ++c) K.V[r * 3 + c] += tmp[r][0] * O.B[c] + tmp[r][1] * O.B[3 + c];
+          K.errB[r] += tmp[r][0] * O.err[0] + tmp[r][1] * O.err[1];
+        }
+      }
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        if (frames[K.frames[k]].constrained) continue;
+        Obs& O = obs[K.obs[k]];
+        T Li[2][2];  // info * w
+        for (int a = 0; a < 2; ++a)
+          for (int c = 0; c < 2; ++c) Li[a][c] = K.info[a * 2 + c] * O.w;
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c) {
+            const T m0 = O.A[r] * Li[0][0] + O.A[6 + r] * Li[1][0];
+            const T m1 = O.A[r] * Li[0][1] + O.A[6 + r] * Li[1][1];
+            O.W[r * 3 + c] = m0 * O.B[c] + m1 * O.B[3 + c];
+          }
+      }
+    }
+  }
+
+  void step_UV() {  // mono_sba_solver.cpp:446-510, invert3x3 :892-899
+    for (Frame& F : frames) {
+      if (F.constrained) continue;
+      std::copy(F.Uo, F.Uo + 36, F.Ua);
+      for (int i = 0; i < 6; ++i) F.Ua[i * 7] *= T(1) + lambda;
+    }
+    const T thr = T(1) / std::max(T(1e6f), lambda * lambda * lambda);
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      T V[9];
+      std::copy(K.V, K.V + 9, V);
+      for (int i = 0; i < 3; ++i) V[i * 4] *= T(1) + lambda;
+      if (!inverse_with_check3<T>(V, K.invVa, thr)) K.skip = true;
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        if (frames[K.frames[k]].constrained) continue;
+        Obs& O = obs[K.obs[k]];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 3; ++c)
+            O.Y[r * 3 + c] = O.W[r * 3] * K.invVa[c] + O.W[r * 3 + 1] * K.invVa[3 + c] + O.W[r * 3 + 2] * K.invVa[6 + c];
+      }
+    }
+  }
+
+  // JacobiSVD solve with threshold n FLT_EPSILON (IterativeSolver, common/types.h:250-290) of a symmetric system:
+  // false when a singular value is at or below the threshold (rank < n); else x with up to 5 refinement steps
+  // kept while the residual norm decreases
+  static bool svd_solve(int n, const std::vector<T>& S, const std::vector<T>& bvec, std::vector<T>& x) {
+    std::vector<T> a(S), ev(n), q(static_cast<size_t>(n) * n);
+    sym_jacobi<T>(n, a.data(), ev.data(), q.data());
+    T smax = 0;
+    for (int k = 0; k < n; ++k) smax = std::max(smax, std::fabs(ev[k]));
+    const T thr = T(n) * T(kFltEps) * smax;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += std::fabs(ev[k]) > thr;
+    if (rank < n) return false;
+    auto psolve = [&](const std::vector<T>& rhs, std::vector<T>& out) {
+      out.assign(n, T(0));
+      for (int k = 0; k < n; ++k) {
+        T s = 0;
+        for (int i = 0; i < n; ++i) s += q[i * n + k] * rhs[i];
+        s /= ev[k];
+        for (int i = 0; i < n; ++i) out[i] += q[i * n + k] * s;
+      }
+    };
+    auto norm = [](const std::vector<T>& v) {
+      T s = 0;
+      for (T e : v) s += e * e;
+      return std::sqrt(s);
+    };
+    std::vector<T> xt(n, T(0)), bd(bvec), d;
+    x.assign(n, T(0));
+    T min_norm = std::numeric_limits<T>::max(), nrm = norm(bvec);
+    for (int it = 0; nrm < min_norm && it < 5; ++it) {
+      min_norm = nrm;
+      x = xt;
+      psolve(bd, d);
+      for (int i = 0; i < n; ++i) xt[i] += d[i];
+      for (int i = 0; i < n; ++i) {
+        T s = bvec[i];
+        for (int j = 0; j < n; ++j) s -= S[static_cast<size_t>(i) * n + j] * xt[j];
+        bd[i] = s;
+      }
+      nrm = norm(bd);
+      if (nrm < T(kFltEps)) {
+        x = xt;
+        break;
+      }
+    }
+    return true;
+  }
+
+  bool calc_adjustments() {  // mono_sba_solver.cpp:812-890 with calculateEps :681-702 and calculateSMatrix :812-870
+    std::vector<int> idx(frames.size(), -1);
+    int nfree = 0;
+    for (size_t f = 0; f < frames.size(); ++f) {
+      Frame& F = frames[f];
+      if (F.constrained) continue;
+      std::copy(F.errA, F.errA + 6, F.eps);
+      idx[f] = nfree++;
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        Frame& F = frames[K.frames[k]];
+        if (F.constrained) continue;
+        const Obs& O = obs[K.obs[k]];
+        for (int r = 0; r < 6; ++r) F.eps[r] -= O.Y[r * 3] * K.errB[0] + O.Y[r * 3 + 1] * K.errB[1] + O.Y[r * 3 + 2] * K.errB[2];
+      }
+    }
+    const int n = 6 * nfree;
+    if (n == 0) {
+      for (Track& K : tracks)
+        if (!K.removed) std::fill(K.db, K.db + 3, T(0));
+      return false;
+    }
+    std::vector<T> S(static_cast<size_t>(n) * n, T(0)), b(n);
+    for (size_t f = 0; f < frames.size(); ++f) {
+      if (idx[f] < 0) continue;
+      for (int r = 0; r < 6; ++r) {
+        b[6 * idx[f] + r] = frames[f].eps[r];
+        for (int c = 0; c < 6; ++c) S[static_cast<size_t>(6 * idx[f] + r) * n + 6 * idx[f] + c] = frames[f].Ua[r * 6 + c];
+      }
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      for (size_t j = 0; j < K.frames.size(); ++j) {
+        if (idx[K.frames[j]] < 0) continue;
+        const Obs& Oj = obs[K.obs[j]];
+        for (size_t k = 0; k < K.frames.size(); ++k) {
+          if (idx[K.frames[k]] < 0) continue;
+          const Obs& Ok = obs[K.obs[k]];
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+              S[static_cast<size_t>(6 * idx[K.frames[j]] + r) * n + 6 * idx[K.frames[k]] + c] -=
+                  Oj.Y[r * 3] * Ok.W[c * 3] + Oj.Y[r * 3 + 1] * Ok.W[c * 3 + 1] + Oj.Y[r * 3 + 2] * Ok.W[c * 3 + 2];
+        }
+      }
+    }
+    const int jj = n - 6;
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) prep[r * 6 + c] = S[static_cast<size_t>(jj + r) * n + jj + c];
+    for (int r = 0; r < 6; ++r) prep[r * 7] /= T(1) + lambda;
+    std::vector<T> x;
+    if (!svd_solve(n, S, b, x)) return false;
+    for (size_t f = 0; f < frames.size(); ++f) {
+      Frame& F = frames[f];
+      if (F.constrained) {
+        std::fill(F.da, F.da + 6, T(0));
+        continue;
+      }
+      for (int i = 0; i < 6; ++i) F.da[i] = x[6 * idx[f] + i];
+    }
+    for (Track& K : tracks) {
+      if (K.skip) continue;
+      T wda[3] = {0, 0, 0};
+      for (size_t k = 0; k < K.frames.size(); ++k) {
+        const Frame& F = frames[K.frames[k]];
+        if (F.constrained) continue;
+        const Obs& O = obs[K.obs[k]];
+        for (int c = 0; c < 3; ++c)
+          for (int r = 0; r < 6; ++r) wda[c] += O.W[r * 3 + c] * F.da[r];
+      }
+      const T pt[3] = {K.errB[0] - wda[0], K.errB[1] - wda[1], K.errB[2] - wda[2]};
+      for (int r = 0; r < 3; ++r) K.db[r] = K.invVa[r * 3] * pt[0] + K.invVa[r * 3 + 1] * pt[1] + K.invVa[r * 3 + 2] * pt[2];
+    }
+    return true;
+  }
+
+  void covariance_from_prep() {  // CompleteOrthogonalDecomposition(prep).pseudoInverse()
+    std::vector<T> P(prep, prep + 36);
+    for (int r = 0; r < 6; ++r)  // the reduced system is symmetric: take its symmetric part
+      for (int c = r + 1; c < 6; ++c) P[r * 6 + c] = P[c * 6 + r] = (P[r * 6 + c] + P[c * 6 + r]) / T(2);
+    sym_pinv<T>(6, P.data(), T(6) * T(kFltEps), cov);
+  }
+
+  int solve(bool& feasible) {  // mono_sba_solver.cpp:205-301
+    int count = max_iterations;
+    feasible = true;
+    filter_hither();
+    update_aDbD();
+    calc_residual();
+    calc_information();
+    compute_weights();
+    residual = calc_residual();
+    const T fmax = T(std::numeric_limits<float>::max());
+    if (residual == fmax) {
+      for (int i = 0; i < 36; ++i) cov[i] = T(i % 7 == 0);
+      feasible = false;
+      return 0;
+    }
+    residual_initial = residual;
+    bool new_values = true;
+    T old = residual;
+    while (min_residual <= residual && count >= 0 && lambda <= T(kMonoLambdaMax)) {
+      if (new_values) {
+        calc_derivatives();
+        calc_UVW();
+      }
+      step_UV();
+      if (calc_adjustments()) {
+        update_aDbD();
+        residual = calc_residual();
+        if (residual == fmax) {
+          feasible = false;
+          return 0;
+        }
+      } else {
+        if (residual == fmax) {
+          feasible = false;
+          return 0;
+        }
+        residual *= T(2);
+      }
+      const T delta = std::fabs(residual - old);
+      if (residual > old) {
+        lambda *= T(10);
+        new_values = false;
+      } else {
+        if (lambda > T(1e-25f)) lambda /= T(10);
+        old = residual;
+        new_values = true;
+        update();
+        filter_hither();
+        calc_information();
+        compute_weights();
+        covariance_from_prep();
+      }
+      if (min_residual_speed * residual > delta) break;
+      --count;
+    }
+    residual = old;
+    failed = lambda > T(kMonoLambdaMax) || count < 0;
+    if (failed) covariance_from_prep();
+    return iteration = max_iterations - count;
+  }
+
+  void export_state(const OrcMonoProblem* p, OrcMonoResult* r) const {
+    for (int f = 0; f < p->num_frames && r->frame_poses; ++f) {
+      const Pose<T> cw = twist_exp<T>(frames[f].a);
+      Pose<T> wc;
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) wc.R[i][j] = cw.R[j][i];
+        wc.t[i] = -(cw.R[0][i] * cw.t[0] + cw.R[1][i] * cw.t[1] + cw.R[2][i] * cw.t[2]);
+      }
+      store_pose(wc, r->frame_poses + 16 * f);
+    }
+    for (int t = 0; t < p->num_tracks && r->points; ++t)
+      for (int d = 0; d < 3; ++d) r->points[3 * t + d] = static_cast<double>(tracks[t].b[d]);
+    for (int i = 0; i < 36; ++i) r->covariance[i] = static_cast<double>(cov[i]);
+    r->residual = static_cast<double>(residual);
+    r->residual_initial = static_cast<double>(residual_initial);
+    r->lambda = static_cast<double>(lambda);
+    r->failed = failed;
+  }
+};
+
+template <class T>
+int mono_impl(const OrcMonoProblem* p, OrcMonoResult* r) {
+  MonoSolver<T> s;
+  s.load(p);
+  bool feasible = true;
+  const int it = s.solve(feasible);
+  r->iterations = it;
+  r->feasible = feasible;
+  s.export_state(p, r);
+  return feasible ? 0 : 2;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1210,5 +1862,35 @@ int orc_winsorize_f64(const OrcProblem* p, int32_t kf, const double* pts, const 
   for (int i = 0; i < p->num_points; ++i) count += reset[i];
   return count;
 }
+
+// The mono SBA solver (libs/sba/mono_sba_solver.cpp) on one window: returns 0, or 2 when no observation is usable.
+int orc_mono_solve_f64(const OrcMonoProblem* p, OrcMonoResult* r) { return mono_impl<double>(p, r); }
+int orc_mono_solve_f32(const OrcMonoProblem* p, OrcMonoResult* r) { return mono_impl<float>(p, r); }
+
+// Pieces of the mono solver for the pins: the projection derivatives (2x6 camera, 2x3 point, row-major) of world
+// point b seen through the camera with twist a; the twist Log of a pose (column-major 4x4); the twist sum.
+void orc_mono_derivs_f64(const double* a, const double* b, double* A, double* B) {
+  MonoSolver<double> s;
+  s.frames.resize(1);
+  s.frames[0].constrained = false;
+  std::copy(a, a + 6, s.frames[0].a);
+  s.tracks.resize(1);
+  std::copy(b, b + 3, s.tracks[0].b);
+  s.tracks[0].frames = {0};
+  s.tracks[0].obs = {0};
+  s.obs.resize(1);
+  s.calc_derivatives();
+  std::copy(s.obs[0].A, s.obs[0].A + 12, A);
+  std::copy(s.obs[0].B, s.obs[0].B + 6, B);
+}
+void orc_twist_log_f64(const double* m16, double* tw) {
+  Pose<double> P;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) P.R[r][c] = m16[c * 4 + r];
+    P.t[r] = m16[12 + r];
+  }
+  twist_log<double>(P, tw);
+}
+void orc_twist_add_f64(const double* lhs, const double* rhs, double* out) { twist_add<double>(lhs, rhs, out); }
 
 }  // extern "C"

@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "liboracle_sba.so")
 
 class OrcOptions(C.Structure):
     _fields_ = [("loss", C.c_int32), ("rcs_solver", C.c_int32), ("dense_schur", C.c_int32),
-                ("fixed_iterations", C.c_int32), ("reorthonormalize", C.c_int32), ("pad", C.c_int32 * 3)]
+                ("fixed_iterations", C.c_int32), ("reorthonormalize", C.c_int32), ("round_reduced_f32", C.c_int32),
+                ("flip_iteration", C.c_int32), ("pad", C.c_int32)]
 
 
 class OrcResult(C.Structure):
@@ -83,10 +84,12 @@ def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def options(loss=0, rcs_solver=0, dense_schur=0, fixed_iterations=0, reorthonormalize=1):
+def options(loss=0, rcs_solver=0, dense_schur=0, fixed_iterations=0, reorthonormalize=1, round_reduced_f32=0,
+            flip_iteration=0):
     o = OrcOptions()
     o.loss, o.rcs_solver, o.dense_schur = loss, rcs_solver, dense_schur
     o.fixed_iterations, o.reorthonormalize = fixed_iterations, reorthonormalize
+    o.round_reduced_f32, o.flip_iteration = round_reduced_f32, flip_iteration
     return o
 
 
@@ -271,3 +274,82 @@ def winsorize(problem, key_frame, points=None, poses=None):
     mask = np.zeros(problem.points.shape[0], np.uint8)
     n = lib().orc_winsorize_f64(C.byref(v), key_frame, sp, sq, mask.ctypes.data_as(C.POINTER(C.c_uint8)))
     return mask.astype(bool), n
+
+
+# ---- the mono SBA solver (libs/sba/mono_sba_solver.cpp) -------------------------------------------------------------
+class OrcMonoProblem(C.Structure):
+    _fields_ = [("num_frames", C.c_int32), ("num_tracks", C.c_int32), ("num_obs", C.c_int32),
+                ("constrained", C.POINTER(C.c_int32)), ("frame_poses", C.POINTER(C.c_float)),
+                ("points", C.POINTER(C.c_float)), ("obs_frame", C.POINTER(C.c_int32)),
+                ("obs_track", C.POINTER(C.c_int32)), ("obs_xy", C.POINTER(C.c_float)), ("max_iterations", C.c_int32),
+                ("min_residual", C.c_float), ("min_residual_speed", C.c_float)]
+
+
+class OrcMonoResult(C.Structure):
+    _fields_ = [("frame_poses", C.POINTER(C.c_double)), ("points", C.POINTER(C.c_double)),
+                ("covariance", C.c_double * 36), ("residual", C.c_double), ("residual_initial", C.c_double),
+                ("lambda_", C.c_double), ("iterations", C.c_int32), ("failed", C.c_int32), ("feasible", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+def mono_solve(window, precision="f64"):
+    """The mono solver on a window dict (constrained (F,), frame_poses (F,4,4) world_from_camera, points (T,3),
+    obs_frame, obs_track (O,), obs_xy (O,2), max_iterations, min_residual, min_residual_speed)."""
+    L = lib()
+    for name in ("orc_mono_solve_f64", "orc_mono_solve_f32"):
+        getattr(L, name).argtypes = [C.POINTER(OrcMonoProblem), C.POINTER(OrcMonoResult)]
+        getattr(L, name).restype = C.c_int
+    con = np.ascontiguousarray(window["constrained"], np.int32)
+    poses = np.ascontiguousarray(np.asarray(window["frame_poses"], np.float32).transpose(0, 2, 1))
+    pts = np.ascontiguousarray(window["points"], np.float32)
+    of = np.ascontiguousarray(window["obs_frame"], np.int32)
+    ot = np.ascontiguousarray(window["obs_track"], np.int32)
+    xy = np.ascontiguousarray(window["obs_xy"], np.float32)
+    p = OrcMonoProblem(len(con), len(pts), len(of), con.ctypes.data_as(C.POINTER(C.c_int32)),
+                       poses.ctypes.data_as(C.POINTER(C.c_float)), pts.ctypes.data_as(C.POINTER(C.c_float)),
+                       of.ctypes.data_as(C.POINTER(C.c_int32)), ot.ctypes.data_as(C.POINTER(C.c_int32)),
+                       xy.ctypes.data_as(C.POINTER(C.c_float)), int(window.get("max_iterations", 1000)),
+                       float(window.get("min_residual", 0.001)), float(window.get("min_residual_speed", 0.001)))
+    out_poses = np.zeros((len(con), 16))
+    out_pts = np.zeros((len(pts), 3))
+    r = OrcMonoResult()
+    r.frame_poses, r.points = _d(out_poses), _d(out_pts)
+    fn = L.orc_mono_solve_f64 if precision == "f64" else L.orc_mono_solve_f32
+    rc = fn(C.byref(p), C.byref(r))
+    return dict(status=rc, frame_poses=out_poses.reshape(-1, 4, 4).transpose(0, 2, 1), points=out_pts,
+                covariance=np.array(r.covariance).reshape(6, 6), residual=r.residual,
+                residual_initial=r.residual_initial, lam=r.lambda_, iterations=r.iterations, failed=bool(r.failed),
+                feasible=bool(r.feasible))
+
+
+def mono_derivs(a, b):
+    """(A 2x6, B 2x3): the mono solver's projection derivatives of world point b through camera twist a."""
+    L = lib()
+    L.orc_mono_derivs_f64.argtypes = [C.POINTER(C.c_double)] * 4
+    L.orc_mono_derivs_f64.restype = None
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    A, B = np.zeros(12), np.zeros(6)
+    L.orc_mono_derivs_f64(_d(a), _d(b), _d(A), _d(B))
+    return A.reshape(2, 6), B.reshape(2, 3)
+
+
+def twist_log(M):
+    L = lib()
+    L.orc_twist_log_f64.argtypes = [C.POINTER(C.c_double)] * 2
+    L.orc_twist_log_f64.restype = None
+    m = np.ascontiguousarray(np.asarray(M, np.float64).T)
+    tw = np.zeros(6)
+    L.orc_twist_log_f64(_d(m), _d(tw))
+    return tw
+
+
+def twist_add(a, b):
+    L = lib()
+    L.orc_twist_add_f64.argtypes = [C.POINTER(C.c_double)] * 3
+    L.orc_twist_add_f64.restype = None
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros(6)
+    L.orc_twist_add_f64(_d(a), _d(b), _d(out))
+    return out

@@ -1,0 +1,69 @@
+"""FP64 sensitivity of a solve to the rounding of its FP32 inputs, and to its damping decisions (test infrastructure).
+
+The solver reads FP32 inputs and computes its bulk in FP32, so its result can differ from the FP64 oracle's by
+what the oracle's own trajectory does under perturbations of the size of FP32 rounding. This measures that
+directly: the FP64 oracle is rerun on copies of the problem whose FP32 inputs (points, observation coordinates,
+pose translations) are each moved by one FP32 unit in the last place in a random direction, and the spread of
+the final costs is reported relative to the unperturbed FP64 solve. A window whose spread is large has a
+trajectory that amplifies last-bit differences (few key frames, points seen from two or three of them, not
+converged): there the distance between the FP32 path and the FP64 oracle is explained by conditioning and is
+bounded by a multiple of this spread.
+
+The spread stays at the 1e-8 level on the benchmark windows, and so does an FP32 rounding of the reduced system
+itself (the oracle's round_reduced_f32 option): neither explains a distance of 1e-6 or more. What does is a damping
+decision: a gain ratio within about 1e-3 of 0.25 or 0.75 can land on the other side of the threshold in FP32, and
+from there the two solvers follow different but equally valid LM trajectories (lambda 8x or 40x apart), whose
+costs after a fixed iteration count can differ by far more than any rounding. branch_variants() gives the FP64
+trajectories of every such flip, and a solver's result is compared with the nearest of them.
+"""
+import numpy as np
+
+from oracle import sba_oracle as O
+
+
+def nudge(problem, rng):
+    """A copy with every point coordinate, observation coordinate and pose translation moved by one FP32 ulp."""
+    q = problem.copy()
+
+    def ulp(a):
+        a = np.asarray(a, np.float32)
+        up = rng.integers(0, 2, a.shape).astype(bool)
+        return np.where(up, np.nextafter(a, np.float32(np.inf)), np.nextafter(a, np.float32(-np.inf))).astype(np.float32)
+
+    q.points = ulp(q.points)
+    q.observation_xys = ulp(q.observation_xys)
+    free = q.rig_from_world.shape[0] - q.num_fixed_key_frames
+    q.rig_from_world[:free, :3, 3] = ulp(q.rig_from_world[:free, :3, 3])
+    return q
+
+
+def fp32_input_sensitivity(problem, loss, trials=3, seed=0, fixed_iterations=1, reference=None):
+    """(max over trials of |F_perturbed - F| / F, the unperturbed FP64 result) for the FP64 oracle's solve."""
+    ref = reference if reference is not None else O.solve(problem, "f64", loss=loss, fixed_iterations=fixed_iterations)
+    rng = np.random.default_rng(seed)
+    spread = 0.0
+    for _ in range(trials):
+        r = O.solve(nudge(problem, rng), "f64", loss=loss, fixed_iterations=fixed_iterations)
+        spread = max(spread, abs(r["last_cost"] - ref["last_cost"]) / ref["last_cost"])
+    return spread, ref
+
+
+def branch_variants(problem, loss, margin=0.01, fixed_iterations=1, reference=None, **opt):
+    """The FP64 oracle's solve and, for every LM iteration whose gain ratio lies within `margin` of 0.25 or 0.75,
+    the FP64 solve that takes the other side of that threshold there. Returns [(flip iteration (0: none), result)]."""
+    ref = reference if reference is not None else O.solve(problem, "f64", loss=loss, fixed_iterations=fixed_iterations,
+                                                          **opt)
+    out = [(0, ref)]
+    ok = (ref["trace_flags"] & 6) == 0  # a factorization ran and no stop test ended the iteration
+    for k in range(ref["iterations"]):
+        rho = ref["trace_rho"][k]
+        if ok[k] and min(abs(rho - 0.25), abs(rho - 0.75)) < margin:
+            out.append((k + 1, O.solve(problem, "f64", loss=loss, fixed_iterations=fixed_iterations,
+                                       flip_iteration=k + 1, **opt)))
+    return out
+
+
+def nearest_variant(variants, cost):
+    """(relative cost distance, flip iteration, result) of the variant whose final cost is nearest `cost`."""
+    best = min(variants, key=lambda v: abs(cost - v[1]["last_cost"]))
+    return abs(cost - best[1]["last_cost"]) / best[1]["last_cost"], best[0], best[1]

@@ -1,9 +1,14 @@
 """GPU parity: the HIP path through the C ABI against the FP64 oracle on the same seeded inputs."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import cuvslam_amd as ca
 from oracle import sba_oracle as O
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -193,24 +198,23 @@ def test_batch_matches_single(solver_huber):
 
 
 def test_every_window_of_an_uneven_batch_matches_oracle():
-    # windows of 5 to 40 key frames, points seen from two key frames up to all of them, outliers up to 10%: each
-    # window of the batch against its own FP64 oracle solve. Windows of few key frames whose points are seen from
-    # two or three of them are weakly constrained and not converged after 8 iterations: there the FP32 path's
-    # last-bit differences grow to 1e-6 .. 7e-6 of the cost over the trajectory (measured over variants of the
-    # summation order, at most two windows of ten past 1e-6); most stay within 1e-6.
+    # windows of 5 to 40 key frames, points seen from two key frames up to all of them, outliers up to 10%, exactly 8
+    # iterations: each window of the batch against its own FP64 oracle solve at 1e-6. A window whose FP64 gain ratio
+    # comes within 0.01 of a damping threshold (here window 9: 0.7512 at iteration 3) is compared with the nearest
+    # of the FP64 trajectories that take either side of it (tests/sensitivity.py).
+    from sensitivity import branch_variants, nearest_variant
     probs = ca.scene.batched_windows(10, seed=5, max_iterations=8, pt_range=(300, 2500))
     qs = [p.copy() for p in probs]
-    assert all(ca.SchurComplementBundlerGpu(loss=HUBER).solve_batch(qs))
-    errs = []
+    assert all(ca.SchurComplementBundlerGpu(loss=HUBER, fixed_iterations=True).solve_batch(qs))
+    flips = []
     for p, q in zip(probs, qs):
-        ref = O.solve(p, "f64", loss=HUBER)
-        assert q.iterations == ref["iterations"]
-        assert q.accepted == ref["accepted"]
+        err, flip, ref = nearest_variant(branch_variants(p, HUBER, fixed_iterations=1), q.last_cost_f64)
+        assert err < 1e-6, (err, flip)
+        assert (q.iterations, q.accepted) == (ref["iterations"], ref["accepted"])
         c, _ = O.cost(q, loss=HUBER)
         assert abs(c - q.last_cost_f64) / ref["last_cost"] < 1e-9  # the returned state has the reported cost
-        errs.append(abs(q.last_cost_f64 - ref["last_cost"]) / ref["last_cost"])
-    assert max(errs) < 5e-5, errs
-    assert sum(e < 1e-6 for e in errs) >= 6, errs
+        flips.append(flip)
+    assert sum(f != 0 for f in flips) <= 2, flips
 
 
 def test_autotuner_and_chunking_leave_results_unchanged():
@@ -291,3 +295,30 @@ def test_winsorize_needs_fifty_observations_and_checks_arguments(solver_huber):
     with pytest.raises(ca.SbaError):
         b.winsorize(4)
     b.close()
+
+
+def test_repeated_solves_are_bitwise_identical():
+    # every sum of a solve has a fixed order (chunk records, per-point pair sums, per-chunk cost slots), so repeated
+    # runs of a batch and repeated single-window solves return the same bits
+    probs = ca.scene.batched_windows(6, seed=9, max_iterations=6, pt_range=(300, 2000))
+    s = ca.SchurComplementBundlerGpu(loss=HUBER, fixed_iterations=True)
+    b = s.batch(probs)
+    runs = []
+    for _ in range(3):
+        b.run()
+        qs = [p.copy() for p in probs]
+        b.read(qs)
+        runs.append(qs)
+    for qs in runs[1:]:
+        for a, c in zip(runs[0], qs):
+            assert a.last_cost_f64 == c.last_cost_f64 and a.final_lambda == c.final_lambda
+            np.testing.assert_array_equal(a.points, c.points)
+            np.testing.assert_array_equal(a.rig_from_world, c.rig_from_world)
+    b.close()
+    singles = []
+    for _ in range(2):
+        q = probs[0].copy()
+        assert s.solve(q)
+        singles.append(q)
+    assert singles[0].last_cost_f64 == singles[1].last_cost_f64
+    np.testing.assert_array_equal(singles[0].points, singles[1].points)

@@ -383,3 +383,12 @@ def test_winsorize_needs_fifty_observations():
     assert e.size < 50
     mask, n = O.winsorize(p, 0)
     assert n == 0 and not mask.any()
+
+
+def test_oracle_is_clean_under_address_and_undefined_behaviour_sanitizers():
+    # every oracle entry point on generated windows, built with -fsanitize=address,undefined (oracle/asan_driver.cpp)
+    import subprocess
+    here = os.path.join(os.path.dirname(HERE), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "asan_driver"], check=True)
+    out = subprocess.run([os.path.join(here, "asan_driver")], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.strip() == "OK", out.stdout + out.stderr
