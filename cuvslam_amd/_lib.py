@@ -75,6 +75,7 @@ SIGNATURES = {
     "sba_last_error": (C.c_char_p, [C.c_void_p]),
     "sba_abi_version": (C.c_int, []),
     "sba_solve": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.POINTER(Result)]),
+    "sba_solve_noalloc": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.POINTER(Result)]),
     "sba_solve_batch": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.c_int32, C.POINTER(Result)]),
     "sba_batch_create": (C.c_int, [C.c_void_p, C.POINTER(ProblemView), C.c_int32, C.POINTER(C.c_void_p)]),
     "sba_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -49,6 +49,7 @@ hipError_t launch_gather(const Batch& b, hipStream_t s);
 hipError_t launch_copy_calibrate(float4* dst, const float4* src, int64_t n, hipStream_t s);
 hipError_t launch_mfma_layout(const float* A, const float* B, float* D, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+int scratch_setup(int device, void** ptr, size_t* bytes);
 hipError_t set_kernel_attributes();
 }  // namespace sba_dev
 
@@ -93,6 +94,9 @@ struct sba_handle {
   char* ws = nullptr;
   size_t ws_bytes = 0;
   HostPack pack;  // host packing buffers, kept across calls
+  // this device's address of the fixed scratch array of sba_solve_noalloc (looked up by sba_create)
+  char* scratch = nullptr;
+  size_t scratch_bytes = 0;
 };
 
 struct sba_batch {
@@ -467,7 +471,7 @@ struct Layout {
   }
 };
 
-// Where a batch's device block comes from: its own allocation or the handle's workspace.
+// Where a batch's device block comes from: its own allocation, the handle's workspace, or the scratch array.
 using MemorySource = std::function<int(sba_batch* b, size_t bytes, char** base)>;
 
 // Binds the layout to `base`, uploads the host streams and clears the zeroed ones on the handle's stream, then
@@ -576,6 +580,13 @@ int sba_create(const sba_options* opt, sba_handle** out) {
     return SBA_E_DEVICE;
   }
   (void)set_kernel_attributes();  // raises the dynamic shared-memory limit where the runtime needs it
+  void* scratch = nullptr;
+  if (scratch_setup(h->opt.device, &scratch, &h->scratch_bytes) != 0) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SBA_E_DEVICE;
+  }
+  h->scratch = static_cast<char*>(scratch);
   *out = h;
   return SBA_OK;
 }
@@ -1636,6 +1647,17 @@ int sba_solve_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, 
 int sba_solve(sba_handle* h, const sba_problem_view* problem, sba_result* result) {
   return solve_one(h, problem, result, [h](sba_batch*, size_t bytes, char** base) {
     return workspace_memory(h, bytes, base);
+  });
+}
+
+int sba_solve_noalloc(sba_handle* h, const sba_problem_view* problem, sba_result* result) {
+  return solve_one(h, problem, result, [h](sba_batch*, size_t bytes, char** base) -> int {
+    if (!h->scratch) return fail(h, SBA_E_DEVICE, "the scratch array of sba_solve_noalloc is not set up");
+    if (bytes > h->scratch_bytes)
+      return fail(h, SBA_E_CAPACITY, "window needs " + std::to_string(bytes) + " bytes of device memory; "
+                  "sba_solve_noalloc holds at most " + std::to_string(h->scratch_bytes) + " (use sba_solve)");
+    *base = h->scratch;
+    return SBA_OK;
   });
 }
 

@@ -110,6 +110,17 @@ int sba_abi_version(void);
  * The return value is the window's status; SBA_OK corresponds to the reference's `true`. */
 int sba_solve(sba_handle* h, const sba_problem_view* problem, sba_result* result);
 
+/* sba_solve without any device allocation: no reference counterpart. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The window's
+ * streams, solver state and reduced system are carved out of a fixed device scratch array of 64 MiB that the
+ * library holds on each GPU it runs on until the process ends (sba_create looks up its address); no hipMalloc,
+ * hipFree or other allocating call is made, and the solve is launched eagerly on the handle's stream (what must
+ * start at zero is cleared by a fill kernel, not a memset). The host side packs into ordinary host memory.
+ * A window that needs more than the scratch holds returns SBA_E_CAPACITY and changes nothing (sba_solve takes any
+ * size). Every handle on one GPU shares that GPU's array: calls that overlap on one GPU (several threads or
+ * streams) race on it and must be serialized by the caller. Results are those of sba_solve. */
+int sba_solve_noalloc(sba_handle* h, const sba_problem_view* problem, sba_result* result);
+
 /* Solves n independent windows of uneven size in one batch: every kernel of an LM iteration covers all
  * windows, and the iterations run as one hipGraph. No reference counterpart (the reference solves one
  * window per call). Returns SBA_OK when the batch ran; per-window statuses are in results[i].status. */

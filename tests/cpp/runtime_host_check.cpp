@@ -2,7 +2,7 @@
 // stand-ins of the HIP calls it makes, whose "device" memory is host memory, and of the kernel launches, which run
 // nothing but check the batch they are handed: every stream a kernel reads or writes is bound, lies inside a live
 // allocation with room for what the kernels index, and does not overlap another stream. The entry points are then
-// driven the way the tests drive them (one window, the handle's workspace growing and shrinking,
+// driven the way the tests drive them (one window, the handle's workspace growing and shrinking, the fixed scratch,
 // uneven batches, multi-camera windows whose pairs hold several observations, a mapped window).
 // Prints one line per case; exits non-zero on a failed check.
 #include <hip/hip_runtime.h>
@@ -246,6 +246,17 @@ hipError_t launch_winsorize_apply(const Batch&, int, int, int, const uint8_t*, h
 hipError_t launch_copy_calibrate(float4*, const float4*, int64_t, hipStream_t) { return hipSuccess; }
 hipError_t launch_mfma_layout(const float*, const float*, float*, hipStream_t) { return hipSuccess; }
 hipError_t set_kernel_attributes() { return hipSuccess; }
+static void* g_scratch = nullptr;
+constexpr size_t kStubScratch = size_t(64) << 20;
+int scratch_setup(int, void** ptr, size_t* bytes) {
+  if (!g_scratch) {
+    g_scratch = std::calloc(1, kStubScratch);
+    g_allocs[static_cast<char*>(g_scratch)] = kStubScratch;
+  }
+  *ptr = g_scratch;
+  *bytes = kStubScratch;
+  return 0;
+}
 }  // namespace sba_dev
 
 // ---- driving the entry points ---------------------------------------------------------------------------------
@@ -308,6 +319,25 @@ int main() {
     sba_result r = result_for(sc);
     const int rc = sba_solve(h, &sc.v, &r);
     CHECK(rc == SBA_OK, (std::string("sba_solve: ") + sba_last_error(h)).c_str());
+  }
+
+  g_case = "solve_noalloc: no allocation, the capacity error";
+  std::printf("%s\n", g_case.c_str());
+  {
+    const int mallocs = g_mallocs, frees = g_frees;
+    g_forbid_alloc = true;
+    Scene a, b;
+    make_scene(a, 14, 3000, 21);
+    make_scene(b, 6, 600, 22);
+    sba_result ra = result_for(a), rb = result_for(b);
+    CHECK(sba_solve_noalloc(h, &a.v, &ra) == SBA_OK, sba_last_error(h));
+    CHECK(sba_solve_noalloc(h, &b.v, &rb) == SBA_OK, sba_last_error(h));
+    Scene big;
+    make_scene(big, 40, 40000, 23);
+    sba_result rbig = result_for(big);
+    CHECK(sba_solve_noalloc(h, &big.v, &rbig) == SBA_E_CAPACITY, "capacity error");
+    g_forbid_alloc = false;
+    CHECK(g_mallocs == mallocs && g_frees == frees, "no allocation or free in sba_solve_noalloc");
   }
 
   g_case = "batch: uneven windows, graph";
@@ -390,7 +420,8 @@ int main() {
   }
   sba_destroy(h);
   g_case = "leaks";
-  CHECK(g_allocs.empty(), "every allocation freed");
+  // the fake scratch stays (it lives as long as the process); everything else is freed
+  CHECK(g_allocs.size() == 1, "every allocation freed");
   std::printf("%s (%d failed checks, %d launches)\n", g_fail ? "FAILED" : "OK", g_fail, sba_dev::g_launches);
   return g_fail ? 1 : 0;
 }
