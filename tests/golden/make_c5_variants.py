@@ -1,7 +1,8 @@
 """Writes tests/golden/c5_variants.json: the FP64 oracle's solve of the 200-key-frame multi-camera map (seed 5,
-10 LM iterations, Huber) and the FP64 solves that take the other side of each damping threshold its gain ratio
-comes within 0.01 of (tests/sensitivity.py: branch_variants). The GPU test compares its solve with the nearest of
-these trajectories; the oracle takes about 25 s per solve, so the results are kept here rather than recomputed.
+10 LM iterations, Huber), the FP64 solves that take the other side of each damping threshold its gain ratio comes
+within 0.01 of (tests/sensitivity.py: branch_variants), the spread of FP64 solves under one-ulp perturbations of the
+FP32 inputs and the distance of the oracle's FP32 restatement of the method (fp32_input_sensitivity,
+fp32_method_spread). The oracle takes about 25 s per solve, so the results are kept here rather than recomputed.
 
 Run:  python tests/golden/make_c5_variants.py
 """
@@ -14,15 +15,17 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import cuvslam_amd as ca  # noqa: E402
-from sensitivity import branch_variants, fp32_input_sensitivity  # noqa: E402
+from sensitivity import branch_variants, fp32_input_sensitivity, fp32_method_spread  # noqa: E402
 
 
 def main():
     p, _, _ = ca.scene.loop(seed=5, max_iterations=10)
     variants = branch_variants(p, ca.LOSS_HUBER, fixed_iterations=1)
     spread, _ = fp32_input_sensitivity(p, ca.LOSS_HUBER, trials=2, reference=variants[0][1])
+    spread32, _, r32 = fp32_method_spread(p, ca.LOSS_HUBER, reference=variants[0][1])
     out = {"generator": "tests/golden/make_c5_variants.py", "scene": "ca.scene.loop(seed=5, max_iterations=10)",
-           "observations": p.num_observations, "fp32_input_spread": spread,
+           "observations": p.num_observations, "fp32_input_spread": spread, "fp32_method_spread": spread32,
+           "fp32_method_accepted": int(r32["accepted"]),
            "trace_rho": [float(x) for x in variants[0][1]["trace_rho"]],
            "trace_flags": [int(x) for x in variants[0][1]["trace_flags"]],
            "variants": [{"flip_iteration": f, "last_cost": r["last_cost"], "iterations": r["iterations"],
