@@ -7,7 +7,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcuvslam_sba.so")
 
-SBA_OK, SBA_E_EMPTY, SBA_E_INFEASIBLE, SBA_E_DEVICE, SBA_E_CAPACITY, SBA_E_INVALID = range(6)
+SBA_OK, SBA_E_EMPTY, SBA_E_INFEASIBLE, SBA_E_DEVICE, SBA_E_CAPACITY, SBA_E_INVALID, SBA_E_NO_PROGRESS = range(7)
 LOSS_STUDENT_T, LOSS_HUBER = 0, 1
 
 _fp = C.POINTER(C.c_float)
@@ -67,7 +67,51 @@ class LoopConfig(C.Structure):
     ]
 
 
-# Every exported symbol of include/cuvslam_sba.h with its signature.
+# include/cuvslam_sba_imu.h
+class ImuMeasurement(C.Structure):
+    _fields_ = [("time_ns", C.c_int64), ("linear_acceleration", C.c_float * 3), ("angular_velocity", C.c_float * 3)]
+
+
+class ImuCalibration(C.Structure):
+    _fields_ = [("rig_from_imu", C.c_float * 16), ("gyroscope_noise_density", C.c_float),
+                ("gyroscope_random_walk", C.c_float), ("accelerometer_noise_density", C.c_float),
+                ("accelerometer_random_walk", C.c_float), ("frequency", C.c_float)]
+
+
+class ImuPreintegration(C.Structure):
+    _fields_ = [("cov", C.c_float * 81), ("acc_rw_cov", C.c_float * 9), ("gyro_rw_cov", C.c_float * 9),
+                ("dR", C.c_float * 9), ("dV", C.c_float * 3), ("dP", C.c_float * 3), ("JRg", C.c_float * 9),
+                ("JVg", C.c_float * 9), ("JVa", C.c_float * 9), ("JPg", C.c_float * 9), ("JPa", C.c_float * 9),
+                ("dT_s", C.c_float), ("gyro_bias", C.c_float * 3), ("acc_bias", C.c_float * 3),
+                ("gyro_bias_updated", C.c_float * 3), ("acc_bias_updated", C.c_float * 3),
+                ("first_measurement", C.c_int32), ("num_measurements", C.c_int32)]
+
+
+class ImuProblemView(C.Structure):
+    _fields_ = [
+        ("num_points", C.c_int32), ("num_observations", C.c_int32), ("num_poses", C.c_int32),
+        ("num_fixed_key_frames", C.c_int32), ("num_cameras", C.c_int32), ("num_measurements", C.c_int32),
+        ("points", _fp), ("observation_xys", _fp), ("observation_infos", _fp),
+        ("point_ids", _ip), ("pose_ids", _ip), ("camera_ids", C.POINTER(C.c_int8)),
+        ("camera_from_rig", _fp), ("w_from_imu", _fp), ("velocity", _fp), ("gyro_bias", _fp), ("acc_bias", _fp),
+        ("preintegration", C.POINTER(ImuPreintegration)), ("measurements", C.POINTER(ImuMeasurement)),
+        ("gravity", C.c_float * 3), ("max_iterations", C.c_int32), ("robustifier_scale", C.c_float),
+        ("robustifier_scale_pose", C.c_float), ("prior_gyro", C.c_float), ("prior_acc", C.c_float),
+        ("imu_penalty", C.c_float), ("acc_rw_penalty", C.c_float), ("boundary_imu_penalty", C.c_float),
+        ("reintegration_thresh", C.c_float),
+    ]
+
+
+class ImuResult(C.Structure):
+    _fields_ = [
+        ("points", _fp), ("w_from_imu", _fp), ("velocity", _fp), ("gyro_bias", _fp), ("acc_bias", _fp),
+        ("preintegration", C.POINTER(ImuPreintegration)), ("status", C.c_int32), ("iterations", C.c_int32),
+        ("accepted", C.c_int32), ("initial_cost", C.c_float), ("lambda_", C.c_float), ("pad", C.c_int32),
+        ("initial_cost_f64", C.c_double), ("last_cost_f64", C.c_double),
+    ]
+
+
+# Every exported symbol of include/cuvslam_sba.h and include/cuvslam_sba_imu.h with its signature.
 SIGNATURES = {
     "sba_default_options": (None, [C.POINTER(Options)]),
     "sba_create": (C.c_int, [C.POINTER(Options), C.POINTER(C.c_void_p)]),
@@ -103,6 +147,11 @@ SIGNATURES = {
     "sba_scene_view": (C.c_int, [C.c_void_p, C.POINTER(ProblemView)]),
     "sba_scene_ground_truth": (C.c_int, [C.c_void_p, C.POINTER(_fp), C.POINTER(_fp)]),
     "sba_scene_free": (None, [C.c_void_p]),
+    "sba_imu_solve": (C.c_int, [C.c_void_p, C.POINTER(ImuCalibration), C.POINTER(ImuProblemView),
+                                C.POINTER(ImuResult)]),
+    "sba_imu_integrate": (C.c_int, [C.POINTER(ImuCalibration), C.POINTER(ImuMeasurement), C.c_int32, _fp, _fp,
+                                    C.POINTER(ImuPreintegration)]),
+    "sba_imu_information": (C.c_int, [C.POINTER(ImuPreintegration), _fp, _fp, _fp]),
 }
 
 _lib = None

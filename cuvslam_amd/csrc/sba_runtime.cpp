@@ -97,6 +97,9 @@ struct sba_handle {
   // this device's address of the fixed scratch array of sba_solve_noalloc (looked up by sba_create)
   char* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // device workspace of the inertial solve (sba_imu_solve), kept and regrown like ws
+  char* imu_ws = nullptr;
+  size_t imu_ws_bytes = 0;
 };
 
 struct sba_batch {
@@ -596,6 +599,7 @@ int sba_destroy(sba_handle* h) {
   (void)hipSetDevice(h->opt.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->ws) (void)hipFree(h->ws);
+  if (h->imu_ws) (void)hipFree(h->imu_ws);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SBA_OK;
@@ -604,6 +608,30 @@ int sba_destroy(sba_handle* h) {
 const char* sba_last_error(const sba_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
 }  // extern "C"
+
+namespace sba_internal {
+
+hipStream_t stream_of(sba_handle* h) { return h->stream; }
+int device_of(const sba_handle* h) { return h->opt.device; }
+int set_error(sba_handle* h, int code, const std::string& msg) { return fail(h, code, msg); }
+
+int imu_workspace(sba_handle* h, size_t bytes, char** out) {
+  if (bytes > h->imu_ws_bytes) {
+    if (h->imu_ws) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      HIP_TRY(hipFree(h->imu_ws));
+      h->imu_ws = nullptr;
+      h->imu_ws_bytes = 0;
+    }
+    const size_t grown = std::max(bytes, 2 * h->imu_ws_bytes);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&h->imu_ws), grown));
+    h->imu_ws_bytes = grown;
+  }
+  *out = h->imu_ws;
+  return SBA_OK;
+}
+
+}  // namespace sba_internal
 
 namespace {
 

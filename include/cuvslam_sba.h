@@ -38,7 +38,9 @@ typedef enum sba_status {
   SBA_E_INFEASIBLE = 2,    /* every observation is inside the near plane at the start (gpu.cpp:165-168) */
   SBA_E_DEVICE = 3,        /* a HIP runtime error */
   SBA_E_CAPACITY = 4,      /* a window exceeds what the kernels support */
-  SBA_E_INVALID = 5        /* a bad argument (null pointer, index out of range, ...) */
+  SBA_E_INVALID = 5,       /* a bad argument (null pointer, index out of range, ...) */
+  SBA_E_NO_PROGRESS = 6    /* inertial SBA: the final cost is not below the initial one; the reference returns
+                              false (imu_sba_gpu_fixed_vel.cpp:362) */
 } sba_status;
 
 typedef enum sba_loss {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,7 +17,9 @@
 #include <vector>
 
 #include "../../cuvslam_amd/csrc/sba_device.h"
+#include "../../cuvslam_amd/csrc/sba_imu_device.h"
 #include "../../include/cuvslam_sba.h"
+#include "../../include/cuvslam_sba_imu.h"
 
 using namespace sba_dev;
 
@@ -259,6 +262,100 @@ int scratch_setup(int, void** ptr, size_t* bytes) {
 }
 }  // namespace sba_dev
 
+// ---- stand-ins of the inertial launches: bounds and index checks of the window they are handed ----------------
+namespace sba_imu_dev {
+static int g_step = 0;
+static double g_cost = 0;
+
+static void check_window(const Window& w) {
+  const size_t N = w.N, P = w.P, K = w.K, n = w.n, np = w.npairs, n15 = 15 * n;
+  auto atl = [](size_t v) { return std::max<size_t>(v, 1); };
+  if (!w.pose_obs || !inside(w.pose_obs, 4 * (n + 1)) || !w.blk_tri || !inside(w.blk_tri, 4 * (w.nblocks + 1))) {
+    CHECK(false, "inertial window index heads not bound");
+    return;
+  }
+  const size_t nkobs = atl(w.pose_obs[n]), ntri = atl(w.blk_tri[w.nblocks]);
+  check_ranges({{"xy", w.xy, 16 * N}, {"info", w.info, 32 * N}, {"opid", w.opid, 4 * N}, {"okid", w.okid, 4 * N},
+                {"ocid", w.ocid, 4 * N}, {"cam", w.cam, 96 * size_t(w.C)}, {"point_obs", w.point_obs, 4 * (P + 1)},
+                {"pobs", w.pobs, 4 * N}, {"point_pair", w.point_pair, 4 * (P + 1)},
+                {"pair_obs", w.pair_obs, 8 * atl(np)}, {"pair_pose", w.pair_pose, 4 * atl(np)},
+                {"pair_point", w.pair_point, 4 * atl(np)}, {"pose_obs", w.pose_obs, 4 * (n + 1)},
+                {"kobs", w.kobs, 4 * nkobs}, {"pose_pair", w.pose_pair, 4 * (n + 1)}, {"kpair", w.kpair, 4 * atl(np)},
+                {"blk_jk", w.blk_jk, 8 * atl(w.nblocks)}, {"blk_tri", w.blk_tri, 4 * (w.nblocks + 1)},
+                {"tri", w.tri, 8 * ntri}, {"pre", w.pre, sizeof(Preint) * K}, {"pts0", w.pts[0], 24 * P},
+                {"pts1", w.pts[1], 24 * P}, {"pose0", w.pose[0], 8 * kPS * K}, {"pose1", w.pose[1], 8 * kPS * K},
+                {"om", w.om, sizeof(ObsModel) * N}, {"em", w.em, sizeof(EdgeModel) * K}, {"U", w.U, 8 * n15 * n15},
+                {"ea", w.ea, 8 * n15}, {"V", w.V, 72 * P}, {"eb", w.eb, 24 * P}, {"W", w.W, 144 * atl(np)},
+                {"Vinv", w.Vinv, 72 * P}, {"rp", w.rp, 24 * P}, {"Y", w.Y, 144 * atl(np)}, {"A", w.A, 8 * n15 * n15},
+                {"rhs", w.rhs, 8 * n15}, {"dp", w.dp, 24 * P}, {"ocost", w.ocost, 8 * N}, {"ecost", w.ecost, 8 * K},
+                {"ppred", w.ppred, 8 * P}, {"pmax", w.pmax, 8 * P}, {"cpred", w.cpred, 8 * n15},
+                {"out", w.out, sizeof(IterScalars)}, {"chol_ok", w.chol_ok, 4}});
+  CHECK(n <= size_t(kMaxFree) && w.F + w.n == w.K, "free key frames");
+  for (size_t o = 0; o < N; ++o)
+    CHECK(w.opid[o] >= 0 && size_t(w.opid[o]) < P && w.okid[o] >= 0 && size_t(w.okid[o]) < K && w.ocid[o] >= 0 &&
+              w.ocid[o] < w.C, "observation indexes");
+  CHECK(w.point_obs[0] == 0 && size_t(w.point_obs[P]) == N, "point ranges");
+  std::vector<int> seen(N, 0);
+  for (size_t i = 0; i < P; ++i) {
+    CHECK(w.point_obs[i] <= w.point_obs[i + 1], "point ranges increase");
+    for (int q = w.point_obs[i]; q < w.point_obs[i + 1]; ++q) {
+      CHECK(size_t(w.opid[w.pobs[q]]) == i, "pobs groups observations by point");
+      ++seen[w.pobs[q]];
+    }
+    for (int p = w.point_pair[i]; p < w.point_pair[i + 1]; ++p) {
+      CHECK(size_t(w.pair_point[p]) == i && w.pair_pose[p] >= 0 && size_t(w.pair_pose[p]) < n, "pair indexes");
+      CHECK(w.pair_obs[2 * p] >= w.point_obs[i] && w.pair_obs[2 * p] < w.pair_obs[2 * p + 1] &&
+                w.pair_obs[2 * p + 1] <= w.point_obs[i + 1], "pair observation range inside its point's");
+      for (int q = w.pair_obs[2 * p]; q < w.pair_obs[2 * p + 1]; ++q)
+        CHECK(w.okid[w.pobs[q]] == w.F + w.pair_pose[p], "pair observations of its key frame");
+    }
+  }
+  for (size_t o = 0; o < N; ++o) CHECK(seen[o] == 1, "pobs is a permutation");
+  CHECK(size_t(w.point_pair[P]) == np && size_t(w.pose_pair[n]) == np, "pair counts");
+  for (size_t j = 0; j < n; ++j) {
+    for (int q = w.pose_obs[j]; q < w.pose_obs[j + 1]; ++q) CHECK(size_t(w.okid[w.kobs[q]]) == w.F + j, "kobs");
+    for (int q = w.pose_pair[j]; q < w.pose_pair[j + 1]; ++q) CHECK(size_t(w.pair_pose[w.kpair[q]]) == j, "kpair");
+  }
+  CHECK(size_t(w.nblocks) == n * (n + 1) / 2, "block count");
+  for (int b = 0; b < w.nblocks; ++b) {
+    const int j = w.blk_jk[2 * b], k = w.blk_jk[2 * b + 1];
+    CHECK(j >= k && k >= 0 && size_t(j) < n, "block (j, k)");
+    for (int t = w.blk_tri[b]; t < w.blk_tri[b + 1]; ++t) {
+      const int2 pr = w.tri[t];
+      CHECK(pr.x >= 0 && size_t(pr.x) < np && pr.y >= 0 && size_t(pr.y) < np, "tri pair indexes");
+      CHECK(w.pair_pose[pr.x] == j && w.pair_pose[pr.y] == k && w.pair_point[pr.x] == w.pair_point[pr.y],
+            "tri pairs of the block's key frames and one point");
+    }
+  }
+}
+
+hipError_t imu_model(const Window& w, hipStream_t) {
+  check_window(w);
+  ++sba_dev::g_launches;
+  return hipSuccess;
+}
+hipError_t imu_cost(const Window& w, hipStream_t) {
+  check_window(w);
+  g_cost = 100.0;
+  *w.out = IterScalars{g_cost, 0.0, 0.0, 0.0, 0, 1, {0, 0}};
+  return hipSuccess;
+}
+// every fourth step fails to factor; the others halve the cost with the prediction that makes rho = 1
+hipError_t imu_step(const Window& w, hipStream_t) {
+  check_window(w);
+  const int ok = (++g_step % 4) != 0;
+  *w.out = IterScalars{0.5 * g_cost, 0.5 * g_cost, 1.0, 1.0, 0, ok, {0, 0}};
+  return hipSuccess;
+}
+// the accepted state moves the free key frames' gyroscope biases past the re-integration threshold
+hipError_t imu_accept(const Window& w, hipStream_t) {
+  check_window(w);
+  g_cost *= 0.5;
+  for (int k = w.F; k < w.K; ++k) w.pose[0][kPS * k + kBg] += 0.01;
+  return hipSuccess;
+}
+}  // namespace sba_imu_dev
+
 // ---- driving the entry points ---------------------------------------------------------------------------------
 struct Scene {
   sba_scene* s = nullptr;
@@ -418,6 +515,108 @@ int main() {
     sba_batch_debug_step(h, b, 1e-3, nullptr, nullptr, nullptr, nullptr, sc8);
     sba_batch_destroy(h, b);
   }
+  g_case = "inertial solve: packing, layout, re-integration";
+  std::printf("%s\n", g_case.c_str());
+  {
+    const int K = 6, F = 2, P = 40, S = 40;
+    sba_imu_calibration cal{};
+    for (int i = 0; i < 4; ++i) cal.rig_from_imu[5 * i] = 1.f;
+    cal.gyroscope_noise_density = 1.7e-4f;
+    cal.gyroscope_random_walk = 1.9e-5f;
+    cal.accelerometer_noise_density = 2e-3f;
+    cal.accelerometer_random_walk = 3e-3f;
+    cal.frequency = 200.f;
+    std::vector<sba_imu_measurement> meas(S * (K - 1));
+    for (size_t i = 0; i < meas.size(); ++i) {
+      meas[i].time_ns = int64_t(i + 1) * 5000000;
+      meas[i].linear_acceleration[1] = -9.81f;
+      meas[i].angular_velocity[0] = 0.1f;
+    }
+    std::vector<sba_imu_preintegration> pre(K);
+    const float zero[3] = {0, 0, 0};
+    for (int k = 0; k < K; ++k) {
+      const int n = k + 1 < K ? S : 0;
+      CHECK(sba_imu_integrate(&cal, meas.data() + S * k, n, zero, zero, &pre[k]) == SBA_OK, "integrate");
+      pre[k].first_measurement = S * k;
+    }
+    std::vector<float> pts(3 * P), w_from_imu(16 * K, 0.f), vel(3 * K, 0.f), bg(3 * K, 0.f), ba(3 * K, 0.f);
+    for (int i = 0; i < P; ++i) {
+      pts[3 * i] = 0.1f * (i % 7) - 0.3f;
+      pts[3 * i + 1] = 0.1f * (i % 5) - 0.2f;
+      pts[3 * i + 2] = 4.f + 0.05f * i;
+    }
+    for (int k = 0; k < K; ++k) {
+      for (int i = 0; i < 4; ++i) w_from_imu[16 * k + 5 * i] = 1.f;
+      w_from_imu[16 * k + 12] = 0.1f * k;
+    }
+    std::vector<float> xy, info;
+    std::vector<int32_t> pid, kid;
+    std::vector<int8_t> cid;
+    for (int k = 0; k < K; ++k)
+      for (int i = 0; i < P; ++i) {
+        if ((i + k) % 3 == 0) continue;  // points seen by some key frames only
+        xy.push_back(0.f);
+        xy.push_back(0.f);
+        for (float f : {1e4f, 0.f, 0.f, 1e4f}) info.push_back(f);
+        pid.push_back(i);
+        kid.push_back(k);
+        cid.push_back(int8_t(i % 2));
+      }
+    std::vector<float> cams(32, 0.f);
+    for (int c = 0; c < 2; ++c)
+      for (int i = 0; i < 4; ++i) cams[16 * c + 5 * i] = 1.f;
+    cams[16 + 12] = -0.12f;
+    sba_imu_problem_view v{};
+    v.num_points = P;
+    v.num_observations = int(pid.size());
+    v.num_poses = K;
+    v.num_fixed_key_frames = F;
+    v.num_cameras = 2;
+    v.num_measurements = int(meas.size());
+    v.points = pts.data();
+    v.observation_xys = xy.data();
+    v.observation_infos = info.data();
+    v.point_ids = pid.data();
+    v.pose_ids = kid.data();
+    v.camera_ids = cid.data();
+    v.camera_from_rig = cams.data();
+    v.w_from_imu = w_from_imu.data();
+    v.velocity = vel.data();
+    v.gyro_bias = bg.data();
+    v.acc_bias = ba.data();
+    v.preintegration = pre.data();
+    v.measurements = meas.data();
+    v.gravity[1] = 9.81f;
+    v.max_iterations = 7;
+    v.robustifier_scale = 1.5f;
+    v.imu_penalty = 1e-2f;
+    v.acc_rw_penalty = 0.1f;
+    v.boundary_imu_penalty = 1e-2f;
+    v.reintegration_thresh = 1e-3f;
+    std::vector<float> opts(3 * P), opose(16 * K), ovel(3 * K), obg(3 * K), oba(3 * K);
+    std::vector<sba_imu_preintegration> opre(K);
+    sba_imu_result r{};
+    r.points = opts.data();
+    r.w_from_imu = opose.data();
+    r.velocity = ovel.data();
+    r.gyro_bias = obg.data();
+    r.acc_bias = oba.data();
+    r.preintegration = opre.data();
+    sba_imu_dev::g_step = 0;
+    CHECK(sba_imu_solve(h, &cal, &v, &r) == SBA_OK, sba_last_error(h));
+    CHECK(r.iterations == 7 && r.accepted == 6, "iterations: 7, the 4th failing to factor");
+    for (int k = F; k < K - 1; ++k)  // the biases moved by 0.01 per accepted step: re-integrated there
+      CHECK(std::fabs(opre[k].gyro_bias[0] - obg[3 * k]) < 1e-6f && std::fabs(obg[3 * k] - 0.06f) < 1e-6f,
+            "preintegration re-integrated at the final biases");
+    CHECK(opre[0].gyro_bias[0] == 0.f, "fixed key frames keep their preintegration");
+    v.num_fixed_key_frames = 0;
+    v.num_poses = K;
+    CHECK(sba_imu_solve(h, &cal, &v, &r) == SBA_OK, sba_last_error(h));
+    v.num_fixed_key_frames = F;
+    v.pose_ids = nullptr;
+    CHECK(sba_imu_solve(h, &cal, &v, &r) == SBA_E_INVALID, "null buffer");
+  }
+
   sba_destroy(h);
   g_case = "leaks";
   // the fake scratch stays (it lives as long as the process); everything else is freed
