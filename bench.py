@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=4, help="windows of the one-core CPU baseline (c3)")
     ap.add_argument("--no-extra-c2", action="store_true", help="c3: skip the single 30-key-frame window extra")
+    ap.add_argument("--no-extra-inertial", action="store_true", help="skip the inertial-SBA timings")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--chunk-pairs", type=int, default=0)
@@ -351,6 +352,8 @@ def main():
         extra["single_window_30kf"] = single_window_extra(ca, RF, args)
     if args.workload in ("c2", "c3"):
         extra["production_window_7kf"] = production_window_extra(ca, args)
+    if args.workload in ("c2", "c3") and not args.no_extra_inertial:
+        extra["inertial_sba"] = inertial_extra(args)
 
     line = {
         "metric": "ms per BA solve (10 LM iters) + batched solves/s; rel. cost err vs FP64 oracle",
@@ -443,6 +446,43 @@ def production_window_extra(ca, args):
            "ms_host_buffers_median": round(1e3 * float(np.median(host)), 4),
            "ms_device_resident_median": round(1e3 * float(np.median(dev)), 4)}
     log(f"[bench] production window: {out}")
+    return out
+
+
+def inertial_extra(args):
+    """The inertial solve (IMUBundlerGpuFixedVel::solve, 10 LM iterations, the settings of run_imu_sba()) from host
+    buffers on the reference's own benchmark shapes (libs/cuda_modules/test/imu_sba_test.cpp:426-429: 10 key frames
+    of which 1 fixed, 450 / 800 / 1000 points on 4-frame tracks; the 20-key-frame, 5900-point initialization window),
+    median of 20 runs; beside it the FP64 oracle's solve of the same window on one core (the CPU baseline)."""
+    from cuvslam_amd import imu, imu_scene
+    out = {}
+    for name, shape, init in (("10kf_450pts", (10, 1, 450, 4), False), ("10kf_800pts", (10, 1, 800, 4), False),
+                              ("10kf_1000pts", (10, 1, 1000, 4), False), ("init_20kf_5900pts", imu_scene.INIT_SHAPE, True)):
+        p, calib, _ = imu_scene.make_scenario(1, shape=shape)
+        if init:
+            imu_scene.use_init_settings(p)
+        bundler = imu.IMUBundlerGpuFixedVel(calib)
+        ms = []
+        for k in range(25):
+            q = p.copy()
+            t0 = time.perf_counter()
+            bundler.solve(q)
+            if k >= 5:
+                ms.append(1e3 * (time.perf_counter() - t0))
+        bundler.close()
+        row = {"observations": p.num_observations, "iterations": int(q.iterations), "accepted": int(q.accepted),
+               "ms_per_solve_median": round(float(np.median(ms)), 3)}
+        if not args.no_cpu_baseline:
+            from oracle import imu_oracle
+            t = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                ref = imu_oracle.solve(p, calib)
+                t.append(1e3 * (time.perf_counter() - t0))
+            row["cpu_baseline_ms_fp64_oracle_one_core"] = round(float(np.median(t)), 3)
+            row["rel_cost_err_vs_fp64_oracle"] = abs(q.last_cost_f64 - ref["last_cost"]) / ref["last_cost"]
+        out[name] = row
+        log(f"[bench] inertial {name}: {row}")
     return out
 
 
