@@ -111,7 +111,21 @@ class ImuResult(C.Structure):
     ]
 
 
-# Every exported symbol of include/cuvslam_sba.h and include/cuvslam_sba_imu.h with its signature.
+# include/cuvslam_sba_mono.h
+class MonoWindow(C.Structure):
+    _fields_ = [("num_frames", C.c_int32), ("num_tracks", C.c_int32), ("num_obs", C.c_int32),
+                ("constrained", _ip), ("frame_poses", _fp), ("points", _fp), ("obs_frame", _ip),
+                ("obs_track", _ip), ("obs_xy", _fp), ("max_iterations", C.c_int32), ("min_residual", C.c_float),
+                ("min_residual_speed", C.c_float)]
+
+
+class MonoResult(C.Structure):
+    _fields_ = [("frame_poses", _fp), ("points", _fp), ("covariance", C.c_float * 36), ("residual", C.c_double),
+                ("residual_initial", C.c_double), ("lambda_", C.c_double), ("iterations", C.c_int32),
+                ("failed", C.c_int32), ("feasible", C.c_int32), ("status", C.c_int32)]
+
+
+# Every exported symbol of include/cuvslam_sba.h, cuvslam_sba_imu.h and cuvslam_sba_mono.h with its signature.
 SIGNATURES = {
     "sba_default_options": (None, [C.POINTER(Options)]),
     "sba_create": (C.c_int, [C.POINTER(Options), C.POINTER(C.c_void_p)]),
@@ -152,6 +166,7 @@ SIGNATURES = {
     "sba_imu_integrate": (C.c_int, [C.POINTER(ImuCalibration), C.POINTER(ImuMeasurement), C.c_int32, _fp, _fp,
                                     C.POINTER(ImuPreintegration)]),
     "sba_imu_information": (C.c_int, [C.POINTER(ImuPreintegration), _fp, _fp, _fp]),
+    "sba_mono_solve_batch": (C.c_int, [C.c_void_p, C.POINTER(MonoWindow), C.c_int32, C.POINTER(MonoResult)]),
 }
 
 _lib = None

@@ -640,7 +640,7 @@ int sba_imu_solve(sba_handle* h, const sba_imu_calibration* calib, const sba_imu
   const size_t o_pmax = L.add<double>(k.P), o_cpred = L.add<double>(n15);
   const size_t o_out = L.add<IterScalars>(1), o_chol = L.add<int32_t>(1);
   char* base = nullptr;
-  if (int rc = sba_internal::imu_workspace(h, L.off, &base); rc != SBA_OK) return rc;
+  if (int rc = sba_internal::aux_workspace(h, L.off, &base); rc != SBA_OK) return rc;
 
   Window w{};
   w.P = k.P; w.N = k.N; w.K = k.K; w.F = k.F; w.n = k.n; w.C = k.C;

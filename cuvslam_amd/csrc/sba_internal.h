@@ -13,7 +13,8 @@ hipStream_t stream_of(sba_handle* h);
 int device_of(const sba_handle* h);
 // Records msg as the handle's last error and returns code.
 int set_error(sba_handle* h, int code, const std::string& msg);
-// The handle's device workspace of the inertial solve, at least `bytes` long (regrown to twice the need).
-int imu_workspace(sba_handle* h, size_t bytes, char** out);
+// The handle's auxiliary device workspace (the inertial and the mono solves), at least `bytes` long (regrown to
+// twice the need).
+int aux_workspace(sba_handle* h, size_t bytes, char** out);
 
 }  // namespace sba_internal

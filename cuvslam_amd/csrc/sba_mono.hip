@@ -1,0 +1,28 @@
+// HIP kernel of the batched mono SBA solve: one workgroup per window runs the whole Levenberg-Marquardt loop of
+// MonoSBASolver::solve (mono_sba_solver.cpp:205-301) on the device, from the window's input poses and points to
+// its solved state, covariance and statistics (the solver body is sba_mono_body.h).
+#include <hip/hip_runtime.h>
+
+#include "sba_mono_body.h"
+
+using namespace sba_mono;
+
+__global__ __launch_bounds__(kMonoThreads) void k_mono_solve(const MonoJob* jobs) {
+  __shared__ MonoShared sh;
+  __shared__ MonoJob job;
+  if (threadIdx.x == 0) job = jobs[blockIdx.x];
+  __syncthreads();
+  MonoSolve s{job.w, sh, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x)};
+  s.run();
+  s.export_state(job.poses_out, job.pts_out);
+}
+
+namespace sba_mono {
+
+hipError_t launch_mono_solve(const MonoJob* jobs, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mono_solve, dim3(n), dim3(kMonoThreads), 0, s, jobs);
+  return hipGetLastError();
+}
+
+}  // namespace sba_mono

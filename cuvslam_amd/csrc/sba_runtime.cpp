@@ -615,7 +615,7 @@ hipStream_t stream_of(sba_handle* h) { return h->stream; }
 int device_of(const sba_handle* h) { return h->opt.device; }
 int set_error(sba_handle* h, int code, const std::string& msg) { return fail(h, code, msg); }
 
-int imu_workspace(sba_handle* h, size_t bytes, char** out) {
+int aux_workspace(sba_handle* h, size_t bytes, char** out) {
   if (bytes > h->imu_ws_bytes) {
     if (h->imu_ws) {
       HIP_TRY(hipStreamSynchronize(h->stream));

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in ("cuvslam_sba.h", "cuvslam_sba_imu.h"))
+    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in ("cuvslam_sba.h", "cuvslam_sba_imu.h",
+                                                                    "cuvslam_sba_mono.h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(sba_[a-z_0-9]+)\s*\(", text)))
 
