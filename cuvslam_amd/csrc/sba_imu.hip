@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "sba_imu_device.h"
+#include "sba_tiles.h"
 
 using namespace sba_imu_dev;
 
@@ -262,13 +263,47 @@ __global__ __launch_bounds__(kThreads) void k_imu_obs_model(Window w) {
   w.om[o] = m;
 }
 
-// Residuals and Jacobians of every inertial edge at the current state (imu_sba_fixed_vel.cpp:575-651).
-__global__ __launch_bounds__(64) void k_imu_edge_model(Window w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.K - 1) return;
+// Residuals and Jacobians of every inertial edge at the current state (imu_sba_fixed_vel.cpp:575-651), one workgroup
+// per edge: its first thread evaluates the edge, then the workgroup forms H = J^T (pen Lambda) J and g = J^T (pen
+// Lambda) r, each entry a fixed-order sum.
+__device__ void edge_model_eval(const Window& w, int i, EdgeModel& m);
+
+__global__ __launch_bounds__(kEdgeThreads) void k_imu_edge_model(Window w) {
+  __shared__ double Js[9][24], LJ[9][25];  // column 24 of LJ: pen Lambda r
+  __shared__ double info[81];
+  const int i = blockIdx.x, tid = threadIdx.x;
   EdgeModel& m = w.em[i];
-  m.active = edge_active(w, i);
-  if (!m.active) return;
+  const bool active = edge_active(w, i);
+  if (!active) {
+    if (tid == 0) m.active = 0;
+    return;
+  }
+  if (tid == 0) edge_model_eval(w, i, m);
+  if (tid < 81) info[tid] = w.pre[i].info[tid];
+  __syncthreads();  // m (global memory, written by thread 0) is visible to the workgroup
+  if (tid < 216) Js[tid / 24][tid % 24] = jac(m, tid / 24, tid % 24);
+  __syncthreads();
+  const double pen = edge_penalty(w, i);
+  if (tid < 225) {
+    const int r = tid / 25, v = tid % 25;
+    double s = 0.0;
+    for (int q = 0; q < 9; ++q) s += info[9 * r + q] * (v < 24 ? Js[q][v] : m.r[q]);
+    LJ[r][v] = pen * s;
+  }
+  __syncthreads();
+  for (int e = tid; e < 24 * 25; e += blockDim.x) {
+    const int a = e / 25, b = e % 25;
+    double s = 0.0;
+    for (int r = 0; r < 9; ++r) s += Js[r][a] * LJ[r][b];
+    if (b < 24)
+      m.H[a][b] = s;
+    else
+      m.g[a] = s;
+  }
+}
+
+__device__ void edge_model_eval(const Window& w, int i, EdgeModel& m) {
+  m.active = 1;
   EdgeEval e;
   edge_eval(w, 0, i, e);
   for (int a = 0; a < 9; ++a) m.r[a] = e.r[a];
@@ -375,24 +410,33 @@ __global__ __launch_bounds__(kThreads) void k_imu_point_full(Window w) {
 // (imu_sba_fixed_vel.cpp:701-1003 with the GPU bundler's penalties, sba_imu_v1.cu:1521-1805). Every element is
 // written by one thread: no atomics, no separate clearing.
 __global__ __launch_bounds__(kThreads) void k_imu_pose_full(Window w) {
-  __shared__ double vis[42];  // the reprojection block (6x6) and right-hand side (6)
+  constexpr int kSlices = 6;
+  __shared__ double part[42 * kSlices];  // the reprojection block (6x6) and right-hand side (6), per slice
+  __shared__ double vis[42];
   const int j = blockIdx.x, k = w.F + j, tid = threadIdx.x;
   const int n15 = 15 * w.n;
-  if (tid < 42) {
-    const int a = tid < 36 ? tid / 6 : tid - 36, b = tid % 6;
+  if (tid < 42 * kSlices) {  // value tid % 42 over the observations q = slice (mod kSlices)
+    const int val = tid % 42, slice = tid / 42;
+    const int a = val < 36 ? val / 6 : val - 36, b = val % 6;
     double s = 0.0;
-    for (int q = w.pose_obs[j]; q < w.pose_obs[j + 1]; ++q) {
+    for (int q = w.pose_obs[j] + slice; q < w.pose_obs[j + 1]; q += kSlices) {
       const int o = w.kobs[q];
       const ObsModel& m = w.om[o];
       const double* L = w.info + 4 * o;
       const double ja0 = a < 3 ? m.JR[a] : m.Jt[a - 3], ja1 = a < 3 ? m.JR[3 + a] : m.Jt[a];
-      if (tid < 36) {
+      if (val < 36) {
         const double jb0 = b < 3 ? m.JR[b] : m.Jt[b - 3], jb1 = b < 3 ? m.JR[3 + b] : m.Jt[b];
         s += m.w * (ja0 * (L[0] * jb0 + L[1] * jb1) + ja1 * (L[2] * jb0 + L[3] * jb1));
       } else {
         s -= m.w * (ja0 * (L[0] * m.r[0] + L[1] * m.r[1]) + ja1 * (L[2] * m.r[0] + L[3] * m.r[1]));
       }
     }
+    part[tid] = s;
+  }
+  __syncthreads();
+  if (tid < 42) {
+    double s = 0.0;
+    for (int sl = 0; sl < kSlices; ++sl) s += part[42 * sl + tid];
     vis[tid] = s;
   }
   __syncthreads();
@@ -404,46 +448,27 @@ __global__ __launch_bounds__(kThreads) void k_imu_pose_full(Window w) {
   const EdgeModel* en = has_next ? w.em + k : nullptr;
   const Preint* pp = has_prev ? w.pre + (k - 1) : nullptr;
   const Preint* pn = has_next ? w.pre + k : nullptr;
-  // J^T (pen Lambda) J entry between variables va, vb of one edge
-  auto hij = [](const EdgeModel& m, const Preint& pr, double pen, int va, int vb) {
-    double s = 0.0;
-    for (int r = 0; r < 9; ++r) {
-      const double jr = jac(m, r, va);
-      if (jr == 0.0) continue;
-      double t = 0.0;
-      for (int q = 0; q < 9; ++q) t += pr.info[9 * r + q] * jac(m, q, vb);
-      s += jr * t;
-    }
-    return pen * s;
-  };
   for (int e = tid; e < 15 * n15; e += blockDim.x) {
     const int a = e / n15, c = e % n15, cb = c / 15, b = c % 15;
     double v = 0.0;
+    const bool rw = a >= 9 && b >= 9 && (a < 12) == (b < 12);  // same random-walk block (gyroscope or accelerometer)
+    const bool gyro = a < 12;
+    const int ra = gyro ? a - 9 : a - 12, rb = gyro ? b - 9 : b - 12;
     if (cb == j) {
       if (a < 6 && b < 6) v += vis[6 * a + b];
-      if (has_prev && a < 9 && b < 9) v += hij(*ep, *pp, pen_prev, 15 + a, 15 + b);
-      if (has_next) v += hij(*en, *pn, pen_next, a, b);
-      if (a >= 9 && b >= 9 && (a < 12) == (b < 12)) {
-        const bool gyro = a < 12;
-        const int ra = gyro ? a - 9 : a - 12, rb = gyro ? b - 9 : b - 12;
-        if (has_prev) v += (gyro ? pen_prev * pp->igrw[3 * ra + rb] : apen * pp->iarw[3 * ra + rb]);
-        if (has_next) v += (gyro ? pen_next * pn->igrw[3 * ra + rb] : apen * pn->iarw[3 * ra + rb]);
+      if (has_prev && a < 9 && b < 9) v += ep->H[15 + a][15 + b];
+      if (has_next) v += en->H[a][b];
+      if (rw) {
+        if (has_prev) v += gyro ? pen_prev * pp->igrw[3 * ra + rb] : apen * pp->iarw[3 * ra + rb];
+        if (has_next) v += gyro ? pen_next * pn->igrw[3 * ra + rb] : apen * pn->iarw[3 * ra + rb];
       }
       if (j == 0 && a == b && a >= 9) v += a < 12 ? w.prior_gyro : w.prior_acc;
     } else if (cb == j - 1 && has_prev) {  // key frame k-1 is free: cross terms of edge k-1 (k on its right)
-      if (a < 9) v += hij(*ep, *pp, pen_prev, 15 + a, b);
-      if (a >= 9 && b >= 9 && (a < 12) == (b < 12)) {
-        const bool gyro = a < 12;
-        const int ra = gyro ? a - 9 : a - 12, rb = gyro ? b - 9 : b - 12;
-        v -= gyro ? pen_prev * pp->igrw[3 * ra + rb] : apen * pp->iarw[3 * ra + rb];
-      }
+      if (a < 9) v += ep->H[15 + a][b];
+      if (rw) v -= gyro ? pen_prev * pp->igrw[3 * ra + rb] : apen * pp->iarw[3 * ra + rb];
     } else if (cb == j + 1 && has_next) {  // cross terms of edge k (k on its left)
-      if (b < 9) v += hij(*en, *pn, pen_next, a, 15 + b);
-      if (a >= 9 && b >= 9 && (a < 12) == (b < 12)) {
-        const bool gyro = a < 12;
-        const int ra = gyro ? a - 9 : a - 12, rb = gyro ? b - 9 : b - 12;
-        v -= gyro ? pen_next * pn->igrw[3 * ra + rb] : apen * pn->iarw[3 * ra + rb];
-      }
+      if (b < 9) v += en->H[a][15 + b];
+      if (rw) v -= gyro ? pen_next * pn->igrw[3 * ra + rb] : apen * pn->iarw[3 * ra + rb];
     }
     w.U[static_cast<int64_t>(15 * j + a) * n15 + c] = v;
   }
@@ -451,19 +476,8 @@ __global__ __launch_bounds__(kThreads) void k_imu_pose_full(Window w) {
     const int a = tid;
     double v = a < 6 ? vis[36 + a] : 0.0;
     const double* X = w.pose[0] + kPS * k;
-    auto grad = [](const EdgeModel& m, const Preint& pr, double pen, int va) {
-      double s = 0.0;
-      for (int r = 0; r < 9; ++r) {
-        const double jr = jac(m, r, va);
-        if (jr == 0.0) continue;
-        double t = 0.0;
-        for (int q = 0; q < 9; ++q) t += pr.info[9 * r + q] * m.r[q];
-        s += jr * t;
-      }
-      return pen * s;
-    };
-    if (has_prev && a < 9) v -= grad(*ep, *pp, pen_prev, 15 + a);
-    if (has_next) v -= grad(*en, *pn, pen_next, a);
+    if (has_prev && a < 9) v -= ep->g[15 + a];
+    if (has_next) v -= en->g[a];
     if (a >= 9) {
       const bool gyro = a < 12;
       const int ra = gyro ? a - 9 : a - 12;
@@ -554,83 +568,173 @@ __global__ __launch_bounds__(kThreads) void k_imu_point_inv(Window w) {
 }
 
 // Block (j, k), j >= k, of S = U + lambda diag U - sum_i W_ij^T Vinv_i W_ik, one workgroup per block; workgroups
-// past the blocks form the reduced right-hand side e_a - sum_i W_ij^T Vinv_i e_b,i of one key frame each.
-__global__ __launch_bounds__(kThreads) void k_imu_reduced(Window w) {
+// past the blocks form the reduced right-hand side e_a - sum_i W_ij^T Vinv_i e_b,i of one key frame each. The sums
+// over points are split over kReducedSlices strided slices (slice s takes every kReducedSlices-th term from the s-th)
+// and the slices are then added in order, so every entry is a fixed-order sum.
+__global__ __launch_bounds__(kReducedThreads) void k_imu_reduced(Window w) {
+  __shared__ double part[36 * kReducedSlices];
   const int n15 = 15 * w.n, tid = threadIdx.x;
-  if (blockIdx.x < w.nblocks) {
+  if (blockIdx.x < w.nblocks) {  // workgroup-uniform
     const int j = w.blk_jk[2 * blockIdx.x], k = w.blk_jk[2 * blockIdx.x + 1];
-    if (tid >= 225) return;
-    const int a = tid / 15, c = tid % 15;
-    const int64_t row = 15 * j + a, col = 15 * k + c;
-    double v = w.U[row * n15 + col];
-    if (row == col) v += w.lambda * v;
-    if (a < 6 && c < 6) {
+    if (tid < 36 * kReducedSlices) {
+      const int val = tid % 36, slice = tid / 36, a = val / 6, c = val % 6;
       double s = 0.0;
-      for (int t = w.blk_tri[blockIdx.x]; t < w.blk_tri[blockIdx.x + 1]; ++t) {
+      for (int t = w.blk_tri[blockIdx.x] + slice; t < w.blk_tri[blockIdx.x + 1]; t += kReducedSlices) {
         const int2 pr = w.tri[t];
         const double* Wj = w.W + 18 * pr.x;
         const double* Yk = w.Y + 18 * pr.y;
         s += Wj[a] * Yk[c] + Wj[6 + a] * Yk[6 + c] + Wj[12 + a] * Yk[12 + c];
       }
+      part[tid] = s;
+    }
+    __syncthreads();
+    if (tid >= 256) return;
+    const int a = tid / 16, c = tid % 16;
+    double* T = w.A + 256 * static_cast<int64_t>(blockIdx.x);  // lower tile (j, k), blocks in the same order
+    if (a == 15 || c == 15) {  // padding: zero, and one on the diagonal of a diagonal tile
+      T[tid] = (a == c && j == k) ? 1.0 : 0.0;
+      return;
+    }
+    const int64_t row = 15 * j + a, col = 15 * k + c;
+    double v = w.U[row * n15 + col];
+    if (row == col) v += w.lambda * v;
+    if (a < 6 && c < 6) {
+      double s = 0.0;
+      for (int sl = 0; sl < kReducedSlices; ++sl) s += part[36 * sl + 6 * a + c];
       v -= s;
     }
-    w.A[row * n15 + col] = v;
+    T[tid] = v;
     return;
   }
   const int j = blockIdx.x - w.nblocks;
+  if (tid < 6 * kReducedSlices) {
+    const int val = tid % 6, slice = tid / 6;
+    double s = 0.0;
+    for (int q = w.pose_pair[j] + slice; q < w.pose_pair[j + 1]; q += kReducedSlices) {
+      const int p = w.kpair[q];
+      const double* Wp = w.W + 18 * p;
+      const double* r = w.rp + 3 * w.pair_point[p];
+      s += Wp[val] * r[0] + Wp[6 + val] * r[1] + Wp[12 + val] * r[2];
+    }
+    part[tid] = s;
+  }
+  __syncthreads();
   if (tid >= 15) return;
   double v = w.ea[15 * j + tid];
   if (tid < 6) {
     double s = 0.0;
-    for (int q = w.pose_pair[j]; q < w.pose_pair[j + 1]; ++q) {
-      const int p = w.kpair[q];
-      const double* Wp = w.W + 18 * p;
-      const double* r = w.rp + 3 * w.pair_point[p];
-      s += Wp[tid] * r[0] + Wp[6 + tid] * r[1] + Wp[12 + tid] * r[2];
-    }
+    for (int sl = 0; sl < kReducedSlices; ++sl) s += part[6 * sl + tid];
     v -= s;
   }
   w.rhs[15 * j + tid] = v;
 }
 
-// Dense Cholesky S = L L^T (right-looking, one column at a time, the column kept in LDS) and the two triangular
-// solves, in one workgroup; a non-positive pivot stops it and clears chol_ok (the LM loop then raises lambda,
-// imu_sba_gpu_fixed_vel.cpp:204-209). The step overwrites rhs.
+// Block Cholesky S = L L^T of the reduced system (right-looking; Golub & Van Loan, Matrix Computations, Algorithm
+// 4.2.10), over 16 x 16 FP64 tiles that hold the 15 x 15 blocks of the free key frames padded with a unit diagonal
+// (lower tile (i, j) at A + 256 (i (i + 1) / 2 + j), row-major, i.e. in the matrix-core register layout). It keeps the
+// tile sparsity pattern of L: the inertial terms couple neighbouring key frames and the reprojections the key frames
+// that see common points, so most tiles stay zero and are skipped (fill-in is tracked). Per tile column k:
+//   1. wavefront 0 factors the diagonal tile and forms X = L_kk^-1 (sba_tiles::factor_tile), stores X^T over the
+//      diagonal tile for the backward substitution, and steps the forward substitution: y_k = X b_k;
+//   2. the panel tiles below it, one wavefront each: L_ik = A_ik X^T on the matrix cores, kept transposed in LDS,
+//      and b_i -= L_ik y_k;
+//   3. the trailing tiles A_ab -= L_ak L_bk^T of every pair of panel rows, one wavefront per tile, on the matrix cores.
+// Then the backward substitution x_k = X_k^T (y_k - sum_{i > k} L_ik^T x_i), from the last tile column (column
+// sweeps: x_k is formed, then y_i -= L_ki^T x_k for the tiles left of it). A non-positive pivot clears chol_ok (the LM
+// loop raises lambda, imu_sba_gpu_fixed_vel.cpp:199-203). The step overwrites rhs. One workgroup; every barrier is
+// reached by every thread (loop bounds and the pivot flag are workgroup-uniform, read from LDS after a barrier).
 __global__ __launch_bounds__(kFactorThreads) void k_imu_factor(Window w) {
-  __shared__ double col[15 * kMaxFree];
-  __shared__ double x[15 * kMaxFree];
-  __shared__ int ok;
-  const int M = 15 * w.n, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  using namespace sba_tiles;
+  __shared__ double PT[(kMaxFree - 1) * 256];         // panel tiles of the current column, transposed
+  __shared__ double sT[256], xt[256], yk[16], rdiag[16];
+  __shared__ double x[16 * kMaxFree];                 // right-hand side, then y, then the step (padded by tiles)
+  __shared__ unsigned char nz[kMaxFree * kMaxFree];  // nz[i * nb + j], i >= j: tile (i, j) of L is not zero
+  __shared__ int rows[kMaxFree];
+  __shared__ int nrows, ok;
+  const int nb = w.n, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
-  double* A = w.A;
+  auto tile = [&](int i, int j) { return w.A + 256 * (static_cast<int64_t>(i) * (i + 1) / 2 + j); };
+  auto decode = [](int b, int& i, int& j) {  // b = i (i + 1) / 2 + j, j <= i
+    i = static_cast<int>((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (i * (i + 1) / 2 > b) --i;
+    while ((i + 1) * (i + 2) / 2 <= b) ++i;
+    j = b - i * (i + 1) / 2;
+  };
+  for (int b = wave; b < nb * (nb + 1) / 2; b += nwaves) {
+    int i, j;
+    decode(b, i, j);
+    const double* T = tile(i, j);
+    bool any = false;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) any |= T[64 * v + lane] != 0.0;
+    const bool blk = __any(any);
+    if (lane == 0) nz[i * nb + j] = (blk || i == j) ? 1 : 0;
+  }
+  for (int i = tid; i < 16 * nb; i += blockDim.x) x[i] = (i & 15) < 15 ? w.rhs[15 * (i >> 4) + (i & 15)] : 0.0;
   if (tid == 0) ok = 1;
-  for (int i = tid; i < M; i += blockDim.x) x[i] = w.rhs[i];
   __syncthreads();
-  for (int j = 0; j < M; ++j) {
-    if (tid == 0) {
-      const double d = A[static_cast<int64_t>(j) * M + j];
-      if (!(d > 0.0)) {
-        ok = 0;
-      } else {
-        const double l = sqrt(d);
-        A[static_cast<int64_t>(j) * M + j] = l;
-        col[j] = l;
+  for (int k = 0; k < nb; ++k) {
+    if (wave == 0) {
+      double* D = tile(k, k);
+      double4v t;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) t[v] = D[64 * v + lane];
+      const bool pos = factor_tile(t, sT, xt, rdiag, lane);
+      wave_lds_sync();
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D[64 * v + lane] = xt[64 * v + lane];
+      if (lane < 16) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) s += xt[16 * m + lane] * x[16 * k + m];  // X[lane][m] = X^T[m][lane]
+        yk[lane] = s;
+      }
+      wave_lds_sync();
+      if (lane < 16) x[16 * k + lane] = yk[lane];
+      if (lane == 0) {
+        if (!pos) ok = 0;
+        int n = 0;
+        for (int i = k + 1; i < nb; ++i)
+          if (nz[i * nb + k]) rows[n++] = i;
+        nrows = n;
       }
     }
     __syncthreads();
     if (!ok) break;
-    const double il = 1.0 / col[j];
-    for (int i = j + 1 + tid; i < M; i += blockDim.x) {
-      const double l = A[static_cast<int64_t>(i) * M + j] * il;
-      A[static_cast<int64_t>(i) * M + j] = l;
-      col[i] = l;
+    for (int p = wave; p < nrows; p += nwaves) {
+      double* T = tile(rows[p], k);
+      double4v c = {0.0, 0.0, 0.0, 0.0};
+      // A operand: A_ik[l % 16][4 q + l / 16]; B operand: X^T[4 q + l / 16][l % 16]
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c = mfma64(T[16 * (lane & 15) + 4 * q + (lane >> 4)], xt[64 * q + lane], c);
+      double* P = PT + 256 * p;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        T[64 * v + lane] = c[v];
+        P[16 * (lane & 15) + 4 * v + (lane >> 4)] = c[v];  // element (4 v + l / 16, l % 16) transposed
+      }
+      wave_lds_sync();
+      if (lane < 16) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) s += P[16 * m + lane] * yk[m];  // L_ik[lane][m]
+        x[16 * rows[p] + lane] -= s;
+      }
     }
     __syncthreads();
-    // trailing update of the lower triangle: one wavefront per row
-    for (int i = j + 1 + wave; i < M; i += nwaves) {
-      const double li = col[i];
-      double* Ai = A + static_cast<int64_t>(i) * M;
-      for (int c = j + 1 + lane; c <= i; c += 64) Ai[c] -= li * col[c];
+    const int npairs = nrows * (nrows + 1) / 2;
+    for (int q = wave; q < npairs; q += nwaves) {
+      int a, b;
+      decode(q, a, b);
+      const int ia = rows[a], ib = rows[b];
+      double* C = tile(ia, ib);
+      double4v c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) c[v] = C[64 * v + lane];
+      c = tile_update(c, PT + 256 * a, PT + 256 * b, lane);  // C - L_ak L_bk^T
+#pragma unroll
+      for (int v = 0; v < 4; ++v) C[64 * v + lane] = c[v];
+      if (lane == 0) nz[ia * nb + ib] = 1;
     }
     __syncthreads();
   }
@@ -638,22 +742,31 @@ __global__ __launch_bounds__(kFactorThreads) void k_imu_factor(Window w) {
     if (tid == 0) *w.chol_ok = 0;
     return;
   }
-  // L y = b, then L^T x = y
-  for (int j = 0; j < M; ++j) {
-    if (tid == 0) x[j] /= A[static_cast<int64_t>(j) * M + j];
+  for (int k = nb - 1; k >= 0; --k) {
+    if (wave == 0) {
+      if (lane < 16) {
+        const double* XT = tile(k, k);
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) s += XT[16 * lane + m] * x[16 * k + m];
+        yk[lane] = s;
+      }
+      wave_lds_sync();
+      if (lane < 16) x[16 * k + lane] = yk[lane];
+    }
     __syncthreads();
-    const double y = x[j];
-    for (int i = j + 1 + tid; i < M; i += blockDim.x) x[i] -= A[static_cast<int64_t>(i) * M + j] * y;
+    for (int i = wave; i < k; i += nwaves) {
+      if (!nz[k * nb + i] || lane >= 16) continue;
+      const double* T = tile(k, i);
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) s += T[16 * m + lane] * x[16 * k + m];  // L_ki[m][lane]
+      x[16 * i + lane] -= s;
+    }
     __syncthreads();
   }
-  for (int j = M - 1; j >= 0; --j) {
-    if (tid == 0) x[j] /= A[static_cast<int64_t>(j) * M + j];
-    __syncthreads();
-    const double y = x[j];
-    for (int i = tid; i < j; i += blockDim.x) x[i] -= A[static_cast<int64_t>(j) * M + i] * y;
-    __syncthreads();
-  }
-  for (int i = tid; i < M; i += blockDim.x) w.rhs[i] = x[i];
+  for (int i = tid; i < 16 * nb; i += blockDim.x)
+    if ((i & 15) < 15) w.rhs[15 * (i >> 4) + (i & 15)] = x[i];
   if (tid == 0) *w.chol_ok = 1;
 }
 
@@ -710,16 +823,22 @@ __global__ __launch_bounds__(64) void k_imu_pose_trial(Window w) {
   }
 }
 
-// Row r of dc^T (U + 2 lambda diag U) dc (sba_imu_v1.cu:821-852).
+// Row r of dc^T (U + 2 lambda diag U) dc (sba_imu_v1.cu:821-852), one wavefront per row. U couples a key frame only
+// with its neighbours (the inertial edges), so row r is summed over the column blocks of key frames j-1..j+1, which
+// hold every non-zero of the row (at most 45 columns, one per lane); the lanes are added by a fixed shuffle tree.
 __global__ __launch_bounds__(kThreads) void k_imu_pose_pred(Window w) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
   const int M = 15 * w.n;
-  if (r >= M || !*w.chol_ok) return;
-  const double* U = w.U + static_cast<int64_t>(r) * M;
+  if (r >= M || !*w.chol_ok) return;  // wavefront-uniform
+  const int jb = r / 15;
+  const int c0 = jb > 0 ? 15 * (jb - 1) : 0, c1 = min(M, 15 * (jb + 2));
+  const int c = c0 + lane;
   const double* dc = w.rhs;
   double s = 0.0;
-  for (int c = 0; c < M; ++c) s += U[c] * dc[c] * (c == r ? 1.0 + 2.0 * w.lambda : 1.0);
-  w.cpred[r] = dc[r] * s;
+  if (c < c1) s = w.U[static_cast<int64_t>(r) * M + c] * dc[c] * (c == r ? 1.0 + 2.0 * w.lambda : 1.0);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) w.cpred[r] = dc[r] * s;
 }
 
 // ---- cost ----------------------------------------------------------------------------------------------------------
@@ -836,7 +955,7 @@ static inline unsigned blocks(int n, int t) { return static_cast<unsigned>((n + 
 
 hipError_t imu_model(const Window& w, hipStream_t s) {
   if (w.N > 0) hipLaunchKernelGGL(k_imu_obs_model, dim3(blocks(w.N, kThreads)), dim3(kThreads), 0, s, w);
-  if (w.K > 1) hipLaunchKernelGGL(k_imu_edge_model, dim3(blocks(w.K - 1, 64)), dim3(64), 0, s, w);
+  if (w.K > 1) hipLaunchKernelGGL(k_imu_edge_model, dim3(w.K - 1), dim3(kEdgeThreads), 0, s, w);
   if (w.P > 0) hipLaunchKernelGGL(k_imu_point_full, dim3(blocks(w.P, kThreads)), dim3(kThreads), 0, s, w);
   if (w.n > 0) hipLaunchKernelGGL(k_imu_pose_full, dim3(w.n), dim3(kThreads), 0, s, w);
   return hipGetLastError();
@@ -844,12 +963,12 @@ hipError_t imu_model(const Window& w, hipStream_t s) {
 
 hipError_t imu_step(const Window& w, hipStream_t s) {
   if (w.P > 0) hipLaunchKernelGGL(k_imu_point_inv, dim3(blocks(w.P, kThreads)), dim3(kThreads), 0, s, w);
-  if (w.n > 0) hipLaunchKernelGGL(k_imu_reduced, dim3(w.nblocks + w.n), dim3(kThreads), 0, s, w);
+  if (w.n > 0) hipLaunchKernelGGL(k_imu_reduced, dim3(w.nblocks + w.n), dim3(kReducedThreads), 0, s, w);
   hipLaunchKernelGGL(k_imu_factor, dim3(1), dim3(kFactorThreads), 0, s, w);  // also sets chol_ok when n = 0
   if (w.P > 0) hipLaunchKernelGGL(k_imu_backsub, dim3(blocks(w.P, kThreads)), dim3(kThreads), 0, s, w);
   if (w.n > 0) {
     hipLaunchKernelGGL(k_imu_pose_trial, dim3(blocks(w.n, 64)), dim3(64), 0, s, w);
-    hipLaunchKernelGGL(k_imu_pose_pred, dim3(blocks(15 * w.n, kThreads)), dim3(kThreads), 0, s, w);
+    hipLaunchKernelGGL(k_imu_pose_pred, dim3(blocks(15 * w.n, kThreads / 64)), dim3(kThreads), 0, s, w);
   }
   hipLaunchKernelGGL(k_imu_cost, dim3(blocks(w.N + w.K, kThreads)), dim3(kThreads), 0, s, w, 1);
   hipLaunchKernelGGL(k_imu_reduce, dim3(1), dim3(1024), 0, s, w, 1);

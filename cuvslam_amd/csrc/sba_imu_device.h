@@ -19,7 +19,9 @@
 namespace sba_imu_dev {
 
 constexpr int kThreads = 256;
-constexpr int kFactorThreads = 1024;
+constexpr int kEdgeThreads = 256;  // one workgroup per inertial edge
+constexpr int kReducedThreads = 1024, kReducedSlices = 28;  // 36 x 28 partial sums per reduced block
+constexpr int kFactorThreads = 512;
 constexpr int kMaxFree = 64;  // free key frames of one window (reduced system order <= 960)
 // one key frame's state: R (9, row-major), t (3), v (3), bg (3), ba (3)
 constexpr int kPS = 21;
@@ -48,6 +50,10 @@ struct ObsModel {
 struct EdgeModel {
   double r[9], rg[3], ra[3];
   double J[8][27];
+  // the edge's Gauss-Newton blocks with its penalty: H = J^T (pen Lambda) J over the 24 variables, g = J^T (pen
+  // Lambda) r (the random walks and priors are added separately)
+  double H[24][24];
+  double g[24];
   int32_t active, pad;
 };
 

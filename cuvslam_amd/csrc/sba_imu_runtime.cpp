@@ -635,7 +635,8 @@ int sba_imu_solve(sba_handle* h, const sba_imu_calibration* calib, const sba_imu
   const size_t o_V = L.add<double>(9 * size_t(k.P)), o_eb = L.add<double>(3 * size_t(k.P));
   const size_t o_W = L.add<double>(18 * size_t(k.npairs)), o_Vinv = L.add<double>(9 * size_t(k.P));
   const size_t o_rp = L.add<double>(3 * size_t(k.P)), o_Y = L.add<double>(18 * size_t(k.npairs));
-  const size_t o_A = L.add<double>(n15 * n15), o_rhs = L.add<double>(n15), o_dp = L.add<double>(3 * size_t(k.P));
+  const size_t o_A = L.add<double>(256 * size_t(std::max(k.nblocks, 1)));  // lower 16 x 16 tiles
+  const size_t o_rhs = L.add<double>(n15), o_dp = L.add<double>(3 * size_t(k.P));
   const size_t o_ocost = L.add<double>(k.N), o_ecost = L.add<double>(k.K), o_ppred = L.add<double>(k.P);
   const size_t o_pmax = L.add<double>(k.P), o_cpred = L.add<double>(n15);
   const size_t o_out = L.add<IterScalars>(1), o_chol = L.add<int32_t>(1);

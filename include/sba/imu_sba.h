@@ -14,28 +14,7 @@
 
 #include "cuvslam_sba_imu.h"
 #include "sba/bundle_adjustment_problem.h"
-
-namespace cuvslam {
-
-template <int N>
-struct SquareMatrixT {  // column-major N x N
-  float m[N * N] = {};
-  static SquareMatrixT Zero() { return SquareMatrixT(); }
-  static SquareMatrixT Identity() {
-    SquareMatrixT r;
-    for (int i = 0; i < N; ++i) r.m[i * N + i] = 1.f;
-    return r;
-  }
-  float& operator()(int r, int c) { return m[c * N + r]; }
-  float operator()(int r, int c) const { return m[c * N + r]; }
-  float* data() { return m; }
-  const float* data() const { return m; }
-};
-using Matrix3T = SquareMatrixT<3>;
-using Matrix9T = SquareMatrixT<9>;
-using Matrix15T = SquareMatrixT<15>;
-
-}  // namespace cuvslam
+#include "sba/small_matrix.h"
 
 namespace cuvslam::imu {
 

@@ -286,7 +286,8 @@ static void check_window(const Window& w) {
                 {"pts1", w.pts[1], 24 * P}, {"pose0", w.pose[0], 8 * kPS * K}, {"pose1", w.pose[1], 8 * kPS * K},
                 {"om", w.om, sizeof(ObsModel) * N}, {"em", w.em, sizeof(EdgeModel) * K}, {"U", w.U, 8 * n15 * n15},
                 {"ea", w.ea, 8 * n15}, {"V", w.V, 72 * P}, {"eb", w.eb, 24 * P}, {"W", w.W, 144 * atl(np)},
-                {"Vinv", w.Vinv, 72 * P}, {"rp", w.rp, 24 * P}, {"Y", w.Y, 144 * atl(np)}, {"A", w.A, 8 * n15 * n15},
+                {"Vinv", w.Vinv, 72 * P}, {"rp", w.rp, 24 * P}, {"Y", w.Y, 144 * atl(np)},
+                {"A", w.A, 8 * 256 * atl(w.nblocks)},
                 {"rhs", w.rhs, 8 * n15}, {"dp", w.dp, 24 * P}, {"ocost", w.ocost, 8 * N}, {"ecost", w.ecost, 8 * K},
                 {"ppred", w.ppred, 8 * P}, {"pmax", w.pmax, 8 * P}, {"cpred", w.cpred, 8 * n15},
                 {"out", w.out, sizeof(IterScalars)}, {"chol_ok", w.chol_ok, 4}});
