@@ -125,7 +125,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5", "inertial", "mono"])
     ap.add_argument("--cpu-warmups", type=int, default=1)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=4, help="windows of the one-core CPU baseline (c3)")
@@ -150,6 +150,10 @@ def main():
 
     if args.workload == "c4":
         return run_c4(args, ca)
+    if args.workload == "inertial":
+        return run_inertial(args)
+    if args.workload == "mono":
+        return run_mono(args)
     t0 = time.time()
     if args.workload == "c2":
         problems = [ca.scene.generate("c2", seed=2, num_fixed_keyframes=2, max_iterations=10)[0]]
@@ -484,6 +488,79 @@ def inertial_extra(args):
         out[name] = row
         log(f"[bench] inertial {name}: {row}")
     return out
+
+
+def run_inertial(args):
+    """The inertial solve alone on the reference's initialization window (20 key frames, 5900 points; the settings of
+    the reference's init path): `steps` solves from host buffers after `warmup`, with the FP64 oracle's solve of the
+    same window on one core beside it."""
+    from cuvslam_amd import imu, imu_scene
+    p, calib, _ = imu_scene.make_scenario(1, shape=imu_scene.INIT_SHAPE)
+    imu_scene.use_init_settings(p)
+    bundler = imu.IMUBundlerGpuFixedVel(calib)
+    t = []
+    for k in range(args.warmup + args.steps):
+        q = p.copy()
+        t0 = time.perf_counter()
+        bundler.solve(q)
+        if k >= args.warmup:
+            t.append(time.perf_counter() - t0)
+    bundler.close()
+    med = float(np.median(t))
+    line = {
+        "metric": "ms per inertial SBA solve (10 LM iters); rel. cost err vs FP64 oracle",
+        "value": round(1.0 / med, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * med, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "inertial initialization window: 20 key frames (1 fixed), 5900 points, IMU at 200 Hz, "
+                               "10 LM iterations, from host buffers", "observations": p.num_observations,
+                   "iterations": int(q.iterations), "accepted": int(q.accepted)},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import imu_oracle
+        c = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref = imu_oracle.solve(p, calib)
+            c.append(time.perf_counter() - t0)
+        line["cpu_baseline"] = {"value": round(1.0 / float(np.median(c)), 4), "unit": "solves/s", "cores": 1,
+                                "kind": "port", "sample": "the same window, FP64 oracle, median of 3"}
+        line["rel_cost_err_vs_fp64_oracle"] = abs(q.last_cost_f64 - ref["last_cost"]) / ref["last_cost"]
+    print(json.dumps(line), flush=True)
+
+
+def run_mono(args):
+    """Mono SBA windows (MonoSBASolver::solve) in batches of 512, one workgroup per window, from host buffers."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import mono_cases as MC
+    from cuvslam_amd import mono
+    solver = mono.MonoSolverGpu()
+    base = [MC.mono_window(100 + k) for k in range(64)]
+    ws = (base * 8)[:512]
+    t = []
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        solver.solve_batch(ws)
+        if k >= args.warmup:
+            t.append(time.perf_counter() - t0)
+    med = float(np.median(t))
+    line = {
+        "metric": "mono SBA windows/s", "value": round(len(ws) / med, 2), "unit": "windows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * med, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "512 monocular windows per batch (8 frames, 3 constrained, 300 tracks, 0.5 px "
+                               "noise), up to 30 LM iterations with the reference's stop tests, from host buffers"},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import sba_oracle as O
+        c = []
+        for w in base[:8]:
+            t0 = time.perf_counter()
+            O.mono_solve(w)
+            c.append(time.perf_counter() - t0)
+        line["cpu_baseline"] = {"value": round(1.0 / float(np.median(c)), 2), "unit": "windows/s", "cores": 1,
+                                "kind": "port", "sample": "8 of the windows, FP64 oracle, median"}
+    print(json.dumps(line), flush=True)
 
 
 def run_c4(args, ca):

@@ -408,9 +408,11 @@ __global__ __launch_bounds__(kThreads) void k_imu_point_full(Window w) {
 // Row block j (15 rows, all 15n columns) of the pose system U and e_a of free key frame j: its reprojection terms,
 // the inertial and random-walk terms of edges k-1 and k (k = F + j), the bias priors of the first free key frame
 // (imu_sba_fixed_vel.cpp:701-1003 with the GPU bundler's penalties, sba_imu_v1.cu:1521-1805). Every element is
-// written by one thread: no atomics, no separate clearing.
-__global__ __launch_bounds__(kThreads) void k_imu_pose_full(Window w) {
-  constexpr int kSlices = 6;
+// written by one thread: no atomics, no separate clearing. The reprojection sums run over the key frame's
+// observations in kSlices strided slices, added in slice order; the edge terms are the H and g blocks that
+// k_imu_edge_model formed.
+__global__ __launch_bounds__(kPoseFullThreads) void k_imu_pose_full(Window w) {
+  constexpr int kSlices = kPoseFullThreads / 42;
   __shared__ double part[42 * kSlices];  // the reprojection block (6x6) and right-hand side (6), per slice
   __shared__ double vis[42];
   const int j = blockIdx.x, k = w.F + j, tid = threadIdx.x;
@@ -957,7 +959,7 @@ hipError_t imu_model(const Window& w, hipStream_t s) {
   if (w.N > 0) hipLaunchKernelGGL(k_imu_obs_model, dim3(blocks(w.N, kThreads)), dim3(kThreads), 0, s, w);
   if (w.K > 1) hipLaunchKernelGGL(k_imu_edge_model, dim3(w.K - 1), dim3(kEdgeThreads), 0, s, w);
   if (w.P > 0) hipLaunchKernelGGL(k_imu_point_full, dim3(blocks(w.P, kThreads)), dim3(kThreads), 0, s, w);
-  if (w.n > 0) hipLaunchKernelGGL(k_imu_pose_full, dim3(w.n), dim3(kThreads), 0, s, w);
+  if (w.n > 0) hipLaunchKernelGGL(k_imu_pose_full, dim3(w.n), dim3(kPoseFullThreads), 0, s, w);
   return hipGetLastError();
 }
 

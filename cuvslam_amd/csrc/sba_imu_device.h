@@ -21,6 +21,7 @@ namespace sba_imu_dev {
 constexpr int kThreads = 256;
 constexpr int kEdgeThreads = 256;  // one workgroup per inertial edge
 constexpr int kReducedThreads = 1024, kReducedSlices = 28;  // 36 x 28 partial sums per reduced block
+constexpr int kPoseFullThreads = 1024;  // 42 values x 24 slices of a key frame's reprojection sums
 constexpr int kFactorThreads = 512;
 constexpr int kMaxFree = 64;  // free key frames of one window (reduced system order <= 960)
 // one key frame's state: R (9, row-major), t (3), v (3), bg (3), ba (3)

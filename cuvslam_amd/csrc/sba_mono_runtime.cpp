@@ -81,22 +81,28 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
     o.blk = in.add(k.blk.data(), 4 * k.blk.size());
     o.bpr = in.add(k.bpair.data(), 4 * k.bpair.size());
   }
-  // work areas (doubles unless noted) and outputs
+  // work areas (doubles unless noted), then every window's outputs in one region (read back with one copy):
+  // scalars (8 doubles), covariance (36 doubles), poses (16 floats per frame), points (3 floats per track)
   std::vector<size_t> work(n), outp(n);
   size_t total = in.bytes.size();
   auto round = [](size_t b) { return (b + 255) & ~size_t(255); };
+  constexpr size_t kResHead = 64 + 288;
   std::vector<size_t> wbytes(n);
   for (int i = 0; i < n; ++i) {
     const MonoPack& k = packs[i];
     const size_t F = k.nF, T = k.nT, O = k.nO, m = 6 * size_t(k.nFree);
     const size_t doubles = F * (6 * 5 + 36 * 2 + 12 + 9) + T * (3 * 4 + 9 * 2 + 4) + O * (2 + 1 + 12 + 6 + 18 + 18) +
-                           3 * (m * m + 1) + 6 * (m + 1) + 2 * (m + 2) + 36 * 2 + 8 + 64;
+                           3 * (m * m + 1) + 6 * (m + 1) + 2 * (m + 2) + 36 + 64;
     wbytes[i] = round(8 * doubles + 4 * (T + 1) + 40 * 256);  // every array is rounded up to 256 bytes
     work[i] = total;
     total += wbytes[i];
-    outp[i] = total;
-    total += round(64 * F) + round(12 * T);
   }
+  const size_t res_base = total;
+  for (int i = 0; i < n; ++i) {
+    outp[i] = total - res_base;
+    total += round(kResHead + 64 * size_t(packs[i].nF) + 12 * size_t(packs[i].nT));
+  }
+  const size_t res_bytes = total - res_base;
   const size_t jobs_off = total;
   total += round(sizeof(MonoJob) * size_t(n));
   char* base = nullptr;
@@ -143,30 +149,34 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
     w.oY = take(18 * O);
     w.S = take(m * m + 1); w.Sb = take(m + 1); w.Q = take(m * m + 1); w.E = take(m * m + 1); w.ev = take(m + 1);
     w.x = take(m + 1); w.xt = take(m + 1); w.bd = take(m + 1); w.dv = take(m + 1); w.rot = take(2 * (m + 2));
-    w.prep = take(36); w.cov = take(36); w.out_scalars = take(8);
+    w.prep = take(36);
     w.tskip = reinterpret_cast<int32_t*>(p);
     p += round(4 * (T + 1));
     if (p > base + work[i] + wbytes[i]) return set_error(h, SBA_E_INVALID, "mono work area overflow");
-    jobs[i].poses_out = reinterpret_cast<float*>(base + outp[i]);
-    jobs[i].pts_out = reinterpret_cast<float*>(base + outp[i] + round(64 * F));
+    char* r = base + res_base + outp[i];
+    w.out_scalars = reinterpret_cast<double*>(r);
+    w.cov = reinterpret_cast<double*>(r + 64);
+    jobs[i].poses_out = reinterpret_cast<float*>(r + kResHead);
+    jobs[i].pts_out = reinterpret_cast<float*>(r + kResHead + 64 * F);
   }
   hipError_t e = hipMemcpyAsync(base, in.bytes.data(), in.bytes.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(MonoJob) * size_t(n), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_mono_solve(reinterpret_cast<const MonoJob*>(base + jobs_off), n, st);
   if (e != hipSuccess) return set_error(h, SBA_E_DEVICE, std::string("mono solve: ") + hipGetErrorString(e));
-  std::vector<double> sc(8), cov(36);
-  for (int i = 0; i < n && e == hipSuccess; ++i) {
+  std::vector<char> host_res(res_bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(host_res.data(), base + res_base, res_bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return set_error(h, SBA_E_DEVICE, std::string("mono results: ") + hipGetErrorString(e));
+  double sc[8], cov[36];
+  for (int i = 0; i < n; ++i) {
     const MonoPack& k = packs[i];
     sba_mono_result& r = results[i];
-    e = hipMemcpyAsync(sc.data(), jobs[i].w.out_scalars, 64, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(cov.data(), jobs[i].w.cov, 288, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && r.frame_poses && k.nF)
-      e = hipMemcpyAsync(r.frame_poses, jobs[i].poses_out, 64 * size_t(k.nF), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && r.points && k.nT)
-      e = hipMemcpyAsync(r.points, jobs[i].pts_out, 12 * size_t(k.nT), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) break;
+    const char* src = host_res.data() + outp[i];
+    std::memcpy(sc, src, 64);
+    std::memcpy(cov, src + 64, 288);
+    if (r.frame_poses && k.nF) std::memcpy(r.frame_poses, src + kResHead, 64 * size_t(k.nF));
+    if (r.points && k.nT) std::memcpy(r.points, src + kResHead + 64 * size_t(k.nF), 12 * size_t(k.nT));
     for (int q = 0; q < 36; ++q) r.covariance[q] = float(cov[q]);
     r.residual = sc[0];
     r.residual_initial = sc[1];
@@ -176,7 +186,6 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
     r.feasible = int32_t(sc[5]);
     r.status = r.feasible ? SBA_OK : SBA_E_INFEASIBLE;
   }
-  if (e != hipSuccess) return set_error(h, SBA_E_DEVICE, std::string("mono results: ") + hipGetErrorString(e));
   return SBA_OK;
 }
 
