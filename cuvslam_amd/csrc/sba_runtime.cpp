@@ -8,15 +8,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
@@ -55,20 +60,122 @@ hipError_t set_kernel_attributes();
 
 using namespace sba_dev;
 
+// Page-locked host memory for the streams that are uploaded: their copies to the device are then DMA transfers at
+// the link's rate instead of staged copies of pageable memory.
+template <class T>
+struct PinnedAllocator {
+  using value_type = T;
+  PinnedAllocator() = default;
+  template <class U>
+  PinnedAllocator(const PinnedAllocator<U>&) {}
+  T* allocate(size_t n) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess || !p)
+      throw std::bad_alloc();
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+  // resize() default-initializes: the packing writes every element, so the streams are not cleared first
+  template <class U>
+  void construct(U* p) {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+  template <class U>
+  bool operator==(const PinnedAllocator<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const PinnedAllocator<U>&) const { return false; }
+};
+template <class T>
+using pinned_vector = std::vector<T, PinnedAllocator<T>>;
+
+// Host threads kept by a handle: run(n, fn) calls fn(0) .. fn(n - 1), fn(0) on the calling thread, and returns when
+// all have returned. Threads are started on first need and end with the handle, so a solve does not pay for
+// thread creation. Not reentrant: fn must not call run().
+class HostPool {
+ public:
+  HostPool() = default;
+  HostPool(const HostPool&) = delete;
+  HostPool& operator=(const HostPool&) = delete;
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  void run(int n, const std::function<void(int)>& fn) {
+    if (n <= 1) {
+      fn(0);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(m_);
+    while (static_cast<int>(threads_.size()) < n - 1) {
+      const int id = static_cast<int>(threads_.size()) + 1;
+      threads_.emplace_back([this, id] { worker(id); });
+    }
+    job_ = &fn;
+    job_n_ = n;
+    pending_ = static_cast<int>(threads_.size());
+    ++gen_;
+    lk.unlock();
+    wake_.notify_all();
+    fn(0);
+    lk.lock();
+    done_.wait(lk, [this] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void worker(int id) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(m_);
+    for (;;) {
+      wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      const std::function<void(int)>* job = job_;
+      const int n = job_n_;
+      lk.unlock();
+      if (id < n) (*job)(id);
+      lk.lock();
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::mutex m_;
+  std::condition_variable wake_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void(int)>* job_ = nullptr;
+  int job_n_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// One window's sort order and pair counts, from the first packing phase to the third.
+struct WindowSort {
+  std::vector<int32_t> cnt, start, order, npair, nfree, first_pair;
+  int npairs = 0;
+};
+
 // The host side of a batch while it is packed. A handle keeps one and clears it per batch, so that its vectors keep
 // their capacity across calls: a repeated solve writes into memory that is already mapped (first-touch page faults
-// of fresh buffers cost as much as the packing itself).
+// of fresh buffers cost as much as the packing itself). The uploaded streams are page-locked.
 struct HostPack {
-  std::vector<float2> obs_xy;
-  std::vector<float4> obs_info;
-  std::vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
-  std::vector<int4> pair_rec;
-  std::vector<float2> pair_xy;
-  std::vector<float4> pair_info;
-  std::vector<float4> pts, poses, cams;
-  std::vector<Tile> tiles, btiles;
-  // per-window scratch of the packing
-  std::vector<int32_t> cnt, start, order, npair, nfree, first_pair;
+  pinned_vector<float2> obs_xy;
+  pinned_vector<float4> obs_info;
+  pinned_vector<int32_t> pair_obs, pair_pose, point_pair, point_nfree, obs_orig;
+  pinned_vector<int4> pair_rec;
+  pinned_vector<float2> pair_xy;
+  pinned_vector<float4> pair_info;
+  pinned_vector<float4> pts, poses, cams;
+  pinned_vector<Tile> tiles, btiles;
+  // per-window scratch of the packing (the sort order and pair counts, the tiles before they are joined)
+  std::vector<WindowSort> sorts;
+  std::vector<std::vector<Tile>> wtiles, wbtiles;
   void clear() {
     for (auto* v : {&pair_obs, &pair_pose, &point_pair, &point_nfree, &obs_orig}) v->clear();
     obs_xy.clear();
@@ -94,6 +201,7 @@ struct sba_handle {
   char* ws = nullptr;
   size_t ws_bytes = 0;
   HostPack pack;  // host packing buffers, kept across calls
+  HostPool pool;  // host threads of the packing
   // this device's address of the fixed scratch array of sba_solve_noalloc (looked up by sba_create)
   char* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -144,57 +252,50 @@ int fail(sba_handle* h, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(h, SBA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Room for `extra` more elements with geometric growth: an exact reserve per window would copy every earlier window
-// again (quadratic in the batch).
-template <class V>
-void reserve_more(V& v, size_t extra) {
-  if (v.size() + extra > v.capacity()) v.reserve(std::max(v.size() + extra, 2 * v.capacity()));
-}
-
-// Runs fn(t, begin, end) over [0, n) cut into `nthreads` contiguous ranges, range t on its own thread (range 0 on
-// the caller's).
+// Runs fn(t, begin, end) over [0, n) cut into `nthreads` contiguous ranges, range t on the handle's pool thread t
+// (range 0 on the caller's).
 template <class F>
-void parallel_ranges(int nthreads, int64_t n, const F& fn) {
+void parallel_ranges(sba_handle* h, int nthreads, int64_t n, const F& fn) {
   if (nthreads <= 1 || n < 2 * nthreads) {
     fn(0, int64_t(0), n);
     return;
   }
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nthreads; ++t) pool.emplace_back(fn, t, n * t / nthreads, n * (t + 1) / nthreads);
-  fn(0, int64_t(0), n / nthreads);
-  for (auto& th : pool) th.join();
+  h->pool.run(nthreads, [&](int t) { fn(t, n * t / nthreads, n * (t + 1) / nthreads); });
 }
 
-// Host threads that pack one window: the option when set, else up to 16 for windows of 65536 observations or more.
-int pack_threads(const sba_handle* h, int no) {
+// Host threads of the packing: the option when set, else up to 16 (for one window, only when it has 65536
+// observations or more).
+int pack_threads(const sba_handle* h, int64_t work) {
   if (h->opt.pack_threads > 0) return h->opt.pack_threads;
-  if (no < (1 << 16)) return 1;
+  if (work < (1 << 16)) return 1;
   return static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
 }
 
-// Validates one window and appends it to the packed streams. The phases over observations and points run on
-// pack_threads() host threads; every output element has one writer, so the result does not depend on the count.
-int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, WindowDesc& W) {
+// Packing phase 1 of window w: validates it, fills the sizes of W and sorts its observations into the order
+// (point, key frame, camera, input index): a stable counting sort by point (per-thread histograms, then each
+// thread scatters its own input range), then within each point a stable sort by (key frame, camera); then the
+// pairs (one per run of equal key frames in a point's segment) are counted per point. T threads share the work;
+// every output element has one writer, so the result does not depend on T. On failure `err` says why.
+int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSort& S, WindowDesc& W, std::string& err) {
   const int np = p.num_points, nc = p.num_poses, no = p.num_observations, ncam = p.num_cameras;
-  if (np < 0 || nc < 0 || no < 0 || ncam < 0) return fail(h, SBA_E_INVALID, "negative size in window " + std::to_string(w));
+  const std::string where = " in window " + std::to_string(w);
+  if (np < 0 || nc < 0 || no < 0 || ncam < 0) return err = "negative size" + where, SBA_E_INVALID;
   if (p.num_fixed_points < 0 || p.num_fixed_points > np || p.num_fixed_key_frames < 0 || p.num_fixed_key_frames > nc)
-    return fail(h, SBA_E_INVALID, "fixed counts out of range in window " + std::to_string(w));
+    return err = "fixed counts out of range" + where, SBA_E_INVALID;
   if ((np && !p.points) || (nc && !p.rig_from_world) || (ncam && !p.camera_from_rig) ||
       (no && (!p.observation_xys || !p.observation_infos || !p.point_ids || !p.pose_ids || !p.camera_ids)))
-    return fail(h, SBA_E_INVALID, "null buffer in window " + std::to_string(w));
+    return err = "null buffer" + where, SBA_E_INVALID;
   const int ncf = nc - p.num_fixed_key_frames;
-  if (nc > kMaxPoses) return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + " has more than 256 key frames");
-  if (ncam > kMaxCameras) return fail(h, SBA_E_CAPACITY, "more than 32 cameras");
-  const int T = pack_threads(h, no);
+  if (nc > kMaxPoses) return err = "window " + std::to_string(w) + " has more than 256 key frames", SBA_E_CAPACITY;
+  if (ncam > kMaxCameras) return err = "more than 32 cameras", SBA_E_CAPACITY;
   std::vector<uint8_t> bad(T, 0);
-  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
     for (int64_t o = a; o < e; ++o)
       if (p.point_ids[o] < 0 || p.point_ids[o] >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 ||
           p.camera_ids[o] >= ncam)
         bad[t] = 1;
   });
-  if (std::find(bad.begin(), bad.end(), 1) != bad.end())
-    return fail(h, SBA_E_INVALID, "observation index out of range in window " + std::to_string(w));
+  if (std::find(bad.begin(), bad.end(), 1) != bad.end()) return err = "observation index out of range" + where, SBA_E_INVALID;
   W = WindowDesc{};
   W.np = np;
   W.npf = np - p.num_fixed_points;
@@ -202,25 +303,18 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
   W.ncf = ncf;
   W.ncam = ncam;
   W.no = no;
-  W.point_off = static_cast<int32_t>(H.pts.size());
-  W.pair_off = static_cast<int32_t>(H.pair_pose.size());
-  W.obs_off = static_cast<int32_t>(H.obs_xy.size());
-  W.pose_off = static_cast<int32_t>(H.poses.size() / 3);
-  W.cam_off = static_cast<int32_t>(H.cams.size() / 3);
   W.max_iterations = p.max_iterations;
   W.robust = p.robustifier_scale;
   W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
-  // order (point, key frame, camera, input index): a stable counting sort by point (per-thread histograms, then
-  // each thread scatters its own input range), then within each point a stable sort by (key frame, camera)
   const int64_t stride = static_cast<int64_t>(np) + 1;
-  std::vector<int32_t>& cnt = H.cnt;
+  std::vector<int32_t>& cnt = S.cnt;
   cnt.assign(static_cast<size_t>(T) * stride, 0);
-  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
     int32_t* c = cnt.data() + t * stride;
     for (int64_t o = a; o < e; ++o) ++c[p.point_ids[o]];
   });
-  std::vector<int32_t>& start = H.start;
+  std::vector<int32_t>& start = S.start;
   start.assign(np + 1, 0);
   {
     int32_t run = 0;
@@ -234,14 +328,14 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
     }
     start[np] = run;
   }
-  std::vector<int32_t>& order = H.order;
+  std::vector<int32_t>& order = S.order;
   order.resize(no);
-  parallel_ranges(T, no, [&](int t, int64_t a, int64_t e) {
+  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
     int32_t* c = cnt.data() + t * stride;
     for (int64_t o = a; o < e; ++o) order[c[p.point_ids[o]]++] = static_cast<int32_t>(o);
   });
   auto kc = [&](int32_t o) { return p.pose_ids[o] * ncam + p.camera_ids[o]; };
-  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+  parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
       int32_t* seg = order.data() + start[i];
       const int m = start[i + 1] - start[i];
@@ -258,32 +352,10 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       }
     }
   });
-  const int obs_base = static_cast<int>(H.obs_xy.size());
-  H.obs_orig.resize(obs_base + no);
-  H.obs_xy.resize(obs_base + no);
-  H.obs_info.resize(obs_base + no);
-  // the gather into sorted order reads the input at random (disjoint output ranges per thread)
-  parallel_ranges(T, no, [&](int, int64_t a, int64_t e) {
-    for (int64_t k = a; k < e; ++k) {
-      const int o = order[k];
-      H.obs_orig[obs_base + k] = o;
-      H.obs_xy[obs_base + k] = make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]);
-      const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
-      float4 inf;
-      inf.x = L[0];
-      inf.y = 0.5f * (L[1] + L[2]);
-      inf.z = L[3];
-      int cam = p.camera_ids[o];
-      std::memcpy(&inf.w, &cam, 4);
-      H.obs_info[obs_base + k] = inf;
-    }
-  });
-  // pairs: one per run of equal key frames in a point's segment; counted per point, then written at the point's
-  // offset (prefix over points)
-  std::vector<int32_t>&npair = H.npair, &nfree = H.nfree;
+  std::vector<int32_t>&npair = S.npair, &nfree = S.nfree;
   npair.resize(np);
   nfree.resize(np);
-  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+  parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
       int c = 0, f = 0, last = -1;
       for (int k = start[i]; k < start[i + 1]; ++k) {
@@ -298,25 +370,49 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       nfree[i] = f;
     }
   });
-  const size_t pair_base = H.pair_pose.size(), point_base = H.point_pair.size();
-  std::vector<int32_t>& first_pair = H.first_pair;
+  // a mapped window's chunks stage at most kMaxFreePoses free key frames, so no point may see more
+  if (W.mapped)
+    for (int i = 0; i < np; ++i)
+      if (nfree[i] > kMaxFreePoses)
+        return err = "window " + std::to_string(w) + ": a point is observed from more than 40 free key frames",
+               SBA_E_CAPACITY;
+  std::vector<int32_t>& first_pair = S.first_pair;
   first_pair.assign(np + 1, 0);
   for (int i = 0; i < np; ++i) first_pair[i + 1] = first_pair[i] + npair[i];
-  reserve_more(H.pair_obs, static_cast<size_t>(first_pair[np]));
-  reserve_more(H.pair_pose, static_cast<size_t>(first_pair[np]));
-  H.pair_obs.resize(pair_base + first_pair[np]);
-  H.pair_pose.resize(pair_base + first_pair[np]);
-  reserve_more(H.point_pair, np + 1);
-  reserve_more(H.point_nfree, np);
-  reserve_more(H.pts, np);
-  H.point_pair.resize(point_base + np);
-  H.point_nfree.resize(point_base + np);
-  H.pts.resize(point_base + np);
-  parallel_ranges(T, np, [&](int, int64_t a, int64_t e) {
+  S.npairs = first_pair[np];
+  return SBA_OK;
+}
+
+// Packing phase 3 of window w: writes its streams at the offsets that phase 2 put in W (observations in sorted
+// order, pairs, points, key frames, cameras) and builds its Schur and back-substitution tiles (returned in `tiles`
+// and `btiles`, with batch-wide indexes) and pair records. Only the window's own ranges of the streams are touched,
+// so windows can be written concurrently.
+void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const WindowSort& S, const WindowDesc& W,
+                 HostPack& H, std::vector<Tile>& tiles, std::vector<Tile>& btiles) {
+  const int np = W.np, nc = W.nc, no = W.no, ncam = W.ncam, ncf = W.ncf;
+  const int obs_base = W.obs_off, pair_base = W.pair_off, point_base = W.point_off;
+  const std::vector<int32_t>&order = S.order, &start = S.start, &first_pair = S.first_pair;
+  // the gather into sorted order reads the input at random (disjoint output ranges per thread)
+  parallel_ranges(h, T, no, [&](int, int64_t a, int64_t e) {
+    for (int64_t k = a; k < e; ++k) {
+      const int o = order[k];
+      H.obs_orig[obs_base + k] = o;
+      H.obs_xy[obs_base + k] = make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]);
+      const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
+      float4 inf;
+      inf.x = L[0];
+      inf.y = 0.5f * (L[1] + L[2]);
+      inf.z = L[3];
+      int cam = p.camera_ids[o];
+      std::memcpy(&inf.w, &cam, 4);
+      H.obs_info[obs_base + k] = inf;
+    }
+  });
+  parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
-      size_t q = pair_base + first_pair[i];
-      H.point_pair[point_base + i] = static_cast<int32_t>(q);
-      H.point_nfree[point_base + i] = nfree[i];
+      int q = pair_base + first_pair[i];
+      H.point_pair[point_base + i] = q;
+      H.point_nfree[point_base + i] = S.nfree[i];
       H.pts[point_base + i] = make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f);
       int last = -1;
       for (int k = start[i]; k < start[i + 1]; ++k) {
@@ -330,24 +426,14 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       }
     }
   });
-  auto push_pose = [](std::vector<float4>& dst, const float* m) {
-    for (int r = 0; r < 3; ++r) dst.push_back(make_float4(m[r], m[4 + r], m[8 + r], m[12 + r]));
+  auto put_pose = [](float4* dst, const float* m) {
+    for (int r = 0; r < 3; ++r) dst[r] = make_float4(m[r], m[4 + r], m[8 + r], m[12 + r]);
   };
-  for (int j = 0; j < nc; ++j) push_pose(H.poses, p.rig_from_world + 16 * j);
-  for (int c = 0; c < ncam; ++c) push_pose(H.cams, p.camera_from_rig + 16 * c);
+  for (int j = 0; j < nc; ++j) put_pose(H.poses.data() + 3 * (W.pose_off + j), p.rig_from_world + 16 * j);
+  for (int c = 0; c < ncam; ++c) put_pose(H.cams.data() + 3 * (W.cam_off + c), p.camera_from_rig + 16 * c);
   // tiles: consecutive points with at most kTilePairs pairs and max_points points; in a mapped window also at
   // most kMaxFreePoses distinct free key frames (one tile is staged by one chunk)
-  auto pairs_of = [&](int i) {
-    return static_cast<int>((i + 1 < np ? H.point_pair[W.point_off + i + 1] : static_cast<int>(H.pair_pose.size())) -
-                            H.point_pair[W.point_off + i]);
-  };
   std::vector<uint8_t> in_tile(W.mapped ? ncf : 0);
-  for (int i = 0; i < np && W.mapped; ++i) {
-    int nfree = 0;
-    for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pairs_of(i); ++q) nfree += H.pair_pose[q] < ncf;
-    if (nfree > kMaxFreePoses)
-      return fail(h, SBA_E_CAPACITY, "window " + std::to_string(w) + ": a point is observed from more than 40 free key frames");
-  }
   auto make_tiles = [&](std::vector<Tile>& out, int max_points) {
     int i = 0;
     while (i < np) {
@@ -355,16 +441,17 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       int pairs = 0, nkf = 0;
       std::fill(in_tile.begin(), in_tile.end(), 0);
       while (i < np && i - begin < max_points) {
-        const int pi = pairs_of(i);
+        const int pi = S.npair[i];
         if (pairs + pi > kTilePairs && i > begin) break;
         if (W.mapped && max_points == kTilePoints) {
+          const int q0 = pair_base + first_pair[i];
           int add = 0;
-          for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pi; ++q) {
+          for (int q = q0; q < q0 + pi; ++q) {
             const int j = H.pair_pose[q];
             if (j < ncf && !in_tile[j]) ++add;
           }
           if (nkf + add > kMaxFreePoses && i > begin) break;
-          for (int q = H.point_pair[W.point_off + i]; q < H.point_pair[W.point_off + i] + pi; ++q) {
+          for (int q = q0; q < q0 + pi; ++q) {
             const int j = H.pair_pose[q];
             if (j < ncf && !in_tile[j]) in_tile[j] = 1;
           }
@@ -373,27 +460,29 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
         pairs += pi;
         ++i;
       }
-      const int pb = H.point_pair[W.point_off + begin];
-      out.push_back(Tile{w, W.point_off + begin, W.point_off + i, pb, pb + pairs, {0, 0, 0}});
+      const int pb = pair_base + first_pair[begin];
+      out.push_back(Tile{w, point_base + begin, point_base + i, pb, pb + pairs, {0, 0, 0}});
     }
   };
-  const size_t first_tile = H.tiles.size(), first_btile = H.btiles.size();
-  make_tiles(H.tiles, kTilePoints);
-  make_tiles(H.btiles, kBlock);
+  tiles.clear();
+  btiles.clear();
+  make_tiles(tiles, kTilePoints);
+  make_tiles(btiles, kBlock);
   // pair records: observation range, key frame | point index inside its back-substitution tile << 8, point
-  // index inside its Schur tile | local key frame << 8; first observation by pair
-  H.pair_rec.resize(H.pair_pose.size());
-  const int npoint_all = static_cast<int>(H.point_pair.size());
-  const int npair_all = static_cast<int>(H.pair_pose.size()), nobs_all = static_cast<int>(H.obs_xy.size());
-  parallel_ranges(T, static_cast<int64_t>(H.tiles.size() - first_tile), [&](int, int64_t a, int64_t e) {
+  // index inside its Schur tile | local key frame << 8 (ranges end at the window's own ends)
+  const int pair_end = pair_base + S.npairs, obs_end = obs_base + no;
+  auto pair_range = [&](int i) {  // batch-wide pair range of batch-wide point i of this window
+    const int qa = H.point_pair[i];
+    return std::make_pair(qa, i + 1 < point_base + np ? H.point_pair[i + 1] : pair_end);
+  };
+  parallel_ranges(h, T, static_cast<int64_t>(tiles.size()), [&](int, int64_t a, int64_t e) {
     for (int64_t t = a; t < e; ++t) {
-      const Tile& Tl = H.tiles[first_tile + t];
+      const Tile& Tl = tiles[t];
       for (int i = Tl.point_begin; i < Tl.point_end; ++i) {
-        const int qa = H.point_pair[i];
-        const int qe = i + 1 < npoint_all ? H.point_pair[i + 1] : npair_all;
+        const auto [qa, qe] = pair_range(i);
         for (int q = qa; q < qe; ++q) {
           const int ob = H.pair_obs[q];
-          const int oe = q + 1 < npair_all ? H.pair_obs[q + 1] : nobs_all;
+          const int oe = q + 1 < pair_end ? H.pair_obs[q + 1] : obs_end;
           // local free key frame index: the key frame itself unless the window is mapped (set with its chunks)
           const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
           H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - Tl.point_begin) | (jl << 8));
@@ -401,15 +490,75 @@ int pack_window(sba_handle* h, const sba_problem_view& p, int w, HostPack& H, Wi
       }
     }
   });
-  parallel_ranges(T, static_cast<int64_t>(H.btiles.size() - first_btile), [&](int, int64_t a, int64_t e) {
+  parallel_ranges(h, T, static_cast<int64_t>(btiles.size()), [&](int, int64_t a, int64_t e) {
     for (int64_t t = a; t < e; ++t) {
-      const Tile& Tl = H.btiles[first_btile + t];
+      const Tile& Tl = btiles[t];
       for (int i = Tl.point_begin; i < Tl.point_end; ++i) {
-        const int qe = i + 1 < npoint_all ? H.point_pair[i + 1] : npair_all;
-        for (int q = H.point_pair[i]; q < qe; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - Tl.point_begin) << 8);
+        const auto [qa, qe] = pair_range(i);
+        for (int q = qa; q < qe; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - Tl.point_begin) << 8);
       }
     }
   });
+}
+
+// Packs the n windows into H (phases 1-3 above, phase 2 the batch-wide offsets). A single window uses the threads
+// inside its phases; a batch spreads whole windows over the threads (each window on one thread, taken in turn).
+// The packed streams do not depend on the thread counts.
+int pack_windows(sba_handle* h, const sba_problem_view* problems, int n, HostPack& H, std::vector<WindowDesc>& win) {
+  std::vector<WindowSort>& sorts = H.sorts;
+  std::vector<std::vector<Tile>>&tiles = H.wtiles, &btiles = H.wbtiles;
+  if (static_cast<int>(sorts.size()) < n) sorts.resize(n);
+  if (static_cast<int>(tiles.size()) < n) tiles.resize(n);
+  if (static_cast<int>(btiles.size()) < n) btiles.resize(n);
+  std::vector<int> rc(n, SBA_OK);
+  std::vector<std::string> err(n);
+  int64_t total_obs = 0;
+  for (int w = 0; w < n; ++w) total_obs += std::max(problems[w].num_observations, 0);
+  const int inner = n == 1 ? pack_threads(h, total_obs) : 1;
+  const int outer = n == 1 ? 1 : std::min(n, pack_threads(h, total_obs));
+  std::atomic<int> next{0};
+  auto over_windows = [&](auto&& body) {
+    next = 0;
+    h->pool.run(outer, [&](int) {
+      for (int w = next++; w < n; w = next++) body(w);
+    });
+  };
+  over_windows([&](int w) { rc[w] = sort_window(h, problems[w], w, inner, sorts[w], win[w], err[w]); });
+  for (int w = 0; w < n; ++w)
+    if (rc[w] != SBA_OK) return fail(h, rc[w], err[w]);
+  int64_t obs = 0, pairs = 0, points = 0, poses = 0, cams = 0;
+  for (int w = 0; w < n; ++w) {
+    WindowDesc& W = win[w];
+    W.obs_off = static_cast<int32_t>(obs);
+    W.pair_off = static_cast<int32_t>(pairs);
+    W.point_off = static_cast<int32_t>(points);
+    W.pose_off = static_cast<int32_t>(poses);
+    W.cam_off = static_cast<int32_t>(cams);
+    obs += W.no;
+    pairs += sorts[w].npairs;
+    points += W.np;
+    poses += W.nc;
+    cams += W.ncam;
+  }
+  if (obs > INT32_MAX || pairs > INT32_MAX) return fail(h, SBA_E_CAPACITY, "more than 2^31 observations in a batch");
+  H.obs_orig.resize(obs);
+  H.obs_xy.resize(obs);
+  H.obs_info.resize(obs);
+  H.pair_obs.resize(pairs + 1);
+  H.pair_pose.resize(pairs);
+  H.pair_rec.resize(pairs);
+  H.point_pair.resize(points + 1);
+  H.point_nfree.resize(points);
+  H.pts.resize(points);
+  H.poses.resize(3 * poses);
+  H.cams.resize(3 * cams);
+  H.pair_obs[pairs] = static_cast<int32_t>(obs);  // end sentinels
+  H.point_pair[points] = static_cast<int32_t>(pairs);
+  over_windows([&](int w) { fill_window(h, problems[w], w, inner, sorts[w], win[w], H, tiles[w], btiles[w]); });
+  for (int w = 0; w < n; ++w) {
+    H.tiles.insert(H.tiles.end(), tiles[w].begin(), tiles[w].end());
+    H.btiles.insert(H.btiles.end(), btiles[w].begin(), btiles[w].end());
+  }
   return SBA_OK;
 }
 
@@ -462,8 +611,8 @@ struct Layout {
     items.push_back(Item{total, bytes, host, host_bytes, std::move(bind)});
     total = (total + std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
   }
-  template <class P, class T>
-  void upload(P& field, const std::vector<T>& v) {
+  template <class P, class T, class A>
+  void upload(P& field, const std::vector<T, A>& v) {
     add(std::max<size_t>(v.size(), 1) * sizeof(T), v.empty() ? nullptr : v.data(), v.size() * sizeof(T),
         [&field](char* p) { field = reinterpret_cast<P>(p); });
   }
@@ -637,8 +786,8 @@ namespace {
 
 // Packs and plans the windows on the host, takes the batch's device block from `memory`, uploads, and (graph)
 // records the whole solve as one hipGraph. On failure nothing is left allocated by the batch itself.
-int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph, const MemorySource& memory,
-                sba_batch** out) {
+int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph,
+                     const MemorySource& memory, sba_batch** out) {
   *out = nullptr;
   TraceRange tr("sba pack and plan");
   HostPack& H = h->pack;
@@ -652,9 +801,9 @@ int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool
   int64_t sred_off = 0, a_off = 0, dc_off = 0, big_off = 0, ue_off = 0;
   int max_nc = 1, max_ncam = 1, max_ncf = 1, max_rec = 0;
   int rc = SBA_OK;
+  if ((rc = pack_windows(h, problems, n, H, b->win)) != SBA_OK) return rc;
   for (int w = 0; w < n; ++w) {
     const sba_problem_view& p = problems[w];
-    if ((rc = pack_window(h, p, w, H, b->win[w])) != SBA_OK) return rc;
     WindowDesc& W = b->win[w];
     if (W.np == 0 || W.nc == 0 || W.no == 0) b->status_pre[w] = SBA_E_EMPTY;
     W.nt = W.mapped ? 0 : (6 * W.ncf + 15) / 16;
@@ -685,8 +834,6 @@ int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool
   }
   b->total_obs = static_cast<int64_t>(H.obs_xy.size());
   b->total_pairs = static_cast<int64_t>(H.pair_pose.size());
-  H.point_pair.push_back(static_cast<int32_t>(H.pair_pose.size()));
-  H.pair_obs.push_back(static_cast<int32_t>(H.obs_xy.size()));
 
   // chunks: consecutive tiles of one window, about chunk_pairs pairs of work each; each owns one record
   int64_t target = h->opt.chunk_pairs;
@@ -887,6 +1034,18 @@ int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool
   }
   *out = owned.release();
   return SBA_OK;
+}
+
+// Packs, uploads and (with `graph`) captures a batch; host allocation failures (the packing buffers are
+// page-locked) come back as a status, not as an exception through the C ABI.
+int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph, const MemorySource& memory,
+                sba_batch** out) {
+  try {
+    return build_batch_impl(h, problems, n, graph, memory, out);
+  } catch (const std::bad_alloc&) {
+    *out = nullptr;
+    return fail(h, SBA_E_INVALID, "out of host memory while packing");
+  }
 }
 
 // One window on the handle's stream, its device block taken from `memory`: packed, uploaded, solved with eager

@@ -68,6 +68,14 @@ hipError_t hipFree(void* p) {
   ++g_frees;
   return hipSuccess;
 }
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) {
+  *p = std::malloc(bytes);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void* p) {
+  std::free(p);
+  return hipSuccess;
+}
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k) {
   if (k == hipMemcpyHostToDevice) CHECK(inside(d, n), "H2D copy outside an allocation");
   if (k == hipMemcpyDeviceToHost) CHECK(inside(s, n), "D2H copy outside an allocation");
