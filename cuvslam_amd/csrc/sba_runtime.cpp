@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <condition_variable>
@@ -94,7 +95,9 @@ using pinned_vector = std::vector<T, PinnedAllocator<T>>;
 
 // Host threads kept by a handle: run(n, fn) calls fn(0) .. fn(n - 1), fn(0) on the calling thread, and returns when
 // all have returned. Threads are started on first need and end with the handle, so a solve does not pay for
-// thread creation. Not reentrant: fn must not call run().
+// thread creation. Between jobs a thread polls for the next one for up to kSpin before it sleeps: the packing
+// issues its phases back to back, and waking sleeping threads for each would cost more than the phases. Not
+// reentrant: fn must not call run().
 class HostPool {
  public:
   HostPool() = default;
@@ -103,7 +106,7 @@ class HostPool {
   ~HostPool() {
     {
       std::lock_guard<std::mutex> g(m_);
-      stop_ = true;
+      stop_.store(true);
     }
     wake_.notify_all();
     for (auto& t : threads_) t.join();
@@ -113,46 +116,61 @@ class HostPool {
       fn(0);
       return;
     }
-    std::unique_lock<std::mutex> lk(m_);
-    while (static_cast<int>(threads_.size()) < n - 1) {
-      const int id = static_cast<int>(threads_.size()) + 1;
-      threads_.emplace_back([this, id] { worker(id); });
+    {
+      std::lock_guard<std::mutex> g(m_);
+      while (static_cast<int>(threads_.size()) < n - 1) {
+        const int id = static_cast<int>(threads_.size()) + 1;
+        const uint64_t seen = gen_.load();
+        threads_.emplace_back([this, id, seen] { worker(id, seen); });
+      }
+      job_ = &fn;
+      job_n_ = n;
+      pending_.store(static_cast<int>(threads_.size()));
+      gen_.fetch_add(1, std::memory_order_release);
     }
-    job_ = &fn;
-    job_n_ = n;
-    pending_ = static_cast<int>(threads_.size());
-    ++gen_;
-    lk.unlock();
     wake_.notify_all();
     fn(0);
-    lk.lock();
-    done_.wait(lk, [this] { return pending_ == 0; });
-    job_ = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (pending_.load(std::memory_order_acquire) != 0) {
+      if (std::chrono::steady_clock::now() - t0 > kSpin) {
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [this] { return pending_.load(std::memory_order_acquire) == 0; });
+        break;
+      }
+      std::this_thread::yield();
+    }
   }
 
  private:
-  void worker(int id) {
-    uint64_t seen = 0;
-    std::unique_lock<std::mutex> lk(m_);
+  static constexpr std::chrono::microseconds kSpin{2000};
+  void worker(int id, uint64_t seen) {
     for (;;) {
-      wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
-      if (stop_) return;
-      seen = gen_;
-      const std::function<void(int)>* job = job_;
-      const int n = job_n_;
-      lk.unlock();
-      if (id < n) (*job)(id);
-      lk.lock();
-      if (--pending_ == 0) done_.notify_one();
+      const auto t0 = std::chrono::steady_clock::now();
+      while (gen_.load(std::memory_order_acquire) == seen && !stop_.load()) {
+        if (std::chrono::steady_clock::now() - t0 > kSpin) {
+          std::unique_lock<std::mutex> lk(m_);
+          wake_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
+          break;
+        }
+        std::this_thread::yield();
+      }
+      if (stop_.load()) return;
+      seen = gen_.load(std::memory_order_acquire);
+      if (id < job_n_) (*job_)(id);
+      if (pending_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> g(m_);
+        done_.notify_one();
+      }
     }
   }
   std::mutex m_;
   std::condition_variable wake_, done_;
   std::vector<std::thread> threads_;
   const std::function<void(int)>* job_ = nullptr;
-  int job_n_ = 0, pending_ = 0;
-  uint64_t gen_ = 0;
-  bool stop_ = false;
+  int job_n_ = 0;
+  std::atomic<int> pending_{0};
+  std::atomic<uint64_t> gen_{0};
+  std::atomic<bool> stop_{false};
 };
 
 // One window's sort order and pair counts, from the first packing phase to the third.
@@ -1727,7 +1745,7 @@ int sba_autotune_table(sba_handle* h, const char* path, const sba_problem_view* 
     }
     return v * scale;
   };
-  char key[256];
+  char key[512];
   std::snprintf(key, sizeof(key), "%s loss%d windows%d obs%lld maxfree%lld", prop.gcnArchName, h->opt.loss, n,
                 static_cast<long long>(two_digits(no)), static_cast<long long>(maxf));
   for (char* c = key; *c; ++c)

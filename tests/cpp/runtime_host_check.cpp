@@ -513,6 +513,36 @@ int main() {
     }
   }
 
+  g_case = "a batch packed window by window on 1 and on 8 host threads gives the same streams";
+  std::printf("%s\n", g_case.c_str());
+  {
+    std::vector<Scene> scs(7);
+    std::vector<sba_problem_view> vs;
+    for (int k = 0; k < 6; ++k) {
+      make_scene(scs[k], 5 + 6 * k, 300 + 700 * k, 60 + k);
+      vs.push_back(scs[k].v);
+    }
+    make_loop(scs[6], 48, 3000);  // a mapped window among them
+    vs.push_back(scs[6].v);
+    std::vector<std::vector<char>> packed[2];
+    for (int threads : {1, 8}) {
+      sba_options o1 = o;
+      o1.pack_threads = threads;
+      sba_handle* h1 = nullptr;
+      CHECK(sba_create(&o1, &h1) == SBA_OK, "sba_create");
+      for (int rep = 0; rep < 2; ++rep) {  // the second batch reuses the handle's packing buffers
+        sba_batch* b = nullptr;
+        CHECK(sba_batch_create(h1, vs.data(), static_cast<int>(vs.size()), &b) == SBA_OK, sba_last_error(h1));
+        if (rep == 1) packed[threads == 8] = g_packed;
+        sba_batch_destroy(h1, b);
+      }
+      sba_destroy(h1);
+    }
+    CHECK(packed[0].size() == packed[1].size() && !packed[0].empty(), "stream count");
+    for (size_t k = 0; k < packed[0].size() && k < packed[1].size(); ++k)
+      CHECK(packed[0][k] == packed[1][k], (std::string("stream ") + std::to_string(k) + " differs").c_str());
+  }
+
   g_case = "debug step";
   std::printf("%s\n", g_case.c_str());
   {

@@ -31,10 +31,12 @@ def test_service_cases_under_sanitizers(target):
     assert "OK (0 failures)" in out.stdout
 
 
-def test_host_runtime_binds_every_stream_inside_its_block():
+@pytest.mark.parametrize("target", ["runtime_host_check", "runtime_host_check_tsan"])
+def test_host_runtime_binds_every_stream_inside_its_block(target):
     # the host runtime driven without a GPU (tests/cpp/runtime_host_check.cpp): every stream each kernel launch
-    # receives is bound, inside a live allocation, and apart from the others; sba_solve_noalloc allocates nothing
-    exe = build("runtime_host_check")
+    # receives is bound, inside a live allocation, and apart from the others; sba_solve_noalloc allocates nothing;
+    # packing on several host threads gives the same streams (under the address and the thread sanitizers)
+    exe = build(target)
     out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().splitlines()[-1].startswith("OK"), out.stdout
