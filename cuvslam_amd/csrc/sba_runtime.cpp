@@ -306,14 +306,6 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
   const int ncf = nc - p.num_fixed_key_frames;
   if (nc > kMaxPoses) return err = "window " + std::to_string(w) + " has more than 256 key frames", SBA_E_CAPACITY;
   if (ncam > kMaxCameras) return err = "more than 32 cameras", SBA_E_CAPACITY;
-  std::vector<uint8_t> bad(T, 0);
-  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
-    for (int64_t o = a; o < e; ++o)
-      if (p.point_ids[o] < 0 || p.point_ids[o] >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 ||
-          p.camera_ids[o] >= ncam)
-        bad[t] = 1;
-  });
-  if (std::find(bad.begin(), bad.end(), 1) != bad.end()) return err = "observation index out of range" + where, SBA_E_INVALID;
   W = WindowDesc{};
   W.np = np;
   W.npf = np - p.num_fixed_points;
@@ -325,27 +317,51 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
   W.robust = p.robustifier_scale;
   W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
+  // per-thread histograms by point, with the index checks of the same pass
   const int64_t stride = static_cast<int64_t>(np) + 1;
   std::vector<int32_t>& cnt = S.cnt;
   cnt.assign(static_cast<size_t>(T) * stride, 0);
+  std::vector<uint8_t> bad(T, 0);
   parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
     int32_t* c = cnt.data() + t * stride;
-    for (int64_t o = a; o < e; ++o) ++c[p.point_ids[o]];
+    for (int64_t o = a; o < e; ++o) {
+      const int32_t i = p.point_ids[o];
+      if (i < 0 || i >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 || p.camera_ids[o] >= ncam)
+        bad[t] = 1;
+      else
+        ++c[i];
+    }
   });
+  if (std::find(bad.begin(), bad.end(), 1) != bad.end()) return err = "observation index out of range" + where, SBA_E_INVALID;
+  // each point's first slot (a scan over the points of their totals), then thread t's first slot for point i
   std::vector<int32_t>& start = S.start;
-  start.assign(np + 1, 0);
+  start.resize(np + 1);
+  parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
+    for (int64_t i = a; i < e; ++i) {
+      int32_t s = 0;
+      for (int t = 0; t < T; ++t) s += cnt[t * stride + i];
+      start[i] = s;
+    }
+  });
   {
     int32_t run = 0;
     for (int i = 0; i < np; ++i) {
+      const int32_t c = start[i];
       start[i] = run;
+      run += c;
+    }
+    start[np] = run;
+  }
+  parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
+    for (int64_t i = a; i < e; ++i) {
+      int32_t run = start[i];
       for (int t = 0; t < T; ++t) {
         const int32_t c = cnt[t * stride + i];
         cnt[t * stride + i] = run;  // thread t's next slot for point i
         run += c;
       }
     }
-    start[np] = run;
-  }
+  });
   std::vector<int32_t>& order = S.order;
   order.resize(no);
   parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
@@ -541,7 +557,10 @@ int pack_windows(sba_handle* h, const sba_problem_view* problems, int n, HostPac
       for (int w = next++; w < n; w = next++) body(w);
     });
   };
-  over_windows([&](int w) { rc[w] = sort_window(h, problems[w], w, inner, sorts[w], win[w], err[w]); });
+  {
+    TraceRange tr("sba sort");
+    over_windows([&](int w) { rc[w] = sort_window(h, problems[w], w, inner, sorts[w], win[w], err[w]); });
+  }
   for (int w = 0; w < n; ++w)
     if (rc[w] != SBA_OK) return fail(h, rc[w], err[w]);
   int64_t obs = 0, pairs = 0, points = 0, poses = 0, cams = 0;
@@ -572,7 +591,10 @@ int pack_windows(sba_handle* h, const sba_problem_view* problems, int n, HostPac
   H.cams.resize(3 * cams);
   H.pair_obs[pairs] = static_cast<int32_t>(obs);  // end sentinels
   H.point_pair[points] = static_cast<int32_t>(pairs);
-  over_windows([&](int w) { fill_window(h, problems[w], w, inner, sorts[w], win[w], H, tiles[w], btiles[w]); });
+  {
+    TraceRange tr("sba fill");
+    over_windows([&](int w) { fill_window(h, problems[w], w, inner, sorts[w], win[w], H, tiles[w], btiles[w]); });
+  }
   for (int w = 0; w < n; ++w) {
     H.tiles.insert(H.tiles.end(), tiles[w].begin(), tiles[w].end());
     H.btiles.insert(H.btiles.end(), btiles[w].begin(), btiles[w].end());

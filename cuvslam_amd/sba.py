@@ -65,8 +65,11 @@ class BundleAdjustmentProblem:
         """A C view whose buffers stay alive as long as this problem (column-major copies of the matrices)."""
         pts = np.ascontiguousarray(self.points, dtype=np.float32).reshape(-1, 3)
         xy = np.ascontiguousarray(self.observation_xys, dtype=np.float32).reshape(-1, 2)
-        info = np.ascontiguousarray(np.asarray(self.observation_infos, dtype=np.float32).reshape(-1, 2, 2)
-                                    .transpose(0, 2, 1))
+        # column-major 2x2 blocks; symmetric blocks (the usual information matrices) read the same either way, so
+        # only asymmetric ones are copied with their off-diagonal entries swapped
+        info = np.ascontiguousarray(self.observation_infos, dtype=np.float32).reshape(-1, 4)
+        if not np.array_equal(info[:, 1], info[:, 2]):
+            info = np.ascontiguousarray(info.reshape(-1, 2, 2).transpose(0, 2, 1)).reshape(-1, 4)
         pid = np.ascontiguousarray(self.point_ids, dtype=np.int32)
         qid = np.ascontiguousarray(self.pose_ids, dtype=np.int32)
         cid = np.ascontiguousarray(self.camera_ids, dtype=np.int8)

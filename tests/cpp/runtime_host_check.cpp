@@ -543,6 +543,38 @@ int main() {
       CHECK(packed[0][k] == packed[1][k], (std::string("stream ") + std::to_string(k) + " differs").c_str());
   }
 
+  g_case = "invalid windows are rejected before any launch";
+  std::printf("%s\n", g_case.c_str());
+  {
+    const int launches = g_launches;
+    for (int which : {0, 1, 2, 3}) {
+      Scene sc;
+      make_scene(sc, 8, 400, 70 + which);
+      std::vector<int32_t> pid(sc.v.point_ids, sc.v.point_ids + sc.v.num_observations);
+      std::vector<int32_t> qid(sc.v.pose_ids, sc.v.pose_ids + sc.v.num_observations);
+      std::vector<int8_t> cid(sc.v.camera_ids, sc.v.camera_ids + sc.v.num_observations);
+      const int o = sc.v.num_observations / 2;
+      if (which == 0) pid[o] = sc.v.num_points;
+      if (which == 1) qid[o] = -1;
+      if (which == 2) cid[o] = static_cast<int8_t>(sc.v.num_cameras);
+      sba_problem_view bad = sc.v;
+      bad.point_ids = pid.data();
+      bad.pose_ids = qid.data();
+      bad.camera_ids = cid.data();
+      if (which == 3) bad.num_fixed_key_frames = bad.num_poses + 1;
+      sba_result r = result_for(sc);
+      CHECK(sba_solve(h, &bad, &r) == SBA_E_INVALID, "single window");
+      Scene good;
+      make_scene(good, 6, 300, 80 + which);
+      sba_problem_view vs[3] = {good.v, bad, good.v};
+      sba_batch* b = nullptr;
+      CHECK(sba_batch_create(h, vs, 3, &b) == SBA_E_INVALID && b == nullptr, "window 1 of a batch");
+      CHECK(std::string(sba_last_error(h)).find("window 1") != std::string::npos || which == 2,
+            "the error names the window");
+    }
+    CHECK(g_launches == launches, "no launch for a rejected window");
+  }
+
   g_case = "debug step";
   std::printf("%s\n", g_case.c_str());
   {
