@@ -175,7 +175,9 @@ class HostPool {
 
 // One window's sort order and pair counts, from the first packing phase to the third.
 struct WindowSort {
-  std::vector<int32_t> cnt, start, order, npair, nfree, first_pair;
+  std::vector<int32_t> cnt, cur, start, npair, nfree, first_pair;
+  std::vector<int64_t> ko;    // sorted (key << 32 | input index) of the observations
+  std::vector<int64_t> priv;  // the threads' ranges, each sorted by point
   int npairs = 0;
 };
 
@@ -191,6 +193,8 @@ struct HostPack {
   pinned_vector<float4> pair_info;
   pinned_vector<float4> pts, poses, cams;
   pinned_vector<Tile> tiles, btiles;
+  pinned_vector<Chunk> chunks, bchunks;
+  pinned_vector<int32_t> chunk_map, red_first;
   // per-window scratch of the packing (the sort order and pair counts, the tiles before they are joined)
   std::vector<WindowSort> sorts;
   std::vector<std::vector<Tile>> wtiles, wbtiles;
@@ -206,6 +210,10 @@ struct HostPack {
     cams.clear();
     tiles.clear();
     btiles.clear();
+    chunks.clear();
+    bchunks.clear();
+    chunk_map.clear();
+    red_first.clear();
   }
 };
 
@@ -295,6 +303,8 @@ int pack_threads(const sba_handle* h, int64_t work) {
 // pairs (one per run of equal key frames in a point's segment) are counted per point. T threads share the work;
 // every output element has one writer, so the result does not depend on T. On failure `err` says why.
 int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSort& S, WindowDesc& W, std::string& err) {
+  std::unique_ptr<TraceRange> phase;
+#define PHASE(name) phase.reset(), phase.reset(new TraceRange("sba sort: " name))
   const int np = p.num_points, nc = p.num_poses, no = p.num_observations, ncam = p.num_cameras;
   const std::string where = " in window " + std::to_string(w);
   if (np < 0 || nc < 0 || no < 0 || ncam < 0) return err = "negative size" + where, SBA_E_INVALID;
@@ -317,14 +327,23 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
   W.robust = p.robustifier_scale;
   W.mapped = ncf > kMaxFreePoses ? 1 : 0;
 
-  // per-thread histograms by point, with the index checks of the same pass
+  PHASE("histogram");
+  // The sort by point: each thread counts, then sorts its own contiguous range of the input into its own range of
+  // a private buffer (a counting sort whose writes stay in the thread's range: threads never write the same cache
+  // lines), and the point segments are then gathered from the threads' ranges in thread order, i.e. in input order.
+  // Each word carries the observation's sort key next to its index: key (key frame << 8 | camera) above, input
+  // index below, so that the sort within a point is a sort of the words (stable by key) and the later passes read
+  // the keys in order instead of the input at random.
+  const int TT = (T <= 1 || no < 2 * T) ? 1 : T;  // the ranges parallel_ranges makes
+  auto range_begin = [&](int t) { return static_cast<int64_t>(no) * t / TT; };
   const int64_t stride = static_cast<int64_t>(np) + 1;
-  std::vector<int32_t>& cnt = S.cnt;
-  cnt.assign(static_cast<size_t>(T) * stride, 0);
-  std::vector<uint8_t> bad(T, 0);
-  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
+  std::vector<int32_t>& cnt = S.cnt;  // row t: thread t's counts by point, then its first slot of each point
+  cnt.assign(static_cast<size_t>(TT) * stride, 0);
+  std::vector<uint8_t> bad(TT, 0);
+  auto each_thread = [&](const std::function<void(int)>& fn) { h->pool.run(TT, fn); };
+  each_thread([&](int t) {
     int32_t* c = cnt.data() + t * stride;
-    for (int64_t o = a; o < e; ++o) {
+    for (int64_t o = range_begin(t); o < range_begin(t + 1); ++o) {
       const int32_t i = p.point_ids[o];
       if (i < 0 || i >= np || p.pose_ids[o] < 0 || p.pose_ids[o] >= nc || p.camera_ids[o] < 0 || p.camera_ids[o] >= ncam)
         bad[t] = 1;
@@ -333,13 +352,26 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
     }
   });
   if (std::find(bad.begin(), bad.end(), 1) != bad.end()) return err = "observation index out of range" + where, SBA_E_INVALID;
-  // each point's first slot (a scan over the points of their totals), then thread t's first slot for point i
+  PHASE("slots");
+  std::vector<int32_t>& cur = S.cur;
+  cur.resize(static_cast<size_t>(TT) * stride);
+  each_thread([&](int t) {  // thread t's first slot of each point in its own range, and a cursor copy
+    int32_t* c = cnt.data() + t * stride;
+    int32_t run = 0;
+    for (int i = 0; i < np; ++i) {
+      const int32_t k = c[i];
+      c[i] = run;
+      run += k;
+    }
+    c[np] = run;
+    std::copy(c, c + stride, cur.data() + t * stride);
+  });
   std::vector<int32_t>& start = S.start;
   start.resize(np + 1);
   parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
       int32_t s = 0;
-      for (int t = 0; t < T; ++t) s += cnt[t * stride + i];
+      for (int t = 0; t < TT; ++t) s += cnt[t * stride + i + 1] - cnt[t * stride + i];
       start[i] = s;
     }
   });
@@ -352,40 +384,43 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
     }
     start[np] = run;
   }
+  PHASE("scatter");
+  std::vector<int64_t>&ko = S.ko, &priv = S.priv;
+  ko.resize(no);
+  priv.resize(no);
+  each_thread([&](int t) {
+    int32_t* c = cur.data() + t * stride;
+    int64_t* out = priv.data() + range_begin(t);
+    for (int64_t o = range_begin(t); o < range_begin(t + 1); ++o)
+      out[c[p.point_ids[o]]++] = (static_cast<int64_t>((p.pose_ids[o] << 8) | p.camera_ids[o]) << 32) | o;
+  });
   parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
-      int32_t run = start[i];
-      for (int t = 0; t < T; ++t) {
-        const int32_t c = cnt[t * stride + i];
-        cnt[t * stride + i] = run;  // thread t's next slot for point i
-        run += c;
+      int64_t* dst = ko.data() + start[i];
+      for (int t = 0; t < TT; ++t) {
+        const int64_t* src = priv.data() + range_begin(t);
+        for (int32_t k = cnt[t * stride + i]; k < cnt[t * stride + i + 1]; ++k) *dst++ = src[k];
       }
     }
   });
-  std::vector<int32_t>& order = S.order;
-  order.resize(no);
-  parallel_ranges(h, T, no, [&](int t, int64_t a, int64_t e) {
-    int32_t* c = cnt.data() + t * stride;
-    for (int64_t o = a; o < e; ++o) order[c[p.point_ids[o]]++] = static_cast<int32_t>(o);
-  });
-  auto kc = [&](int32_t o) { return p.pose_ids[o] * ncam + p.camera_ids[o]; };
+  PHASE("segments");
   parallel_ranges(h, T, np, [&](int, int64_t a, int64_t e) {
     for (int64_t i = a; i < e; ++i) {
-      int32_t* seg = order.data() + start[i];
+      int64_t* seg = ko.data() + start[i];
       const int m = start[i + 1] - start[i];
-      if (m <= 48) {  // insertion sort: stable, linear on segments already in key-frame order
+      if (m <= 48) {  // insertion sort: linear on segments already in key-frame order
         for (int x = 1; x < m; ++x) {
-          const int32_t v = seg[x];
-          const int kv = kc(v);
+          const int64_t v = seg[x];
           int y = x;
-          for (; y > 0 && kc(seg[y - 1]) > kv; --y) seg[y] = seg[y - 1];
+          for (; y > 0 && seg[y - 1] > v; --y) seg[y] = seg[y - 1];
           seg[y] = v;
         }
       } else {
-        std::stable_sort(seg, seg + m, [&](int32_t u, int32_t v) { return kc(u) < kc(v); });
+        std::sort(seg, seg + m);
       }
     }
   });
+  PHASE("pairs");
   std::vector<int32_t>&npair = S.npair, &nfree = S.nfree;
   npair.resize(np);
   nfree.resize(np);
@@ -393,7 +428,7 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
     for (int64_t i = a; i < e; ++i) {
       int c = 0, f = 0, last = -1;
       for (int k = start[i]; k < start[i + 1]; ++k) {
-        const int pose = p.pose_ids[order[k]];
+        const int pose = static_cast<int>(ko[k] >> 40);
         if (pose != last) {
           ++c;
           f += pose < ncf;
@@ -425,11 +460,12 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
                  HostPack& H, std::vector<Tile>& tiles, std::vector<Tile>& btiles) {
   const int np = W.np, nc = W.nc, no = W.no, ncam = W.ncam, ncf = W.ncf;
   const int obs_base = W.obs_off, pair_base = W.pair_off, point_base = W.point_off;
-  const std::vector<int32_t>&order = S.order, &start = S.start, &first_pair = S.first_pair;
+  const std::vector<int32_t>&start = S.start, &first_pair = S.first_pair;
+  const std::vector<int64_t>& ko = S.ko;
   // the gather into sorted order reads the input at random (disjoint output ranges per thread)
   parallel_ranges(h, T, no, [&](int, int64_t a, int64_t e) {
     for (int64_t k = a; k < e; ++k) {
-      const int o = order[k];
+      const int o = static_cast<int32_t>(ko[k]);
       H.obs_orig[obs_base + k] = o;
       H.obs_xy[obs_base + k] = make_float2(p.observation_xys[2 * o], p.observation_xys[2 * o + 1]);
       const float* L = p.observation_infos + 4 * o;  // column-major (L00, L10, L01, L11)
@@ -437,7 +473,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       inf.x = L[0];
       inf.y = 0.5f * (L[1] + L[2]);
       inf.z = L[3];
-      int cam = p.camera_ids[o];
+      int cam = static_cast<int>((ko[k] >> 32) & 255);
       std::memcpy(&inf.w, &cam, 4);
       H.obs_info[obs_base + k] = inf;
     }
@@ -450,7 +486,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       H.pts[point_base + i] = make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f);
       int last = -1;
       for (int k = start[i]; k < start[i + 1]; ++k) {
-        const int pose = p.pose_ids[order[k]];
+        const int pose = static_cast<int>(ko[k] >> 40);
         if (pose != last) {
           H.pair_obs[q] = obs_base + k;
           H.pair_pose[q] = pose;
@@ -666,15 +702,16 @@ struct Layout {
 // Where a batch's device block comes from: its own allocation, the handle's workspace, or the scratch array.
 using MemorySource = std::function<int(sba_batch* b, size_t bytes, char** base)>;
 
-// Binds the layout to `base`, uploads the host streams and clears the zeroed ones on the handle's stream, then
-// waits for the stream (the host streams may be released after this returns).
-int place(sba_handle* h, const Layout& L, char* base) {
+// Binds the layout to `base`, uploads the host streams and clears the zeroed ones on the handle's stream; with
+// `wait`, waits for the stream. Without it the copies may still read the handle's packing buffers when this
+// returns: the next packing waits for the stream before it writes them.
+int place(sba_handle* h, const Layout& L, char* base, bool wait) {
   TraceRange tr("sba upload");
   for (const auto& it : L.items) it.bind(base + it.off);
   for (const auto& it : L.items)
     if (it.host_bytes) HIP_TRY(hipMemcpyAsync(base + it.off, it.host, it.host_bytes, hipMemcpyHostToDevice, h->stream));
   if (L.total > L.zero_begin) HIP_TRY(launch_fill_zero(base + L.zero_begin, L.total - L.zero_begin, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (wait) HIP_TRY(hipStreamSynchronize(h->stream));
   return SBA_OK;
 }
 
@@ -826,11 +863,12 @@ namespace {
 
 // Packs and plans the windows on the host, takes the batch's device block from `memory`, uploads, and (graph)
 // records the whole solve as one hipGraph. On failure nothing is left allocated by the batch itself.
-int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph,
+int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph, bool wait_upload,
                      const MemorySource& memory, sba_batch** out) {
   *out = nullptr;
   TraceRange tr("sba pack and plan");
   HostPack& H = h->pack;
+  HIP_TRY(hipStreamSynchronize(h->stream));  // no copy of an earlier batch still reads the packing buffers
   H.clear();
   std::unique_ptr<sba_batch, int (*)(sba_batch*)> owned(new (std::nothrow) sba_batch, free_batch);
   sba_batch* b = owned.get();
@@ -881,8 +919,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   const int64_t btarget = h->opt.bchunk_pairs > 0 ? h->opt.bchunk_pairs : target;
   // In a mapped window a chunk also stages at most kMaxFreePoses free key frames: its map lists them in
   // increasing order and its pairs carry their index in that list.
-  std::vector<Chunk> chunks;
-  std::vector<int32_t> chunk_map;
+  pinned_vector<Chunk>& chunks = H.chunks;
+  pinned_vector<int32_t>& chunk_map = H.chunk_map;
   int64_t part_off = 0;
   {
     size_t t = 0;
@@ -941,7 +979,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
   }
   // back-substitution chunks: the same grouping over the larger back-substitution tiles
-  std::vector<Chunk> bchunks;
+  pinned_vector<Chunk>& bchunks = H.bchunks;
   {
     size_t t = 0;
     while (t < H.btiles.size()) {
@@ -960,7 +998,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   }
 
   // record reduction work list: 64 entries per workgroup for windows of many chunks, 256 otherwise
-  std::vector<int32_t> red_first(n + 1, 0);
+  pinned_vector<int32_t>& red_first = H.red_first;
+  red_first.assign(n + 1, 0);
   for (int w = 0; w < n; ++w) {
     const WindowDesc& W = b->win[w];
     const int per = W.nchunks < 64 ? 256 : 64;
@@ -1053,7 +1092,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
     return fail(h, SBA_E_CAPACITY, "Schur partial does not fit in shared memory");
   char* base = nullptr;
   if ((rc = memory(b, L.total, &base)) != SBA_OK) return rc;
-  if ((rc = place(h, L, base)) != SBA_OK) return rc;
+  if ((rc = place(h, L, base, wait_upload)) != SBA_OK) return rc;
   if (!pair_streams) {
     // every pair has exactly one observation: the observation streams are already in pair order
     d.pair_xy = d.obs_xy;
@@ -1079,9 +1118,9 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
 // Packs, uploads and (with `graph`) captures a batch; host allocation failures (the packing buffers are
 // page-locked) come back as a status, not as an exception through the C ABI.
 int build_batch(sba_handle* h, const sba_problem_view* problems, int32_t n, bool graph, const MemorySource& memory,
-                sba_batch** out) {
+                sba_batch** out, bool wait_upload = true) {
   try {
-    return build_batch_impl(h, problems, n, graph, memory, out);
+    return build_batch_impl(h, problems, n, graph, wait_upload, memory, out);
   } catch (const std::bad_alloc&) {
     *out = nullptr;
     return fail(h, SBA_E_INVALID, "out of host memory while packing");
@@ -1100,7 +1139,7 @@ int solve_one(sba_handle* h, const sba_problem_view* problem, sba_result* result
   HIP_TRY(hipSetDevice(h->opt.device));
   TraceRange tr("SchurComplementBundlerGpu::solve");
   sba_batch* b = nullptr;
-  int rc = build_batch(h, problem, 1, false, memory, &b);
+  int rc = build_batch(h, problem, 1, false, memory, &b, false);  // sba_batch_read below waits for the stream
   if (rc != SBA_OK) return rc;
   std::unique_ptr<sba_batch, int (*)(sba_batch*)> owned(b, free_batch);
   if ((rc = enqueue_solve(h, b, false)) != SBA_OK) return rc;
