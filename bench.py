@@ -550,12 +550,19 @@ def run_mono(args):
         if k >= args.warmup:
             t.append(time.perf_counter() - t0)
     med = float(np.median(t))
+    single = []  # one window per call, as the reference's MonoSBASolver::solve
+    for k in range(12):
+        t0 = time.perf_counter()
+        solver.solve_batch([base[k % len(base)]])
+        if k >= 2:
+            single.append(time.perf_counter() - t0)
     line = {
         "metric": "mono SBA windows/s", "value": round(len(ws) / med, 2), "unit": "windows/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * med, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "512 monocular windows per batch (8 frames, 3 constrained, 300 tracks, 0.5 px "
                                "noise), up to 30 LM iterations with the reference's stop tests, from host buffers"},
+        "single_window_ms_median": round(1e3 * float(np.median(single)), 4),
     }
     if not args.no_cpu_baseline:
         from oracle import sba_oracle as O

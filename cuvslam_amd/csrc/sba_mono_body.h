@@ -24,6 +24,7 @@
 namespace sba_mono {
 
 constexpr int kMonoThreads = 256;
+constexpr int kPart = 42;  // partial sums per thread: a frame's U (36) and e (6), or a reduced-system block (36)
 constexpr double kFltEps = 1.1920928955078125e-07;     // FLT_EPSILON, common/types.h:203-206
 constexpr double kFltMax = 3.4028234663852886e+38;     // std::numeric_limits<float>::max()
 constexpr double kHither = 0.1;                        // epipolar/near_plane.h:33-38
@@ -81,12 +82,14 @@ struct MonoWin {
   double* Sb;                  // [n]
   double* Q;                   // [n][n] eigenvectors
   double* E;                   // [n][n] work copy of S
+  double* E2;                  // [n][n] the Jacobi sweeps' second buffer (when E does not fit in LDS)
   double* ev;                  // [n]
   double* x;                   // [n] and four more work vectors
   double* xt;
   double* bd;
   double* dv;
   double* rot;                 // [2 (n + 1)] the Jacobi rotations of one step (p, r, c, s), also psolve's scratch
+  double* part;                // [kMonoThreads * kPart] per-thread partial sums of the sliced loops
   // outputs
   double* prep;                // [36] the last free frame's block of the reduced system (undamped diagonal)
   double* cov;                 // [36]
@@ -271,7 +274,7 @@ __host__ __device__ inline bool inverse3(const double* m, double* out, double th
 // The control state of one window's solve, shared by the workgroup (LDS on the GPU).
 struct MonoShared {
   double residual, old, lambda, red[kMonoThreads];
-  int32_t count, new_values, failed, feasible, iteration, ok, total, n, stop, ired[kMonoThreads];
+  int32_t count, new_values, failed, feasible, iteration, ok, total, n, stop, chol, ired[kMonoThreads];
 };
 
 // ---- the solver: one workgroup per window -------------------------------------------------------------------------
@@ -280,16 +283,29 @@ struct MonoSolve {
   const MonoWin& w;
   MonoShared& sh;
   int tid, nth;
+  double* lds = nullptr;  // workgroup memory for the eigen-decomposition (GPU), lds_cap doubles
+  int64_t lds_cap = 0;
 
-  // workgroup sum of v (fixed order: per-thread partials, then thread 0 sums them in thread order)
+  // workgroup sum of v in a fixed order: on the GPU lane l of the first wavefront adds the partials of threads
+  // l, l + 64, l + 128, l + 192 and the 64 lane sums are added by a fixed butterfly; on the host (one thread) the
+  // partial itself
   __host__ __device__ double sum(double v) {
     sh.red[tid] = v;
     MONO_SYNC();
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(kMonoThreads == 256, "the reduction below covers 256 threads");
+    if (tid < 64) {
+      double s = (sh.red[tid] + sh.red[tid + 64]) + (sh.red[tid + 128] + sh.red[tid + 192]);
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (tid == 0) sh.red[0] = s;
+    }
+#else
     if (tid == 0) {
       double s = 0.0;
       for (int i = 0; i < nth; ++i) s += sh.red[i];
       sh.red[0] = s;
     }
+#endif
     MONO_SYNC();
     const double s = sh.red[0];
     MONO_SYNC();
@@ -307,6 +323,15 @@ struct MonoSolve {
     const int s = sh.ired[0];
     MONO_SYNC();
     return s;
+  }
+
+  // Sliced sums: `items` independent sums (frames, reduced-system blocks), each over a list of terms, are split
+  // into `slices` strided slices per item (slice s takes terms s, s + slices, ...); thread (item, slice) adds its
+  // terms into kPart partials, and the partials of an item are then added in slice order. The result depends on the
+  // workgroup size only through `slices`, which is fixed for a launch.
+  __host__ __device__ int slices_for(int items) const {
+    const int s = items > 0 ? nth / items : 1;
+    return s < 1 ? 1 : s;
   }
 
   // frames' poses from their twists (Exp(a)) and, with `nearest`, the nearest rotations
@@ -462,25 +487,39 @@ struct MonoSolve {
   // U and errA per free frame (its observations in track order), V and errB per track, W per observation
   // (mono_sba_solver.cpp:586-678)
   __host__ __device__ void calc_UVW() {
-    for (int f = tid; f < w.nF; f += nth) {
-      if (w.constrained[f]) continue;
-      double U[36] = {}, e[6] = {};
-      for (int q = w.frame_obs[f]; q < w.frame_obs[f + 1]; ++q) {
-        const int o = w.fobs[q], t = w.otrack[o];
-        if (w.tskip[t]) continue;
-        const double* A = w.oA + 12 * o;
-        const double* L = w.tinfo + 4 * t;
-        const double wo = w.ow[o];
-        double tmp[6][2];
-        for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 2; ++c) tmp[r][c] = A[r] * (wo * L[c]) + A[6 + r] * (wo * L[2 + c]);
-        for (int r = 0; r < 6; ++r) {
-          for (int c = 0; c < 6; ++c) U[6 * r + c] += tmp[r][0] * A[c] + tmp[r][1] * A[6 + c];
-          e[r] += tmp[r][0] * w.oerr[2 * o] + tmp[r][1] * w.oerr[2 * o + 1];
+    // U and errA of every frame, sliced over its observations
+    const int fs = slices_for(w.nF);
+    for (int it = tid; it < w.nF * fs; it += nth) {
+      const int f = it / fs, sl = it % fs;
+      double P[kPart] = {};
+      if (!w.constrained[f]) {
+        for (int q = w.frame_obs[f] + sl; q < w.frame_obs[f + 1]; q += fs) {
+          const int o = w.fobs[q], t = w.otrack[o];
+          if (w.tskip[t]) continue;
+          const double* A = w.oA + 12 * o;
+          const double* L = w.tinfo + 4 * t;
+          const double wo = w.ow[o];
+          double tmp[6][2];
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 2; ++c) tmp[r][c] = A[r] * (wo * L[c]) + A[6 + r] * (wo * L[2 + c]);
+          for (int r = 0; r < 6; ++r) {
+            for (int c = 0; c < 6; ++c) P[6 * r + c] += tmp[r][0] * A[c] + tmp[r][1] * A[6 + c];
+            P[36 + r] += tmp[r][0] * w.oerr[2 * o] + tmp[r][1] * w.oerr[2 * o + 1];
+          }
         }
       }
-      for (int i = 0; i < 36; ++i) w.fUo[36 * f + i] = U[i];
-      for (int i = 0; i < 6; ++i) w.ferrA[6 * f + i] = e[i];
+      for (int i = 0; i < kPart; ++i) w.part[kPart * it + i] = P[i];
+    }
+    MONO_SYNC();
+    for (int it = tid; it < w.nF * kPart; it += nth) {
+      const int f = it / kPart, i = it % kPart;
+      if (w.constrained[f]) continue;
+      double v = 0.0;
+      for (int sl = 0; sl < fs; ++sl) v += w.part[kPart * (f * fs + sl) + i];
+      if (i < 36)
+        w.fUo[36 * f + i] = v;
+      else
+        w.ferrA[6 * f + i - 36] = v;
     }
     for (int t = tid; t < w.nT; t += nth) {
       if (w.tskip[t]) continue;
@@ -551,77 +590,190 @@ struct MonoSolve {
 
   // Parallel cyclic Jacobi (Brent-Luk round-robin ordering: each step rotates n/2 disjoint index pairs at once) of
   // the n x n symmetric E into eigenvalues ev and eigenvectors Q; sweeps until the off-diagonal norm falls to
-  // eps ||E||_F (Golub & Van Loan, section 8.5).
+  // eps ||E||_F (Golub & Van Loan, section 8.5). A step computes its n/2 rotations from its input E, then the
+  // thread of the 2 x 2 block (pair k rows, pair k2 columns) writes J_k^T E J_k2 of its block to the other buffer
+  // and the thread of (row r, pair k) rotates Q's row r in place. E and Q live in workgroup memory when 3 n^2
+  // doubles fit in it, else in the window's work area.
+  __host__ __device__ void rotation(const double* E, int n, int k, int step, int m, int& a, int& b, double& c,
+                                    double& s) const {
+    a = k == 0 ? 0 : 1 + (k - 1 + step) % (m - 1);
+    b = 1 + (m - 2 - k + step) % (m - 1);
+    if (a > b) {
+      const int tmp = a;
+      a = b;
+      b = tmp;
+    }
+    c = 1.0;
+    s = 0.0;
+    if (b < n) {
+      const double apr = E[a * n + b];
+      if (apr != 0.0) {
+        const double tau = (E[b * n + b] - E[a * n + a]) / (2.0 * apr);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = t * c;
+      }
+    }
+  }
   __host__ __device__ void eigen(int n) {
-    double* E = w.E;
-    double* Q = w.Q;
-    for (int i = tid; i < n * n; i += nth) Q[i] = (i / n == i % n) ? 1.0 : 0.0;
+    const int nn = n * n;
+    const bool in_lds = lds != nullptr && 3 * static_cast<int64_t>(nn) <= lds_cap;
+    double* E0 = in_lds ? lds : w.E;
+    double* E1 = in_lds ? lds + nn : w.E2;
+    double* Q = in_lds ? lds + 2 * nn : w.Q;
+    double* rot = in_lds && 3 * static_cast<int64_t>(nn) + 2 * (n + 2) <= lds_cap ? lds + 3 * nn : w.rot;
+    for (int i = tid; i < nn; i += nth) {
+      if (in_lds) E0[i] = w.E[i];
+      Q[i] = (i / n == i % n) ? 1.0 : 0.0;
+    }
+    MONO_SYNC();
     double fr = 0.0;
-    for (int i = tid; i < n * n; i += nth) fr += E[i] * E[i];
+    for (int i = tid; i < nn; i += nth) fr += E0[i] * E0[i];
     const double fro = sum(fr);
     const double tol = 2.220446049250313e-16 * sqrt(fro);
     const int m = n + (n & 1);  // even number of slots; slot m-1 is a dummy when n is odd
+    const int h = m / 2;
     for (int sweep = 0; sweep < 60; ++sweep) {
       double off = 0.0;
-      for (int i = tid; i < n * n; i += nth)
-        if (i % n > i / n) off += E[i] * E[i];
+      for (int i = tid; i < nn; i += nth)
+        if (i % n > i / n) off += E0[i] * E0[i];
       if (sqrt(2.0 * sum(off)) <= tol) break;
       for (int step = 0; step < m - 1; ++step) {
-        // pair k: (p, r) of the round-robin schedule, p < r
-        for (int k = tid; k < m / 2; k += nth) {
-          int a = k == 0 ? 0 : 1 + (k - 1 + step) % (m - 1);
-          int b = 1 + (m - 2 - k + step) % (m - 1);
-          if (a > b) {
-            const int tmp = a;
-            a = b;
-            b = tmp;
+        // the step's rotations, one per pair (slot indexes and cosine, sine), from the step's input E
+        for (int k = tid; k < h; k += nth) {
+          int a, b;
+          double c, s;
+          rotation(E0, n, k, step, m, a, b, c, s);
+          rot[4 * k] = a;
+          rot[4 * k + 1] = b;
+          rot[4 * k + 2] = c;
+          rot[4 * k + 3] = s;
+        }
+        MONO_SYNC();
+        for (int e = tid; e < h * h + n * h; e += nth) {
+          int a, b;
+          double c, s;
+          auto rotk = [&](int k, int& ra, int& rb, double& rc, double& rs) {
+            ra = int(rot[4 * k]);
+            rb = int(rot[4 * k + 1]);
+            rc = rot[4 * k + 2];
+            rs = rot[4 * k + 3];
+          };
+          if (e < h * h) {
+            const int k = e / h, k2 = e % h;
+            int a2, b2;
+            double c2, s2;
+            rotk(k, a, b, c, s);
+            rotk(k2, a2, b2, c2, s2);
+            const bool rb = b < n, cb = b2 < n;  // the dummy slot has no row or column
+            const double xaa = E0[a * n + a2], xab = cb ? E0[a * n + b2] : 0.0;
+            const double xba = rb ? E0[b * n + a2] : 0.0, xbb = rb && cb ? E0[b * n + b2] : 0.0;
+            // columns first (E J_k2), then rows (J_k^T (E J_k2))
+            const double ya_a = c2 * xaa - s2 * xab, ya_b = s2 * xaa + c2 * xab;
+            const double yb_a = c2 * xba - s2 * xbb, yb_b = s2 * xba + c2 * xbb;
+            E1[a * n + a2] = c * ya_a - s * yb_a;
+            if (cb) E1[a * n + b2] = c * ya_b - s * yb_b;
+            if (rb) E1[b * n + a2] = s * ya_a + c * yb_a;
+            if (rb && cb) E1[b * n + b2] = s * ya_b + c * yb_b;
+          } else {
+            const int r = (e - h * h) / h, k = (e - h * h) % h;
+            rotk(k, a, b, c, s);
+            if (b >= n) continue;
+            double* Qr = Q + r * n;
+            const double qa = Qr[a], qb = Qr[b];
+            Qr[a] = c * qa - s * qb;
+            Qr[b] = s * qa + c * qb;
           }
-          double c = 1.0, s = 0.0;
-          if (b < n) {
-            const double apr = E[a * n + b];
-            if (apr != 0.0) {
-              const double tau = (E[b * n + b] - E[a * n + a]) / (2.0 * apr);
-              const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-              c = 1.0 / sqrt(1.0 + t * t);
-              s = t * c;
-            }
-          }
-          w.rot[4 * k] = a;
-          w.rot[4 * k + 1] = b;
-          w.rot[4 * k + 2] = c;
-          w.rot[4 * k + 3] = s;
         }
         MONO_SYNC();
-        // E <- E J (columns), Q <- Q J: one (row, pair) per thread
-        for (int e = tid; e < n * (m / 2); e += nth) {
-          const int row = e / (m / 2), k = e % (m / 2);
-          const int a = int(w.rot[4 * k]), b = int(w.rot[4 * k + 1]);
-          if (b >= n) continue;
-          const double c = w.rot[4 * k + 2], s = w.rot[4 * k + 3];
-          double* Er = E + row * n;
-          const double ea = Er[a], eb = Er[b];
-          Er[a] = c * ea - s * eb;
-          Er[b] = s * ea + c * eb;
-          double* Qr = Q + row * n;
-          const double qa = Qr[a], qb = Qr[b];
-          Qr[a] = c * qa - s * qb;
-          Qr[b] = s * qa + c * qb;
-        }
-        MONO_SYNC();
-        // E <- J^T E (rows)
-        for (int e = tid; e < n * (m / 2); e += nth) {
-          const int col = e / (m / 2), k = e % (m / 2);
-          const int a = int(w.rot[4 * k]), b = int(w.rot[4 * k + 1]);
-          if (b >= n) continue;
-          const double c = w.rot[4 * k + 2], s = w.rot[4 * k + 3];
-          const double ea = E[a * n + col], eb = E[b * n + col];
-          E[a * n + col] = c * ea - s * eb;
-          E[b * n + col] = s * ea + c * eb;
-        }
-        MONO_SYNC();
+        double* t = E0;
+        E0 = E1;
+        E1 = t;
       }
     }
-    for (int i = tid; i < n; i += nth) w.ev[i] = E[i * n + i];
+    for (int i = tid; i < n; i += nth) w.ev[i] = E0[i * n + i];
+    if (in_lds)
+      for (int i = tid; i < nn; i += nth) w.Q[i] = Q[i];
+    MONO_SYNC();
+  }
+
+  // In-place Cholesky A = L L^T (lower triangle) of the n x n row-major A, right-looking by columns; false on a
+  // non-positive pivot.
+  __host__ __device__ bool cholesky(int n, double* A) {
+    if (tid == 0) sh.chol = 1;
+    MONO_SYNC();
+    for (int k = 0; k < n; ++k) {
+      if (tid == 0) {
+        const double d = A[k * n + k];
+        if (!(d > 0.0)) sh.chol = 0;
+        A[k * n + k] = d > 0.0 ? sqrt(d) : 1.0;
+      }
+      MONO_SYNC();
+      if (!sh.chol) return false;
+      const double dk = A[k * n + k];
+      for (int i = k + 1 + tid; i < n; i += nth) A[i * n + k] /= dk;
+      MONO_SYNC();
+      const int m = n - k - 1;  // trailing lower triangle, row-major over (i, j), k < j <= i
+      for (int e = tid; e < m * (m + 1) / 2; e += nth) {
+        int i = static_cast<int>((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > e) --i;
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        const int j = e - i * (i + 1) / 2;
+        A[(k + 1 + i) * n + (k + 1 + j)] -= A[(k + 1 + i) * n + k] * A[(k + 1 + j) * n + k];
+      }
+      MONO_SYNC();
+    }
+    return true;
+  }
+
+  // The reference solves the reduced system through a rank-revealing decomposition with the rank threshold
+  // n FLT_EPSILON max |eigenvalue|. Full rank is certified without it when S - tau I has a Cholesky factor, with
+  // tau = n FLT_EPSILON times the largest absolute row sum of S (an upper bound of max |eigenvalue|): then every
+  // eigenvalue exceeds tau >= the threshold, the pseudo-inverse is the inverse, and S is factored by Cholesky.
+  // Returns the factor of S (in workgroup memory when it fits), or null when the certificate fails (the caller then
+  // takes the eigen-decomposition).
+  __host__ __device__ double* certified_factor(int n) {
+    const int nn = n * n;
+    double* A = lds != nullptr && static_cast<int64_t>(nn) <= lds_cap ? lds : w.E2;
+    double rmax = 0.0;
+    for (int i = tid; i < n; i += nth) {
+      double r = 0.0;
+      for (int j = 0; j < n; ++j) r += fabs(w.S[i * n + j]);
+      rmax = r > rmax ? r : rmax;
+    }
+    sh.red[tid] = rmax;
+    MONO_SYNC();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int i = 0; i < nth; ++i) m = sh.red[i] > m ? sh.red[i] : m;
+      sh.red[0] = m;
+    }
+    MONO_SYNC();
+    const double tau = double(n) * kFltEps * sh.red[0];
+    MONO_SYNC();
+    for (int e = tid; e < nn; e += nth) A[e] = w.S[e] - (e / n == e % n ? tau : 0.0);
+    MONO_SYNC();
+    if (!cholesky(n, A)) return nullptr;
+    for (int e = tid; e < nn; e += nth) A[e] = w.S[e];
+    MONO_SYNC();
+    if (!cholesky(n, A)) return nullptr;
+    return A;
+  }
+
+  // out = (L L^T)^-1 rhs, forward and backward substitution by one thread
+  __host__ __device__ void chol_solve(int n, const double* L, const double* rhs, double* out) {
+    if (tid == 0) {
+      for (int i = 0; i < n; ++i) {
+        double v = rhs[i];
+        for (int j = 0; j < i; ++j) v -= L[i * n + j] * out[j];
+        out[i] = v / L[i * n + i];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double v = out[i];
+        for (int j = i + 1; j < n; ++j) v -= L[j * n + i] * out[j];
+        out[i] = v / L[i * n + i];
+      }
+    }
     MONO_SYNC();
   }
 
@@ -652,18 +804,29 @@ struct MonoSolve {
   __host__ __device__ bool calc_adjustments() {
     const double lam = sh.lambda;
     const int nf = w.nFree, n = 6 * nf;
-    for (int f = tid; f < w.nF; f += nth) {
-      if (w.constrained[f]) continue;
-      double e[6];
-      for (int r = 0; r < 6; ++r) e[r] = w.ferrA[6 * f + r];
-      for (int q = w.frame_obs[f]; q < w.frame_obs[f + 1]; ++q) {
-        const int o = w.fobs[q], t = w.otrack[o];
-        if (w.tskip[t]) continue;
-        const double* Y = w.oY + 18 * o;
-        const double* eb = w.terrB + 3 * t;
-        for (int r = 0; r < 6; ++r) e[r] -= Y[3 * r] * eb[0] + Y[3 * r + 1] * eb[1] + Y[3 * r + 2] * eb[2];
+    // eps = errA - sum over the frame's observations of Y errB, sliced over the observations
+    const int fs = slices_for(w.nF);
+    for (int it = tid; it < w.nF * fs; it += nth) {
+      const int f = it / fs, sl = it % fs;
+      double P[6] = {};
+      if (!w.constrained[f]) {
+        for (int q = w.frame_obs[f] + sl; q < w.frame_obs[f + 1]; q += fs) {
+          const int o = w.fobs[q], t = w.otrack[o];
+          if (w.tskip[t]) continue;
+          const double* Y = w.oY + 18 * o;
+          const double* eb = w.terrB + 3 * t;
+          for (int r = 0; r < 6; ++r) P[r] += Y[3 * r] * eb[0] + Y[3 * r + 1] * eb[1] + Y[3 * r + 2] * eb[2];
+        }
       }
-      for (int r = 0; r < 6; ++r) w.feps[6 * f + r] = e[r];
+      for (int r = 0; r < 6; ++r) w.part[kPart * it + r] = P[r];
+    }
+    MONO_SYNC();
+    for (int it = tid; it < 6 * w.nF; it += nth) {
+      const int f = it / 6, r = it % 6;
+      if (w.constrained[f]) continue;
+      double v = 0.0;
+      for (int sl = 0; sl < fs; ++sl) v += w.part[kPart * (f * fs + sl) + r];
+      w.feps[6 * f + r] = w.ferrA[6 * f + r] - v;
     }
     MONO_SYNC();
     if (n == 0) {
@@ -672,7 +835,27 @@ struct MonoSolve {
       MONO_SYNC();
       return false;
     }
-    // S = blockdiag(Ua) - sum over tracks of Y_j W_k^T for every ordered pair of its free frames; b = eps
+    // S = blockdiag(Ua) - sum over tracks of Y_j W_k^T for every ordered pair of its free frames, each block's sum
+    // sliced over its track pairs; b = eps
+    const int bs = slices_for(nf * nf);
+    for (int it = tid; it < nf * nf * bs; it += nth) {
+      const int bi = it / bs, sl = it % bs;
+      double P[36] = {};
+      for (int q = w.blk[bi] + sl; q < w.blk[bi + 1]; q += bs) {
+        const int oj = w.bpair[q].x, ok = w.bpair[q].y;
+        if (w.tskip[w.otrack[oj]]) continue;
+        double Y[18], W[18];
+        for (int i = 0; i < 18; ++i) {
+          Y[i] = w.oY[18 * oj + i];
+          W[i] = w.oW[18 * ok + i];
+        }
+        for (int rr = 0; rr < 6; ++rr)
+          for (int cc = 0; cc < 6; ++cc)
+            P[6 * rr + cc] += Y[3 * rr] * W[3 * cc] + Y[3 * rr + 1] * W[3 * cc + 1] + Y[3 * rr + 2] * W[3 * cc + 2];
+      }
+      for (int i = 0; i < 36; ++i) w.part[kPart * it + i] = P[i];
+    }
+    MONO_SYNC();
     for (int e = tid; e < n * n; e += nth) {
       const int r = e / n, c = e % n, j = r / 6, k = c / 6, rr = r % 6, cc = c % 6;
       double v = 0.0;
@@ -681,13 +864,9 @@ struct MonoSolve {
           if (w.free_idx[f] == j) v = w.fUa[36 * f + 6 * rr + cc];
       }
       const int bi = j * nf + k;
-      for (int q = w.blk[bi]; q < w.blk[bi + 1]; ++q) {
-        const int oj = w.bpair[q].x, ok = w.bpair[q].y;
-        if (w.tskip[w.otrack[oj]]) continue;
-        const double* Y = w.oY + 18 * oj;
-        const double* W = w.oW + 18 * ok;
-        v -= Y[3 * rr] * W[3 * cc] + Y[3 * rr + 1] * W[3 * cc + 1] + Y[3 * rr + 2] * W[3 * cc + 2];
-      }
+      double p = 0.0;
+      for (int sl = 0; sl < bs; ++sl) p += w.part[kPart * (bi * bs + sl) + 6 * rr + cc];
+      v -= p;
       w.S[e] = v;
       w.E[e] = v;
     }
@@ -703,18 +882,21 @@ struct MonoSolve {
       for (int r = 0; r < 6; ++r) w.prep[r * 7] /= 1.0 + lam;
     }
     MONO_SYNC();
-    eigen(n);
-    // rank: |ev_k| > n FLT_EPSILON max |ev| for every k
-    if (tid == 0) {
-      double smax = 0.0;
-      for (int k = 0; k < n; ++k) smax = fabs(w.ev[k]) > smax ? fabs(w.ev[k]) : smax;
-      const double thr = double(n) * kFltEps * smax;
-      int rank = 0;
-      for (int k = 0; k < n; ++k) rank += fabs(w.ev[k]) > thr;
-      sh.ok = rank == n;
+    double* L = certified_factor(n);
+    if (!L) {
+      eigen(n);
+      // rank: |ev_k| > n FLT_EPSILON max |ev| for every k
+      if (tid == 0) {
+        double smax = 0.0;
+        for (int k = 0; k < n; ++k) smax = fabs(w.ev[k]) > smax ? fabs(w.ev[k]) : smax;
+        const double thr = double(n) * kFltEps * smax;
+        int rank = 0;
+        for (int k = 0; k < n; ++k) rank += fabs(w.ev[k]) > thr;
+        sh.ok = rank == n;
+      }
+      MONO_SYNC();
+      if (!sh.ok) return false;
     }
-    MONO_SYNC();
-    if (!sh.ok) return false;
     // IterativeSolver: up to five steps x += solve(b - S x), kept while the residual norm decreases
     for (int i = tid; i < n; i += nth) {
       w.xt[i] = 0.0;
@@ -728,7 +910,10 @@ struct MonoSolve {
       min_norm = nrm;
       for (int i = tid; i < n; i += nth) w.x[i] = w.xt[i];
       MONO_SYNC();
-      psolve(n, w.bd, w.dv);
+      if (L)
+        chol_solve(n, L, w.bd, w.dv);
+      else
+        psolve(n, w.bd, w.dv);
       for (int i = tid; i < n; i += nth) w.xt[i] += w.dv[i];
       MONO_SYNC();
       for (int i = tid; i < n; i += nth) {

@@ -15,13 +15,14 @@
 #include "sba_mono_pack.h"
 
 namespace sba_mono {
-hipError_t launch_mono_solve(const MonoJob* jobs, int n, hipStream_t s);
+hipError_t launch_mono_solve(const MonoJob* jobs, int n, int lds_doubles, hipStream_t s);
 }
 
 using namespace sba_mono;
 
 namespace {
 
+constexpr int64_t kMonoLds = 18432;  // doubles of workgroup memory (144 KiB): 3 n^2 for n <= 78 (13 free frames)
 constexpr int kMaxMonoFree = 64;  // free frames per window: reduced system of order <= 384
 
 struct Slab {  // host staging of everything a batch uploads, with the device offset of each piece
@@ -33,6 +34,12 @@ struct Slab {  // host staging of everything a batch uploads, with the device of
     return o;
   }
 };
+
+// Items of the sliced sums of a window: up to a workgroup's threads, and every frame or reduced-system block when
+// there are more of those (then one slice each).
+size_t part_items(const MonoPack& k) {
+  return std::max<size_t>({size_t(kMonoThreads), size_t(k.nF), size_t(k.nFree) * size_t(k.nFree)});
+}
 
 }  // namespace
 
@@ -92,7 +99,7 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
     const MonoPack& k = packs[i];
     const size_t F = k.nF, T = k.nT, O = k.nO, m = 6 * size_t(k.nFree);
     const size_t doubles = F * (6 * 5 + 36 * 2 + 12 + 9) + T * (3 * 4 + 9 * 2 + 4) + O * (2 + 1 + 12 + 6 + 18 + 18) +
-                           3 * (m * m + 1) + 6 * (m + 1) + 2 * (m + 2) + 36 + 64;
+                           4 * (m * m + 1) + 6 * (m + 1) + 2 * (m + 2) + 36 + part_items(k) * kPart + 64;
     wbytes[i] = round(8 * doubles + 4 * (T + 1) + 40 * 256);  // every array is rounded up to 256 bytes
     work[i] = total;
     total += wbytes[i];
@@ -147,9 +154,10 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
     w.tinvVa = take(9 * T); w.tinfo = take(4 * T);
     w.oerr = take(2 * O); w.ow = take(O); w.oA = take(12 * O); w.oB = take(6 * O); w.oW = take(18 * O);
     w.oY = take(18 * O);
-    w.S = take(m * m + 1); w.Sb = take(m + 1); w.Q = take(m * m + 1); w.E = take(m * m + 1); w.ev = take(m + 1);
+    w.S = take(m * m + 1); w.Sb = take(m + 1); w.Q = take(m * m + 1); w.E = take(m * m + 1); w.E2 = take(m * m + 1); w.ev = take(m + 1);
     w.x = take(m + 1); w.xt = take(m + 1); w.bd = take(m + 1); w.dv = take(m + 1); w.rot = take(2 * (m + 2));
     w.prep = take(36);
+    w.part = take(part_items(k) * kPart);
     w.tskip = reinterpret_cast<int32_t*>(p);
     p += round(4 * (T + 1));
     if (p > base + work[i] + wbytes[i]) return set_error(h, SBA_E_INVALID, "mono work area overflow");
@@ -162,7 +170,15 @@ int sba_mono_solve_batch(sba_handle* h, const sba_mono_window* windows, int32_t 
   hipError_t e = hipMemcpyAsync(base, in.bytes.data(), in.bytes.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(base + jobs_off, jobs.data(), sizeof(MonoJob) * size_t(n), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = launch_mono_solve(reinterpret_cast<const MonoJob*>(base + jobs_off), n, st);
+  // workgroup memory for the eigen-decomposition of the largest reduced system, up to kMonoLds
+  int64_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    const int64_t m = 6 * packs[i].nFree;
+    lds = std::max<int64_t>(lds, 3 * m * m + 2 * (m + 2));  // E, its second buffer, Q, a step's rotations
+  }
+  if (lds > kMonoLds) lds = 0;  // the largest does not fit: every window keeps its matrices in its work area
+  if (e == hipSuccess)
+    e = launch_mono_solve(reinterpret_cast<const MonoJob*>(base + jobs_off), n, static_cast<int>(lds), st);
   if (e != hipSuccess) return set_error(h, SBA_E_DEVICE, std::string("mono solve: ") + hipGetErrorString(e));
   std::vector<char> host_res(res_bytes);
   if (e == hipSuccess) e = hipMemcpyAsync(host_res.data(), base + res_base, res_bytes, hipMemcpyDeviceToHost, st);

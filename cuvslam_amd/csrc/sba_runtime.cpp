@@ -1622,7 +1622,10 @@ int adj_prepare(sba_handle* h, sba_batch* b, AdjWork& A) {
   return rc;
 }
 
-// x = H^-1 gd by preconditioned conjugate gradients (enqueued; the iteration count lands in sc[3]).
+// x = H^-1 gd by preconditioned conjugate gradients; the iteration count lands in sc[3]. The iterations are
+// enqueued in blocks of kCgBlock and the host reads the device's convergence flag after each block, so a solve
+// that converges early does not enqueue the remaining iterations (which would only idle on the flag).
+constexpr int kCgBlock = 8;
 hipError_t adj_pcg(const Batch& d, const AdjWork& A, int max_cg, double tol, hipStream_t s) {
   auto vec = [&](int op) {
     return launch_adj_vec(d, A.x, A.r, A.z, A.p, A.y, A.ec, A.zc, A.acc, A.gd, A.sc, A.done, tol, A.nx, A.n6, op, s);
@@ -1639,11 +1642,16 @@ hipError_t adj_pcg(const Batch& d, const AdjWork& A, int max_cg, double tol, hip
   if (e == hipSuccess) e = vec(0);
   if (e == hipSuccess) e = precond();
   if (e == hipSuccess) e = vec(2);
-  for (int it = 0; it < max_cg && e == hipSuccess; ++it) {
+  int32_t done = 0;
+  for (int it = 0; it < max_cg && e == hipSuccess && !done; ++it) {
     e = launch_adj_obs(d, 0, A.spts, A.sposes, A.p, A.y, nullptr, nullptr, nullptr, nullptr, nullptr, A.done, s);
     if (e == hipSuccess) e = vec(3);
     if (e == hipSuccess) e = precond();
     if (e == hipSuccess) e = vec(4);
+    if (e == hipSuccess && (it + 1) % kCgBlock == 0 && it + 1 < max_cg) {
+      e = hipMemcpyAsync(&done, A.done, sizeof(done), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
   }
   return e;
 }

@@ -5,6 +5,7 @@
 // min_residual_speed) and prints the result as one JSON line.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -39,7 +40,7 @@ int main(int argc, char** argv) {
   std::vector<double> fa(6 * F), faD(6 * F), fda(6 * F), ferrA(6 * F), feps(6 * F), fUo(36 * F), fUa(36 * F),
       fM(12 * F), fRn(9 * F), tb(3 * T), tbD(3 * T), tdb(3 * T), terrB(3 * T), tV(9 * T), tinvVa(9 * T), tinfo(4 * T),
       oerr(2 * O), ow(O), oA(12 * O), oB(6 * O), oW(18 * O), oY(18 * O), S(n * n + 1), Sb(n + 1), Q(n * n + 1),
-      E(n * n + 1), ev(n + 1), x(n + 1), xt(n + 1), bd(n + 1), dv(n + 1), rot(2 * (n + 2)), prep(36), cov(36), sc(8);
+      E(n * n + 1), E2(n * n + 1), ev(n + 1), x(n + 1), xt(n + 1), bd(n + 1), dv(n + 1), rot(2 * (n + 2)), part(kPart * std::max({kMonoThreads, F, k.nFree * k.nFree})), prep(36), cov(36), sc(8);
   std::vector<int32_t> tskip(T);
   MonoWin w{};
   w.nF = F; w.nT = T; w.nO = O; w.nFree = k.nFree; w.max_iterations = mi[0];
@@ -53,8 +54,8 @@ int main(int argc, char** argv) {
   w.tinfo = tinfo.data(); w.oframe = k.oframe.data(); w.otrack = k.otrack.data(); w.oxy = xy.data();
   w.oerr = oerr.data(); w.ow = ow.data(); w.oA = oA.data(); w.oB = oB.data(); w.oW = oW.data(); w.oY = oY.data();
   w.blk = k.blk.data(); w.bpair = reinterpret_cast<const int2*>(k.bpair.data()); w.S = S.data(); w.Sb = Sb.data();
-  w.Q = Q.data(); w.E = E.data(); w.ev = ev.data(); w.x = x.data(); w.xt = xt.data(); w.bd = bd.data();
-  w.dv = dv.data(); w.rot = rot.data(); w.prep = prep.data(); w.cov = cov.data(); w.out_scalars = sc.data();
+  w.Q = Q.data(); w.E = E.data(); w.E2 = E2.data(); w.ev = ev.data(); w.x = x.data(); w.xt = xt.data(); w.bd = bd.data();
+  w.dv = dv.data(); w.rot = rot.data(); w.part = part.data(); w.prep = prep.data(); w.cov = cov.data(); w.out_scalars = sc.data();
   static MonoShared sh;
   MonoSolve s{w, sh, 0, 1};
   s.run();
