@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "sba_device.h"
+#include "sba_tiles.h"
 
 using namespace sba_dev;
 
@@ -290,40 +291,47 @@ namespace {
 __device__ __forceinline__ int64_t pk2(int i, int j) { return static_cast<int64_t>(i) * (i + 1) / 2 + j; }
 }  // namespace
 
-// zc = S^-1 ec with the stored FP64 factor (packed lower L, then 1 / L_ii): one wavefront, forward then
-// backward substitution, each row broadcast from its lane.
-__global__ __launch_bounds__(64) void k_adj_chol(const double* Lst, int n, const double* ec, double* zc, const int32_t* done) {
-  if (*done) return;
-  const int lane = threadIdx.x;
+// L^-1 of the stored FP64 factor (packed lower L, then 1 / L_ii), dense row-major n x n (zero above the diagonal),
+// one wavefront per column j: x = L^-1 e_j by forward substitution, each row's dot product over the lanes with a
+// fixed butterfly. Run once per factored solution; the preconditioner then applies S^-1 = L^-T L^-1 as two
+// parallel products instead of two serial triangular solves.
+__global__ __launch_bounds__(64) void k_adj_linv(const double* Lst, int n, double* Linv) {
+  __shared__ double x[256];
+  const int j = blockIdx.x, lane = threadIdx.x;
   const double* idg = Lst + static_cast<int64_t>(n) * (n + 1) / 2;
-  double y[4];
-  for (int s = 0; s < 4; ++s) y[s] = (lane + 64 * s < n) ? ec[lane + 64 * s] : 0.0;
-  for (int s = 0; s < 4; ++s) {
-    for (int ii = 64 * s; ii < min(n, 64 * s + 64); ++ii) {
-      const double lo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(y[s]), ii - 64 * s),
-                                         __builtin_amdgcn_readlane(__double2loint(y[s]), ii - 64 * s));
-      const double xi = lo * idg[ii];
-      for (int s3 = s; s3 < 4; ++s3) {
-        const int k = lane + 64 * s3;
-        if (k > ii && k < n) y[s3] -= Lst[pk2(k, ii)] * xi;
-        else if (k == ii) y[s3] = xi;
-      }
-    }
+  for (int i = lane; i < n; i += 64) x[i] = 0.0;
+  sba_tiles::wave_lds_sync();
+  for (int i = j; i < n; ++i) {
+    double sum = 0.0;
+    for (int k = j + lane; k < i; k += 64) sum += Lst[pk2(i, k)] * x[k];
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane == 0) x[i] = ((i == j ? 1.0 : 0.0) - sum) * idg[i];
+    sba_tiles::wave_lds_sync();
   }
-  for (int s = 3; s >= 0; --s) {
-    for (int ii = min(n - 1, 64 * s + 63); ii >= 64 * s; --ii) {
-      const double yi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(y[s]), ii - 64 * s),
-                                         __builtin_amdgcn_readlane(__double2loint(y[s]), ii - 64 * s));
-      const double xi = yi * idg[ii];
-      for (int s3 = 0; s3 <= s; ++s3) {
-        const int k = lane + 64 * s3;
-        if (k < ii) y[s3] -= Lst[pk2(ii, k)] * xi;
-        else if (k == ii) y[s3] = xi;
-      }
-    }
+  for (int i = lane; i < n; i += 64) Linv[static_cast<int64_t>(i) * n + j] = i >= j ? x[i] : 0.0;
+}
+
+// zc = S^-1 ec = L^-T (L^-1 ec) with the dense inverse factor: y = L^-1 ec, one wavefront per row (lanes over
+// the row, a fixed butterfly), then zc = L^-T y, one thread per entry.
+__global__ __launch_bounds__(256) void k_adj_chol(const double* Linv, int n, const double* ec, double* zc,
+                                                  const int32_t* done) {
+  __shared__ double y[256];
+  if (*done) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  for (int i = wave; i < n; i += nw) {
+    const double* row = Linv + static_cast<int64_t>(i) * n;
+    double sum = 0.0;
+    for (int k = lane; k <= i; k += 64) sum += row[k] * ec[k];
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane == 0) y[i] = sum;
   }
-  for (int s = 0; s < 4; ++s)
-    if (lane + 64 * s < n) zc[lane + 64 * s] = y[s];
+  __syncthreads();
+  for (int jj = tid; jj < n; jj += blockDim.x) {
+    double sum = 0.0;
+#pragma unroll 4
+    for (int i = jj; i < n; ++i) sum += Linv[static_cast<int64_t>(i) * n + jj] * y[i];
+    zc[jj] = sum;
+  }
 }
 
 // Conjugate-gradient vector algebra in one workgroup (nx = 6 ncf + 3 npf entries).
@@ -343,7 +351,21 @@ __global__ __launch_bounds__(1024) void k_adj_vec(Batch b, Cg c, int nx, int n6,
   const int tid = threadIdx.x;
   if (op != 0 && *c.done) return;
   double s0 = 0.0, s1 = 0.0;
-  for (int k = tid; k < nx; k += 1024) {
+  // the dot products of ops 2-4 read distinct vectors: restrict-qualified, unrolled, so that a thread's loads of
+  // several entries are in flight together
+  const double* __restrict__ vr = c.r;
+  const double* __restrict__ vz = c.z;
+  const double* __restrict__ vp = c.p;
+  const double* __restrict__ vy = c.y;
+  if (op == 3) {
+#pragma unroll 8
+    for (int k = tid; k < nx; k += 1024) s0 += vp[k] * vy[k];
+  } else if (op == 4) {
+#pragma unroll 8
+    for (int k = tid; k < nx; k += 1024) s0 += vr[k] * vz[k];
+  }
+  const int gk = blockIdx.x * blockDim.x + tid, gstride = gridDim.x * blockDim.x;
+  for (int k = gk; k < nx && op <= 2; k += gstride) {
     if (op == 0) {
       c.x[k] = 0.0;
       c.r[k] = c.g[k];
@@ -361,15 +383,11 @@ __global__ __launch_bounds__(1024) void k_adj_vec(Batch b, Cg c, int nx, int n6,
         for (int m = 0; m < 3; ++m) s += Vi[a][m] * (c.r[base + m] - c.acc[base + m]);
         c.z[k] = 0.5 * s;
       }
-    } else if (op == 2) {
+    } else {
       c.p[k] = c.z[k];
       s0 += c.r[k] * c.z[k];
       s1 += c.g[k] * c.g[k];
       c.y[k] = 0.0;
-    } else if (op == 3) {
-      s0 += c.p[k] * c.y[k];
-    } else if (op == 4) {
-      s0 += c.r[k] * c.z[k];
     }
   }
   if (op == 0 || op == 1) return;
@@ -405,13 +423,18 @@ __global__ __launch_bounds__(1024) void k_adj_vec(Batch b, Cg c, int nx, int n6,
     }
     const double alpha = c.sc[0] / t0;
     double rr = 0.0;
+    double* __restrict__ wx = c.x;
+    double* __restrict__ wr = c.r;
+    double* __restrict__ wec = c.ec;
+    double* __restrict__ wacc = c.acc;
+#pragma unroll 4
     for (int k = tid; k < nx; k += 1024) {
-      c.x[k] += alpha * c.p[k];
-      const double rk = c.r[k] - alpha * c.y[k];
-      c.r[k] = rk;
+      wx[k] += alpha * vp[k];
+      const double rk = wr[k] - alpha * vy[k];
+      wr[k] = rk;
       rr += rk * rk;
-      if (k < n6) c.ec[k] = rk;
-      c.acc[k] = 0.0;
+      if (k < n6) wec[k] = rk;
+      wacc[k] = 0.0;
     }
     for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o);
     __syncthreads();
@@ -428,9 +451,12 @@ __global__ __launch_bounds__(1024) void k_adj_vec(Batch b, Cg c, int nx, int n6,
   }
   // op 4
   const double beta = t0 / c.sc[0];
+  double* __restrict__ wp = c.p;
+  double* __restrict__ wy = c.y;
+#pragma unroll 4
   for (int k = tid; k < nx; k += 1024) {
-    c.p[k] = c.z[k] + beta * c.p[k];
-    c.y[k] = 0.0;
+    wp[k] = vz[k] + beta * wp[k];
+    wy[k] = 0.0;
   }
   __syncthreads();
   if (tid == 0) c.sc[0] = t0;
@@ -516,8 +542,16 @@ hipError_t launch_adj_obs(const Batch& b, int mode, const double* spts, const do
   return hipGetLastError();
 }
 
-hipError_t launch_adj_chol(const double* Lst, int n, const double* ec, double* zc, const int32_t* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_chol, dim3(1), dim3(64), 0, s, Lst, n, ec, zc, done);
+hipError_t launch_adj_linv(const double* Lst, int n, double* Linv, hipStream_t s) {
+  if (n > 256) return hipErrorInvalidValue;
+  if (n > 0) hipLaunchKernelGGL(k_adj_linv, dim3(n), dim3(64), 0, s, Lst, n, Linv);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_chol(const double* Linv, int n, const double* ec, double* zc, const int32_t* done,
+                           hipStream_t s) {
+  if (n > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_adj_chol, dim3(1), dim3(256), 0, s, Linv, n, ec, zc, done);
   return hipGetLastError();
 }
 
@@ -525,7 +559,11 @@ hipError_t launch_adj_vec(const Batch& b, double* x, double* r, double* z, doubl
                           double* acc, double* g, double* sc, int32_t* done, double tol, int nx, int n6, int op,
                           hipStream_t s) {
   Cg c{x, r, z, p, y, ec, zc, acc, g, sc, done, tol};
-  hipLaunchKernelGGL(k_adj_vec, dim3(1), dim3(1024), 0, s, b, c, nx, n6, op);
+  // ops 0 and 1 are elementwise (any number of workgroups); ops 2-4 reduce in one workgroup
+  if (op <= 1)
+    hipLaunchKernelGGL(k_adj_vec, dim3((nx + 255) / 256), dim3(256), 0, s, b, c, nx, n6, op);
+  else
+    hipLaunchKernelGGL(k_adj_vec, dim3(1), dim3(1024), 0, s, b, c, nx, n6, op);
   return hipGetLastError();
 }
 

@@ -39,7 +39,8 @@ hipError_t launch_adj_obs(const Batch& b, int mode, const double* spts, const do
                           double* ec, const double* zc, double* dxy, double* dinfo, const int32_t* orig, int32_t* done,
                           hipStream_t s);
 hipError_t launch_adj_state(const Batch& b, double* spts, double* sposes, const double* a, int op, hipStream_t s);
-hipError_t launch_adj_chol(const double* Lst, int n, const double* ec, double* zc, const int32_t* done, hipStream_t s);
+hipError_t launch_adj_linv(const double* Lst, int n, double* Linv, hipStream_t s);
+hipError_t launch_adj_chol(const double* Linv, int n, const double* ec, double* zc, const int32_t* done, hipStream_t s);
 hipError_t launch_adj_vec(const Batch& b, double* x, double* r, double* z, double* p, double* y, double* ec, double* zc,
                           double* acc, double* g, double* sc, int32_t* done, double tol, int nx, int n6, int op,
                           hipStream_t s);
@@ -1543,7 +1544,7 @@ namespace {
 // Workspace of the implicit-differentiation path of window 0 (FP64, one allocation).
 struct AdjWork {
   int n6, nx, no;
-  double *x, *r, *z, *p, *y, *ec, *zc, *acc, *gd, *sc, *Lst, *dxy, *dinfo, *spts, *sposes;
+  double *x, *r, *z, *p, *y, *ec, *zc, *acc, *gd, *sc, *Lst, *Linv, *dxy, *dinfo, *spts, *sposes;
   int32_t* done;
 };
 
@@ -1553,11 +1554,11 @@ int adj_workspace(sba_handle* h, sba_batch* b, AdjWork& A) {
   A.nx = A.n6 + 3 * W.npf;
   A.no = W.no;
   const int64_t npk = static_cast<int64_t>(A.n6) * (A.n6 + 1) / 2;
-  // x r z p y ec zc acc g (nx each), sc (8), done (2), factor (npk + n6), dxy (2 no), dinfo (3 no),
-  // FP64 points (3 np) and key frames (12 nc)
+  // x r z p y ec zc acc g (nx each), sc (8), done (2), factor (npk + n6), its inverse (n6 x n6), dxy (2 no),
+  // dinfo (3 no), FP64 points (3 np) and key frames (12 nc)
   const size_t nx = static_cast<size_t>(A.nx);
-  const size_t nd = 9 * nx + 8 + 2 + static_cast<size_t>(npk + A.n6) + 5 * static_cast<size_t>(A.no) +
-                    3 * static_cast<size_t>(W.np) + 12 * static_cast<size_t>(W.nc);
+  const size_t nd = 9 * nx + 8 + 2 + static_cast<size_t>(npk + A.n6) + static_cast<size_t>(A.n6) * A.n6 +
+                    5 * static_cast<size_t>(A.no) + 3 * static_cast<size_t>(W.np) + 12 * static_cast<size_t>(W.nc);
   if (!b->adj) {
     HIP_TRY(hipMalloc(&b->adj, nd * sizeof(double)));
     b->allocs.push_back(b->adj);
@@ -1574,7 +1575,8 @@ int adj_workspace(sba_handle* h, sba_batch* b, AdjWork& A) {
   A.sc = A.gd + nx;
   A.done = reinterpret_cast<int32_t*>(A.sc + 8);
   A.Lst = A.sc + 10;
-  A.dxy = A.Lst + npk + A.n6;
+  A.Linv = A.Lst + npk + A.n6;
+  A.dxy = A.Linv + static_cast<size_t>(A.n6) * A.n6;
   A.dinfo = A.dxy + 2 * static_cast<size_t>(A.no);
   A.spts = A.dinfo + 3 * static_cast<size_t>(A.no);
   A.sposes = A.spts + 3 * static_cast<size_t>(W.np);
@@ -1607,7 +1609,8 @@ int adj_prepare(sba_handle* h, sba_batch* b, AdjWork& A) {
   }
   WindowState chk{};
   if (launch_adj_state(d, A.spts, A.sposes, nullptr, 0, s) != hipSuccess || launch_schur(d, s) != hipSuccess ||
-      launch_reduce(d, s) != hipSuccess || launch_factor(d, nullptr, s, A.Lst) != hipSuccess) {
+      launch_reduce(d, s) != hipSuccess || launch_factor(d, nullptr, s, A.Lst) != hipSuccess ||
+      launch_adj_linv(A.Lst, A.n6, A.Linv, s) != hipSuccess) {
     rc = fail(h, SBA_E_DEVICE, "adjoint factor launch failed");
   } else if (hipMemcpyAsync(&chk, d.st, sizeof(chk), hipMemcpyDeviceToHost, s) != hipSuccess ||
              hipStreamSynchronize(s) != hipSuccess) {
@@ -1632,7 +1635,7 @@ hipError_t adj_pcg(const Batch& d, const AdjWork& A, int max_cg, double tol, hip
   };
   auto precond = [&]() {
     hipError_t e = launch_adj_obs(d, 1, A.spts, A.sposes, A.r, nullptr, A.ec, nullptr, nullptr, nullptr, nullptr, A.done, s);
-    if (e == hipSuccess) e = launch_adj_chol(A.Lst, A.n6, A.ec, A.zc, A.done, s);
+    if (e == hipSuccess) e = launch_adj_chol(A.Linv, A.n6, A.ec, A.zc, A.done, s);
     if (e == hipSuccess)
       e = launch_adj_obs(d, 2, A.spts, A.sposes, nullptr, A.acc, nullptr, A.zc, nullptr, nullptr, nullptr, A.done, s);
     if (e == hipSuccess) e = vec(1);
@@ -1664,7 +1667,7 @@ hipError_t adj_gn_step(const Batch& d, const AdjWork& A, hipStream_t s) {
     e = launch_adj_vec(d, A.x, A.r, A.z, A.p, A.y, A.ec, A.zc, A.acc, A.gd, A.sc, A.done, 0.0, A.nx, A.n6, 0, s);
   if (e == hipSuccess)
     e = launch_adj_obs(d, 1, A.spts, A.sposes, A.r, nullptr, A.ec, nullptr, nullptr, nullptr, nullptr, A.done, s);
-  if (e == hipSuccess) e = launch_adj_chol(A.Lst, A.n6, A.ec, A.zc, A.done, s);
+  if (e == hipSuccess) e = launch_adj_chol(A.Linv, A.n6, A.ec, A.zc, A.done, s);
   if (e == hipSuccess)
     e = launch_adj_obs(d, 2, A.spts, A.sposes, nullptr, A.acc, nullptr, A.zc, nullptr, nullptr, nullptr, A.done, s);
   if (e == hipSuccess)

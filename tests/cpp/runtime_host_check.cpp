@@ -248,6 +248,7 @@ hipError_t launch_adj_obs(const Batch&, int, const double*, const double*, const
                           double*, double*, const int32_t*, int32_t*, hipStream_t) { return hipSuccess; }
 hipError_t launch_adj_state(const Batch&, double*, double*, const double*, int, hipStream_t) { return hipSuccess; }
 hipError_t launch_adj_chol(const double*, int, const double*, double*, const int32_t*, hipStream_t) { return hipSuccess; }
+hipError_t launch_adj_linv(const double*, int, double*, hipStream_t) { return hipSuccess; }
 hipError_t launch_adj_vec(const Batch&, double*, double*, double*, double*, double*, double*, double*, double*, double*,
                           double*, int32_t*, double, int, int, int, hipStream_t) { return hipSuccess; }
 hipError_t launch_winsorize_select(const Batch&, int, int, int, double*, int32_t*, int32_t*, uint8_t*, int32_t*, hipStream_t) {
