@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="windows of the one-core CPU baseline (c3)")
     ap.add_argument("--no-extra-c2", action="store_true", help="c3: skip the single 30-key-frame window extra")
     ap.add_argument("--no-extra-inertial", action="store_true", help="skip the inertial-SBA timings")
+    ap.add_argument("--no-extra-production", action="store_true", help="skip the production-window timings")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--chunk-pairs", type=int, default=0)
@@ -354,7 +355,7 @@ def main():
 
     if args.workload == "c3" and not args.no_extra_c2:
         extra["single_window_30kf"] = single_window_extra(ca, RF, args)
-    if args.workload in ("c2", "c3"):
+    if args.workload in ("c2", "c3") and not args.no_extra_production:
         extra["production_window_7kf"] = production_window_extra(ca, args)
     if args.workload in ("c2", "c3") and not args.no_extra_inertial:
         extra["inertial_sba"] = inertial_extra(args)
