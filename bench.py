@@ -126,8 +126,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3", choices=["c2", "c3", "c4", "c5", "inertial", "mono"])
-    ap.add_argument("--cpu-warmups", type=int, default=1)
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-warmups", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=10)
     ap.add_argument("--cpu-sample", type=int, default=4, help="windows of the one-core CPU baseline (c3)")
     ap.add_argument("--no-extra-c2", action="store_true", help="c3: skip the single 30-key-frame window extra")
     ap.add_argument("--no-extra-inertial", action="store_true", help="skip the inertial-SBA timings")
