@@ -14,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cuvslam_sba_imu.h"
@@ -771,7 +772,10 @@ int sba_imu_solve(sba_handle* h, const sba_imu_calibration* calib, const sba_imu
   // (imu_sba_gpu_fixed_vel.cpp:286-301); `from_original`: the final check after the loop (:339-356), which
   // re-integrates (and only then sets the new biases) where the biases moved from the integration's.
   auto reintegrate = [&](bool from_original) {
-    for (int q = k.F; q < k.K; ++q) {
+    // key frames are independent: spread over host threads (each writes only its own preintegration)
+    const int nthreads = static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    sba_internal::parallel_for(h, k.K - k.F, nthreads, [&](int qq) {
+      const int q = k.F + qq;
       const double* X = &cur_pose[kPS * q];
       const V3 bg{X[kBg], X[kBg + 1], X[kBg + 2]}, ba{X[kBa], X[kBa + 1], X[kBa + 2]};
       Pre& p = pre[q];
@@ -786,7 +790,7 @@ int sba_imu_solve(sba_handle* h, const sba_imu_calibration* calib, const sba_imu
         p.first = first;
         p.n = n;
       }
-    }
+    });
   };
   while (it < v->max_iterations) {
     ++it;
@@ -814,7 +818,7 @@ int sba_imu_solve(sba_handle* h, const sba_imu_calibration* calib, const sba_imu
       if (e != hipSuccess) return hip_fail(e, "download key frames");
       if (it == v->max_iterations) break;
       reintegrate(false);
-      for (int q = k.F; q < k.K; ++q) hpre[q] = device_record(pre[q]);
+      sba_internal::parallel_for(h, k.K - k.F, 16, [&](int qq) { hpre[k.F + qq] = device_record(pre[k.F + qq]); });
       if (k.n > 0) {
         e = hipMemcpyAsync(dpre + k.F, hpre.data() + k.F, size_t(k.n) * sizeof(Preint), hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return hip_fail(e, "upload preintegration");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 
 struct sba_handle;
@@ -16,5 +17,8 @@ int set_error(sba_handle* h, int code, const std::string& msg);
 // The handle's auxiliary device workspace (the inertial and the mono solves), at least `bytes` long (regrown to
 // twice the need).
 int aux_workspace(sba_handle* h, size_t bytes, char** out);
+// Calls fn(i) for i in [0, n) on up to `threads` of the handle's host threads (i = 0, threads, 2 threads, ... on the
+// first); every call has its own i, so results written by index do not depend on the thread count.
+void parallel_for(sba_handle* h, int n, int threads, const std::function<void(int)>& fn);
 
 }  // namespace sba_internal

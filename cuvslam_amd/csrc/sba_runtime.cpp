@@ -839,6 +839,12 @@ const char* sba_last_error(const sba_handle* h) { return h ? h->err.c_str() : "n
 namespace sba_internal {
 
 hipStream_t stream_of(sba_handle* h) { return h->stream; }
+void parallel_for(sba_handle* h, int n, int threads, const std::function<void(int)>& fn) {
+  const int t = std::max(1, std::min(threads, n));
+  h->pool.run(t, [&](int k) {
+    for (int i = k; i < n; i += t) fn(i);
+  });
+}
 int device_of(const sba_handle* h) { return h->opt.device; }
 int set_error(sba_handle* h, int code, const std::string& msg) { return fail(h, code, msg); }
 
