@@ -32,7 +32,6 @@ using namespace sba_tiles;
 namespace {
 
 constexpr int kLB = 64;        // block of the large factorization
-constexpr int kLds = kLB + 1;  // padded LDS row
 
 struct Large {
   double* A;     // n x n, row-major; the lower triangle holds A, then L
@@ -333,67 +332,88 @@ __global__ __launch_bounds__(256) void k_potrf(Batch b, int w, int k) {
   if (tid < kb) L.rhs[k0 + tid] = ys[tid];
 }
 
+// 64 x 64 products C -= P Q^T on the FP64 matrix cores for the panel and trailing kernels below: P and Q are staged
+// in LDS as transposed 16 x 16 tiles (tile (I, M) of P at sPT + 256 (4 I + M), element [m][i] = P[16 I + i][16 M + m],
+// the operand layout of tile_update), and each of the four wavefronts owns four of the 16 output tiles (I, J)
+// = (wave, 0..3), accumulated over the four tile columns M in register tiles.
+// (256 threads: each loads its 16 entries in one batch, then stores them.)
+__device__ __forceinline__ void stage_transposed(double* sT, const double* src, int64_t ld, int rows, int cols,
+                                                 int tid) {
+  constexpr int kPer = kLB * kLB / 256;
+  double v[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int idx = tid + 256 * it, r = idx / kLB, c = idx % kLB;  // row-major: consecutive threads, columns
+    v[it] = (r < rows && c < cols) ? src[static_cast<int64_t>(r) * ld + c] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int idx = tid + 256 * it, r = idx / kLB, c = idx % kLB;
+    sT[256 * (4 * (r >> 4) + (c >> 4)) + 16 * (c & 15) + (r & 15)] = v[it];
+  }
+}
+
+__device__ __forceinline__ void gemm_nt_update(double4v (&acc)[4], const double* sPT, const double* sQT, int wave,
+                                               int lane) {
+#pragma unroll
+  for (int J = 0; J < 4; ++J)
+#pragma unroll
+    for (int M = 0; M < 4; ++M)
+      acc[J] = tile_update(acc[J], sPT + 256 * (4 * wave + M), sQT + 256 * (4 * J + M), lane);
+}
+
 // Panel below diagonal block k: L_ik = A_ik L_kk^-T = A_ik X^T with the kept inverse X, one block row per
-// workgroup (each thread a 4 x 4 piece); then the block row's forward-substitution update e_i -= L_ik y_k.
+// workgroup, on the matrix cores; then the block row's forward-substitution update e_i -= L_ik y_k.
 __global__ __launch_bounds__(256) void k_trsm(Batch b, int w, int k) {
-  __shared__ double sA[kLB * kLds];
-  __shared__ double sX[kLB * kLds];
+  __shared__ double sAT[16 * 256];
+  __shared__ double sXT[16 * 256];
   __shared__ double sy[kLB];
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
   const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int i0 = kLB * (k + 1 + blockIdx.x), ib = min(kLB, n - i0);
   const double* X = L.Linv + static_cast<int64_t>(k) * kLB * kLB;
-  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {
-    const int r = idx / kLB, c = idx % kLB;
-    sA[r * kLds + c] = (r < ib && c < kb) ? L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] : 0.0;
-    sX[r * kLds + c] = X[idx];
-  }
+  stage_transposed(sAT, L.A + static_cast<int64_t>(i0) * n + k0, n, ib, kb, tid);
+  stage_transposed(sXT, X, kLB, kLB, kLB, tid);
   if (tid < kLB) sy[tid] = tid < kb ? L.rhs[k0 + tid] : 0.0;
   __syncthreads();
-  const int r0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
-  double acc[4][4] = {};
-  for (int m = 0; m < kLB; ++m) {
-    double a[4], x[4];
+  double4v acc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = sA[(r0 + q) * kLds + m];
-      x[q] = sX[(c0 + q) * kLds + m];
-    }
+  for (int J = 0; J < 4; ++J) acc[J] = double4v{0.0, 0.0, 0.0, 0.0};
+  gemm_nt_update(acc, sAT, sXT, wave, lane);  // -(A X^T) for the tiles (wave, J)
+  __syncthreads();  // every wavefront has read sAT before it is overwritten with L_ik (row-major)
+  double* sL = sAT;
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+  for (int J = 0; J < 4; ++J)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * x[q];
-  }
-  __syncthreads();  // sA is read by every thread before it is overwritten with L_ik
-#pragma unroll
-  for (int p = 0; p < 4; ++p)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = r0 + p, c = c0 + q;
-      sA[r * kLds + c] = (r < ib && c < kb) ? acc[p][q] : 0.0;
-      if (r < ib && c < kb) L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] = acc[p][q];
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * wave + 4 * v + (lane >> 4), c = 16 * J + (lane & 15);
+      const double val = (r < ib && c < kb) ? -acc[J][v] : 0.0;
+      sL[r * kLB + c] = val;
+      if (r < ib && c < kb) L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] = val;
     }
   __syncthreads();
   if (tid < ib) {
     double s = 0.0;
-    for (int m = 0; m < kb; ++m) s += sA[tid * kLds + m] * sy[m];
+    for (int m = 0; m < kb; ++m) s += sL[tid * kLB + m] * sy[m];
     L.rhs[i0 + tid] -= s;
   }
 }
 
-// Trailing update after block k: A_ij -= L_ik L_jk^T for k < j <= i, one block pair per workgroup, each thread
-// a 4 x 4 piece of the 64 x 64 block.
+// Trailing update after block k: A_ij -= L_ik L_jk^T for k < j <= i, one block pair per workgroup, on the matrix
+// cores (the lower triangle of a diagonal block).
 __global__ __launch_bounds__(256) void k_syrk(Batch b, int w, int k) {
-  __shared__ double sA[kLB * kLds];
-  __shared__ double sB[kLB * kLds];
+  __shared__ double sAT[16 * 256];
+  __shared__ double sBT[16 * 256];
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done || !St->chol_ok) return;
   const Large L = large_of(b, W);
   const int n = L.n, k0 = kLB * k, kb = min(kLB, n - k0), tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   int ii = 0, t = blockIdx.x;
   while (t > ii) {
     t -= ii + 1;
@@ -402,32 +422,25 @@ __global__ __launch_bounds__(256) void k_syrk(Batch b, int w, int k) {
   const int jj = t;  // 0 <= jj <= ii
   const int i0 = kLB * (k + 1 + ii), j0 = kLB * (k + 1 + jj);
   const int ib = min(kLB, n - i0), jb = min(kLB, n - j0);
-  for (int idx = tid; idx < kLB * kLB; idx += blockDim.x) {
-    const int r = idx / kLB, c = idx % kLB;
-    sA[r * kLds + c] = (r < ib && c < kb) ? L.A[static_cast<int64_t>(i0 + r) * n + k0 + c] : 0.0;
-    sB[r * kLds + c] = (r < jb && c < kb) ? L.A[static_cast<int64_t>(j0 + r) * n + k0 + c] : 0.0;
-  }
-  __syncthreads();
-  const int r0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
-  double acc[4][4] = {};
-  for (int m = 0; m < kb; ++m) {
-    double a[4], bb[4];
+  stage_transposed(sAT, L.A + static_cast<int64_t>(i0) * n + k0, n, ib, kb, tid);
+  stage_transposed(sBT, L.A + static_cast<int64_t>(j0) * n + k0, n, jb, kb, tid);
+  // this wavefront's output tiles (wave, J), element (4 v + lane / 16, lane % 16) in register v
+  double4v acc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = sA[(r0 + q) * kLds + m];
-      bb[q] = sB[(c0 + q) * kLds + m];
+  for (int J = 0; J < 4; ++J)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * wave + 4 * v + (lane >> 4), c = 16 * J + (lane & 15);
+      acc[J][v] = (r < ib && c < jb) ? L.A[static_cast<int64_t>(i0 + r) * n + j0 + c] : 0.0;
     }
+  __syncthreads();
+  gemm_nt_update(acc, sAT, sBT, wave, lane);
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+  for (int J = 0; J < 4; ++J)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * bb[q];
-  }
-#pragma unroll
-  for (int p = 0; p < 4; ++p)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = r0 + p, c = c0 + q;
-      if (r < ib && c < jb && (ii != jj || c <= r)) L.A[static_cast<int64_t>(i0 + r) * n + j0 + c] -= acc[p][q];
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * wave + 4 * v + (lane >> 4), c = 16 * J + (lane & 15);
+      if (r < ib && c < jb && (ii != jj || c <= r)) L.A[static_cast<int64_t>(i0 + r) * n + j0 + c] = acc[J][v];
     }
 }
 

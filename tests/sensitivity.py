@@ -85,3 +85,27 @@ def fp32_method_spread(problem, loss, fixed_iterations=1, reference=None):
     q.rig_from_world = r32["poses"].astype(np.float32)
     c, _ = O.cost(q, loss=loss)
     return abs(c - ref["last_cost"]) / ref["last_cost"], ref, r32
+
+
+def permuted(problem, seed):
+    """A copy whose observations are listed in a random order (the same problem; every sum over observations is
+    taken in another order)."""
+    q = problem.copy()
+    perm = np.random.default_rng(seed).permutation(q.point_ids.shape[0])
+    q.observation_xys = q.observation_xys[perm].copy()
+    q.observation_infos = q.observation_infos[perm].copy()
+    q.point_ids = q.point_ids[perm].copy()
+    q.pose_ids = q.pose_ids[perm].copy()
+    q.camera_ids = q.camera_ids[perm].copy()
+    return q
+
+
+def fp32_order_variant(problem, loss, seed, reference, fixed_iterations=1):
+    """(|F32 - F64| / F64, accepted steps) of the oracle's FP32 restatement on the problem with its observations in
+    another order: how far, and along which damping decisions, FP32 rounding alone moves the method."""
+    q = permuted(problem, seed)
+    r32 = O.solve(q, "f32", loss=loss, fixed_iterations=fixed_iterations, rcs_solver=0, dense_schur=0)
+    q.points = r32["points"].astype(np.float32)
+    q.rig_from_world = r32["poses"].astype(np.float32)
+    c, _ = O.cost(q, loss=loss)
+    return abs(c - reference["last_cost"]) / reference["last_cost"], int(r32["accepted"])

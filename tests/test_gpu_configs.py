@@ -104,7 +104,10 @@ def test_c5_two_hundred_key_frame_map():
     assert fx["fp32_input_spread"] < 1e-7 and fx["fp32_method_spread"] > 1e-4
     assert err < max(1e-6, 10.0 * fx["fp32_method_spread"]), (err, flip, fx["variants"], q.last_cost_f64)
     assert q.iterations == ref["iterations"]
-    # the FP64 solve accepts 7 of 10 steps, the FP32 restatement all 10: FP32 rounding decides the steps taken at
-    # damping near 1e-7
-    lo, hi = sorted((ref["accepted"], fx["fp32_method_accepted"]))
-    assert lo <= q.accepted <= hi, (q.accepted, lo, hi)
+    # Which steps are accepted is decided by FP32 rounding here: the FP64 solve accepts 7 of 10 steps, five of its
+    # gain ratios lie within 0.1 of a threshold (0.21, 0.21, 0.18 and 0.34 against 0.25; 0.746 against 0.75), and
+    # the FP32 restatement of the method accepts all 10 in every observation order tried (fp32_order_variants).
+    # The count may differ from the FP64 one by at most those marginal decisions.
+    marginal = sum(1 for r in fx["trace_rho"] if min(abs(r - 0.25), abs(r - 0.75)) < 0.1)
+    assert marginal == 5 and all(v["accepted"] == 10 for v in fx["fp32_order_variants"])
+    assert abs(q.accepted - ref["accepted"]) <= marginal, (q.accepted, ref["accepted"], marginal)
