@@ -57,11 +57,10 @@ struct WindowDesc {
   int32_t max_iterations;
   float robust;
   // windows with more than kMaxFreePoses free key frames ("mapped"): each chunk stages its own <= kMaxFreePoses
-  // free key frames; chunk records are scattered into a dense FP32 sum (big) and a dense FP64 factor in A
+  // free key frames; the chunk records are gathered, in chunk order, into a dense FP64 system in A
   int32_t mapped;
   int32_t chunk_begin;  // the window's first Schur chunk
-  int64_t big_off;      // floats into big: sum Z Z^T, n x n row-major, upper triangle (n = 6 ncf)
-  int64_t ue_off;       // floats into big_ue: U (21 ncf), e (6 ncf), diag U (6 ncf)
+  int32_t gfirst_off;   // ints into gfirst: ncf * ncf + 1 offsets, one per key-frame block (a, b), row-major
   int32_t bchunk_begin, nbchunks;  // the window's back-substitution chunks (and their CostPart slots)
 };
 
@@ -134,8 +133,11 @@ struct Batch {
   const float4* poses_init;
   const float4* cams;
   const int32_t* chunk_map;  // local -> window free key frame, per chunk of a mapped window
-  float* big;        // mapped windows: dense sums of Z Z^T
-  float* big_ue;     // mapped windows: U, e, diag U
+  // mapped windows: the contributions to key-frame block (a, b), a <= b, of the reduced system are
+  // gcon[gfirst[W.gfirst_off + a ncf + b] .. gfirst[... + 1]), in chunk order, each (chunk, la | lb << 8) with
+  // la, lb the chunk-local indices of a and b
+  const int32_t* gfirst;
+  const int2* gcon;
   float* part;       // per-chunk Schur records
   float* sred;       // per-window reduced records
   double* A;

@@ -3,10 +3,9 @@
 //
 // The Schur kernel stages at most kMaxFreePoses free key frames per chunk; a mapped window's chunks carry the
 // list of the free key frames they stage (chunk_map, increasing). Here:
-//   k_zero_big    clears the window's dense FP32 sum of Z Z^T and its U, e, diag U
-//   k_scatter     adds each chunk record into them at the rows of its key frames (FP32 atomics: the order of
-//                 the additions, and so the last bits of the sum, is not fixed for these windows)
-//   k_lassemble   A = U + lambda diag(U) - sum Z Z^T in FP64 (dense, row-major lower triangle), rhs e
+//   k_gather      A = U + lambda diag(U) - sum Z Z^T in FP64 (dense, row-major lower triangle), rhs e: each
+//                 key-frame block sums its contributions from the chunk records in chunk order (a plan built
+//                 with the chunks), so the system, like the small windows' reduction, has fixed bits
 //   k_potrf / k_trsm / k_syrk   right-looking blocked Cholesky (Golub & Van Loan, Matrix Computations,
 //                 Algorithm 4.2.10) with 64 x 64 blocks: the diagonal block is factored with the 16 x 16 FP64
 //                 tile scheme of the small-window factor (matrix cores for the tile updates) and its triangular
@@ -50,91 +49,58 @@ __device__ __forceinline__ Large large_of(const Batch& b, const WindowDesc& W) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_zero_big(Batch b, int w) {
-  const WindowDesc W = b.win[w];
-  if (b.st[w].done) return;
-  const int64_t n = 6 * W.ncf;
-  float* big = b.big + W.big_off;
-  float* ue = b.big_ue + W.ue_off;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  for (int64_t i = g; i < n * n / 4; i += stride) reinterpret_cast<float4*>(big)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int64_t i = 4 * (n * n / 4) + g; i < n * n; i += stride) big[i] = 0.f;
-  for (int64_t i = g; i < 33 * W.ncf; i += stride) ue[i] = 0.f;
-}
-
-// One workgroup per chunk of window w: record element (local row rl, local column cl), rl <= cl, goes to
-// (6 map[rl / 6] + rl % 6, 6 map[cl / 6] + cl % 6), which is again in the upper triangle (map increasing).
-__global__ __launch_bounds__(256) void k_scatter(Batch b, int w) {
-  __shared__ int smap[kMaxFreePoses];
-  const WindowDesc W = b.win[w];
-  if (b.st[w].done) return;
-  const Chunk ch = b.chunks[W.chunk_begin + blockIdx.x];
-  const int nloc = ch.nloc, nt = ch.nt, n = 6 * W.ncf, ncf = W.ncf;
-  for (int k = threadIdx.x; k < nloc; k += blockDim.x) smap[k] = b.chunk_map[ch.map_off + k];
-  __syncthreads();
-  const float* rec = b.part + ch.part_off;
-  float* big = b.big + W.big_off;
-  float* ue = b.big_ue + W.ue_off;
-  const int ntu = nt * (nt + 1) / 2, nl6 = 6 * nloc;
-  for (int idx = threadIdx.x; idx < 256 * ntu; idx += blockDim.x) {
-    int T = idx >> 8, I = 0;
-    while (T >= nt - I) {
-      T -= nt - I;
-      ++I;
-    }
-    const int J = I + T;
-    const int e = idx & 255, v = e >> 6, lane = e & 63;
-    const int rl = 16 * I + 4 * (lane >> 4) + v, cl = 16 * J + (lane & 15);
-    if (rl >= nl6 || cl >= nl6 || rl > cl) continue;
-    const int R = 6 * smap[rl / 6] + rl % 6, C = 6 * smap[cl / 6] + cl % 6;
-    atomicAdd(big + static_cast<int64_t>(R) * n + C, rec[idx]);
-  }
-  const float* tail = rec + 256 * ntu;
-  for (int idx = threadIdx.x; idx < 33 * nloc; idx += blockDim.x) {
-    const float x = tail[idx];
-    if (idx < 21 * nloc) {
-      atomicAdd(ue + 21 * smap[idx / 21] + idx % 21, x);
-    } else if (idx < 27 * nloc) {
-      const int q = idx - 21 * nloc;
-      atomicAdd(ue + 21 * ncf + 6 * smap[q / 6] + q % 6, x);
-    } else {
-      const int q = idx - 27 * nloc;
-      atomicAdd(ue + 27 * ncf + 6 * smap[q / 6] + q % 6, x);
-    }
-  }
-}
-
 // A = U + lambda diag(U) - sum Z Z^T (lower triangle), rhs = e, du = diag(U); chol_ok set for the factor.
-__global__ __launch_bounds__(256) void k_lassemble(Batch b, int w) {
+// One wavefront per key-frame block (a, b), a = blockIdx.y <= b = 4 blockIdx.x + wavefront: lane 6 r + c sums
+// record element (6 la + r, 6 lb + c) over the block's contributions in their listed (chunk) order, in FP64, and
+// writes A(6 b + c, 6 a + r); in a diagonal block lanes with r <= c add U and lambda diag U, lanes 36..41 sum e
+// and lanes 42..47 diag U. The sums do not depend on the order the hardware runs the chunks or the wavefronts.
+__global__ __launch_bounds__(256) void k_gather(Batch b, int w) {
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done) return;
+  const int ncf = W.ncf, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ka = blockIdx.y, kb = 4 * blockIdx.x + wave;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) St->chol_ok = 1;
+  if (kb >= ncf || kb < ka) return;
   const Large L = large_of(b, W);
-  const int n = L.n, ncf = W.ncf;
-  const float* big = b.big + W.big_off;
-  const float* U = b.big_ue + W.ue_off;
-  const float* E = U + 21 * ncf;
-  const float* D = E + 6 * ncf;
-  const double lam = static_cast<double>(St->lambda);
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < static_cast<int64_t>(n) * n;
-       idx += stride) {
-    const int R = static_cast<int>(idx / n), C = static_cast<int>(idx % n);
-    if (C > R) continue;
-    double v = -static_cast<double>(big[static_cast<int64_t>(C) * n + R]);
-    if (R / 6 == C / 6) {
-      const int kf = R / 6, rr = C - 6 * kf, cc = R - 6 * kf;
-      v += static_cast<double>(U[21 * kf + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)]);
+  const int n = L.n;
+  const bool diag = ka == kb;
+  const int r = lane / 6, c = lane % 6;
+  // what this lane sums: 0 Z Z^T element (with U and diag U on the diagonal), 1 e, 2 diag U, -1 nothing
+  int kind = -1;
+  if (lane < 36 && (!diag || r <= c)) kind = 0;
+  else if (diag && lane >= 36 && lane < 48) kind = lane < 42 ? 1 : 2;
+  if (kind < 0) return;
+  const int q = lane >= 36 ? (lane - 36) % 6 : 0;
+  const int32_t* first = b.gfirst + W.gfirst_off + ka * ncf + kb;
+  double z = 0.0, u = 0.0, d = 0.0;
+  for (int k = first[0], end = first[1]; k < end; ++k) {
+    const int2 g = b.gcon[k];
+    const Chunk ch = b.chunks[g.x];
+    const float* rec = b.part + ch.part_off;
+    const int la = g.y & 255, lb = g.y >> 8, nt = ch.nt, nloc = ch.nloc;
+    const float* tail = rec + 256 * (nt * (nt + 1) / 2);
+    if (kind == 0) {
+      const int rl = 6 * la + r, cl = 6 * lb + c, I = rl >> 4, J = cl >> 4, r16 = rl & 15;
+      const int tile = I * nt - (I * (I - 1)) / 2 + (J - I);
+      z += static_cast<double>(rec[256 * tile + 64 * (r16 & 3) + 16 * (r16 >> 2) + (cl & 15)]);
+      if (diag) {
+        u += static_cast<double>(tail[21 * la + 6 * r - (r * (r - 1)) / 2 + (c - r)]);
+        if (r == c) d += static_cast<double>(tail[27 * nloc + 6 * la + r]);
+      }
+    } else {
+      z += static_cast<double>(tail[(kind == 1 ? 21 : 27) * nloc + 6 * la + q]);
     }
-    if (R == C) v += lam * static_cast<double>(D[R]);
-    L.A[idx] = v;
   }
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    L.rhs[i] = static_cast<double>(E[i]);
-    L.du[i] = static_cast<double>(D[i]);
+  if (kind == 0) {
+    double v = u - z;
+    if (diag && r == c) v += static_cast<double>(St->lambda) * d;
+    L.A[static_cast<int64_t>(6 * kb + c) * n + 6 * ka + r] = v;
+  } else if (kind == 1) {
+    L.rhs[6 * ka + q] = z;
+  } else {
+    L.du[6 * ka + q] = z;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) St->chol_ok = 1;
 }
 
 // Cholesky of diagonal block k (64 x 64; rows and columns past the matrix are identity) by one workgroup of 256
@@ -541,22 +507,11 @@ __global__ __launch_bounds__(256) void k_lepilogue(Batch b, int w) {
 
 namespace sba_dev {
 
-hipError_t launch_large_zero(const Batch& b, int w, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_big, dim3(256), dim3(256), 0, s, b, w);
-  return hipGetLastError();
-}
-
-hipError_t launch_large_scatter(const Batch& b, int w, int nchunks, hipStream_t s) {
-  if (nchunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3(nchunks), dim3(256), 0, s, b, w);
-  return hipGetLastError();
-}
-
-// The factor and solve of window w (n = 6 free key frames): assemble, blocked Cholesky, triangular solves,
+// The factor and solve of window w (n = 6 free key frames): gather, blocked Cholesky, triangular solves,
 // epilogue. 3 nb launches for nb 64-row blocks.
 hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s) {
-  const int nb = (n + kLB - 1) / kLB;
-  hipLaunchKernelGGL(k_lassemble, dim3(512), dim3(256), 0, s, b, w);
+  const int nb = (n + kLB - 1) / kLB, ncf = n / 6;
+  hipLaunchKernelGGL(k_gather, dim3((ncf + 3) / 4, ncf), dim3(256), 0, s, b, w);
   hipError_t e = hipGetLastError();
   for (int k = 0; k < nb && e == hipSuccess; ++k) {
     hipLaunchKernelGGL(k_potrf, dim3(1), dim3(256), 0, s, b, w, k);

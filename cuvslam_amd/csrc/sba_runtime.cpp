@@ -45,8 +45,6 @@ hipError_t launch_adj_vec(const Batch& b, double* x, double* r, double* z, doubl
                           double* acc, double* g, double* sc, int32_t* done, double tol, int nx, int n6, int op,
                           hipStream_t s);
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s);
-hipError_t launch_large_zero(const Batch& b, int w, hipStream_t s);
-hipError_t launch_large_scatter(const Batch& b, int w, int nchunks, hipStream_t s);
 hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s);
 hipError_t launch_winsorize_select(const Batch& b, int w, int np, int kf, double* err, int32_t* pt_of,
                                    int32_t* count, uint8_t* mark, int32_t* out, hipStream_t s);
@@ -195,7 +193,8 @@ struct HostPack {
   pinned_vector<float4> pts, poses, cams;
   pinned_vector<Tile> tiles, btiles;
   pinned_vector<Chunk> chunks, bchunks;
-  pinned_vector<int32_t> chunk_map, red_first;
+  pinned_vector<int32_t> chunk_map, red_first, gfirst;
+  pinned_vector<int2> gcon;
   // per-window scratch of the packing (the sort order and pair counts, the tiles before they are joined)
   std::vector<WindowSort> sorts;
   std::vector<std::vector<Tile>> wtiles, wbtiles;
@@ -215,6 +214,8 @@ struct HostPack {
     bchunks.clear();
     chunk_map.clear();
     red_first.clear();
+    gfirst.clear();
+    gcon.clear();
   }
 };
 
@@ -721,7 +722,7 @@ int check_streams(sba_handle* h, const Batch& d) {
   const void* need[] = {d.win, d.st, d.tiles, d.chunks, d.btiles, d.bchunks, d.red_first, d.obs_xy, d.obs_info,
                         d.pair_obs, d.pair_pose, d.pair_rec, d.pair_xy, d.pair_info, d.point_pair, d.point_nfree,
                         d.pts0, d.pts1, d.pts_init, d.paux, d.poses0, d.poses1, d.poses_init, d.cams, d.chunk_map,
-                        d.big, d.big_ue, d.part, d.sred, d.A, d.dc, d.dc64, d.out_pts, d.out_poses, d.trace, d.bpart};
+                        d.gfirst, d.gcon, d.part, d.sred, d.A, d.dc, d.dc64, d.out_pts, d.out_poses, d.trace, d.bpart};
   for (size_t k = 0; k < sizeof(need) / sizeof(need[0]); ++k)
     if (!need[k]) return fail(h, SBA_E_DEVICE, "internal error: device stream " + std::to_string(k) + " is not bound");
   return SBA_OK;
@@ -743,11 +744,9 @@ int enqueue_solve(sba_handle* h, sba_batch* b, bool timed) {
   HIP_TRY(launch_decide(d, true, s));
   for (int it = 0; it < b->max_iter; ++it) {
     if (timed) HIP_TRY(hipEventRecord(b->ev[0], s));
-    for (int w : b->mapped) HIP_TRY(launch_large_zero(d, w, s));
     HIP_TRY(launch_schur(d, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[1], s));
     HIP_TRY(launch_reduce(d, s));
-    for (int w : b->mapped) HIP_TRY(launch_large_scatter(d, w, b->win[w].nchunks, s));
     if (timed) HIP_TRY(hipEventRecord(b->ev[2], s));
     HIP_TRY(launch_factor(d, nullptr, s));
     for (int w : b->mapped) HIP_TRY(launch_large_factor(d, w, 6 * b->win[w].ncf, s));
@@ -883,7 +882,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   b->n = n;
   b->win.resize(n);
   b->status_pre.assign(n, SBA_OK);
-  int64_t sred_off = 0, a_off = 0, dc_off = 0, big_off = 0, ue_off = 0;
+  int64_t sred_off = 0, a_off = 0, dc_off = 0;
   int max_nc = 1, max_ncam = 1, max_ncf = 1, max_rec = 0;
   int rc = SBA_OK;
   if ((rc = pack_windows(h, problems, n, H, b->win)) != SBA_OK) return rc;
@@ -901,11 +900,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
     const int nn = 6 * W.ncf;
     W.a_off = 0;
     if (W.mapped) {
-      // dense FP32 sum of Z Z^T and U, e, diag U; FP64 A (n x n), rhs, diag U
-      W.big_off = big_off;
-      big_off += static_cast<int64_t>(nn) * nn;
-      W.ue_off = ue_off;
-      ue_off += 33 * static_cast<int64_t>(W.ncf);
+      // FP64 A (n x n), rhs, diag U, the inverses of the diagonal blocks
       W.a_off = a_off;
       a_off += static_cast<int64_t>(nn) * nn + 2 * nn + static_cast<int64_t>((nn + 63) / 64) * 64 * 64;
       b->mapped.push_back(w);
@@ -985,6 +980,37 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
       W.nchunks += 1;
     }
   }
+  // Mapped windows: the contributions to each key-frame block (a, b), a <= b, of the reduced system, listed
+  // chunk by chunk (counting sort over the blocks), so that the gather sums every block in chunk order.
+  {
+    pinned_vector<int32_t>& gfirst = H.gfirst;
+    pinned_vector<int2>& gcon = H.gcon;
+    std::vector<int32_t> cur;
+    for (int w : b->mapped) {
+      WindowDesc& W = b->win[w];
+      const int ncf = W.ncf;
+      const size_t nb = static_cast<size_t>(ncf) * ncf, base = gfirst.size();
+      W.gfirst_off = static_cast<int32_t>(base);
+      gfirst.resize(base + nb + 1, 0);
+      int32_t* f = gfirst.data() + base;
+      const int c0 = W.chunk_begin, c1 = W.chunk_begin + W.nchunks;
+      for (int ci = c0; ci < c1; ++ci) {
+        const int32_t* m = chunk_map.data() + chunks[ci].map_off;
+        for (int la = 0; la < chunks[ci].nloc; ++la)
+          for (int lb = la; lb < chunks[ci].nloc; ++lb) ++f[static_cast<size_t>(m[la]) * ncf + m[lb] + 1];
+      }
+      f[0] = static_cast<int32_t>(gcon.size());
+      for (size_t i = 1; i <= nb; ++i) f[i] += f[i - 1];
+      cur.assign(f, f + nb);
+      gcon.resize(static_cast<size_t>(f[nb]));
+      for (int ci = c0; ci < c1; ++ci) {
+        const int32_t* m = chunk_map.data() + chunks[ci].map_off;
+        for (int la = 0; la < chunks[ci].nloc; ++la)
+          for (int lb = la; lb < chunks[ci].nloc; ++lb)
+            gcon[cur[static_cast<size_t>(m[la]) * ncf + m[lb]]++] = make_int2(ci, la | (lb << 8));
+      }
+    }
+  }
   // back-substitution chunks: the same grouping over the larger back-substitution tiles
   pinned_vector<Chunk>& bchunks = H.bchunks;
   {
@@ -1029,6 +1055,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   L.upload(d.tiles, H.tiles);
   L.upload(d.chunks, chunks);
   L.upload(d.chunk_map, chunk_map);
+  L.upload(d.gfirst, H.gfirst);
+  L.upload(d.gcon, H.gcon);
   L.upload(d.btiles, H.btiles);
   L.upload(d.bchunks, bchunks);
   L.upload(d.red_first, red_first);
@@ -1065,8 +1093,6 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   L.zeroed(d.part, static_cast<size_t>(part_off));
   L.zeroed(d.sred, static_cast<size_t>(sred_off));
   L.zeroed(d.A, static_cast<size_t>(a_off));
-  L.zeroed(d.big, static_cast<size_t>(big_off));
-  L.zeroed(d.big_ue, static_cast<size_t>(ue_off));
   L.zeroed(d.dc, static_cast<size_t>(dc_off));
   L.zeroed(d.dc64, static_cast<size_t>(dc_off));
   L.zeroed(d.out_pts, H.pts.size());

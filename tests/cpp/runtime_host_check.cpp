@@ -141,7 +141,7 @@ static void check_batch(const Batch& b) {
     CHECK(false, "window table not bound");
     return;
   }
-  int64_t nobs = 0, npairs = 0, part = 0, sred = 0, dc = 0, a = 0, big = 0, ue = 0;
+  int64_t nobs = 0, npairs = 0, part = 0, sred = 0, dc = 0, a = 0, gf = 1, gc = 1, cmap = 1;
   for (int w = 0; w < nw; ++w) {
     const WindowDesc& W = b.win[w];
     nobs = std::max<int64_t>(nobs, W.obs_off + W.no);
@@ -150,8 +150,7 @@ static void check_batch(const Batch& b) {
     if (W.mapped) {
       const int64_t n = 6 * W.ncf;
       a = std::max<int64_t>(a, W.a_off + n * n + 2 * n + ((n + 63) / 64) * 64 * 64);
-      big = std::max<int64_t>(big, W.big_off + n * n);
-      ue = std::max<int64_t>(ue, W.ue_off + 33 * W.ncf);
+      gf = std::max<int64_t>(gf, W.gfirst_off + static_cast<int64_t>(W.ncf) * W.ncf + 1);
     }
     CHECK(W.nbchunks >= 0 && W.bchunk_begin >= 0 && W.bchunk_begin + W.nbchunks <= b.n_bchunks, "window's cost slots");
   }
@@ -162,6 +161,33 @@ static void check_batch(const Batch& b) {
     const int64_t rec = 256 * (ch.nt * (ch.nt + 1) / 2) + 33 * ch.nloc;
     part = std::max<int64_t>(part, ch.part_off + rec);
     CHECK(ch.window >= 0 && ch.window < nw, "chunk window");
+  }
+  for (int c = 0; c < b.n_chunks; ++c)
+    if (b.win[b.chunks[c].window].mapped) cmap = std::max<int64_t>(cmap, b.chunks[c].map_off + b.chunks[c].nloc);
+  // the gather plan of each mapped window: block (a, b) lists, in increasing chunk order, exactly the chunks of
+  // the window that stage both a and b, with their local indices
+  for (int w = 0; w < nw; ++w) {
+    const WindowDesc& W = b.win[w];
+    if (!W.mapped) continue;
+    const int ncf = W.ncf;
+    const int32_t* f = b.gfirst + W.gfirst_off;
+    gc = std::max<int64_t>(gc, f[static_cast<int64_t>(ncf) * ncf]);
+    std::vector<int> want(static_cast<size_t>(ncf) * ncf, 0);
+    for (int c = W.chunk_begin; c < W.chunk_begin + W.nchunks; ++c) {
+      const Chunk& ch = b.chunks[c];
+      CHECK(ch.window == w && ch.nloc <= kMaxFreePoses, "mapped chunk");
+      for (int la = 0; la < ch.nloc; ++la)
+        for (int lb = la; lb < ch.nloc; ++lb) {
+          const int ka = b.chunk_map[ch.map_off + la], kb = b.chunk_map[ch.map_off + lb];
+          CHECK(ka < kb || la == lb, "chunk map increasing");
+          const int k = f[ka * ncf + kb] + want[ka * ncf + kb]++;
+          CHECK(k < f[ka * ncf + kb + 1], "gather plan: block holds fewer contributions than chunks stage it");
+          if (k < f[ka * ncf + kb + 1])
+            CHECK(b.gcon[k].x == c && b.gcon[k].y == (la | (lb << 8)), "gather plan entry (chunk order)");
+        }
+    }
+    for (int k = 0; k < ncf * ncf; ++k)
+      CHECK(f[k] + want[k] == f[k + 1], "gather plan: block holds contributions of no chunk");
   }
   for (int c = 0; c < b.n_bchunks; ++c) nbtiles = std::max<int64_t>(nbtiles, b.bchunks[c].tile_end);
   for (int64_t t = 0; t < ntiles; ++t) {
@@ -187,7 +213,7 @@ static void check_batch(const Batch& b) {
       {"part", b.part, 4 * part}, {"sred", b.sred, 4 * sred}, {"dc", b.dc, 4 * dc}, {"dc64", b.dc64, 8 * dc},
       {"out_pts", b.out_pts, 16 * static_cast<size_t>(b.n_points)}, {"out_poses", b.out_poses, 48 * static_cast<size_t>(b.n_poses)},
       {"trace", b.trace, sizeof(TraceEntry) * kTrace * nw}, {"bpart", b.bpart, sizeof(CostPart) * b.n_bchunks},
-      {"A", b.A, 8 * a}, {"big", b.big, 4 * big}, {"big_ue", b.big_ue, 4 * ue}, {"chunk_map", b.chunk_map, 4}};
+      {"A", b.A, 8 * a}, {"gfirst", b.gfirst, 4 * gf}, {"gcon", b.gcon, 8 * gc}, {"chunk_map", b.chunk_map, 4 * cmap}};
   // the pair streams alias the observation streams when every pair has one observation
   if (b.pair_xy == b.obs_xy) {
     CHECK(npairs == nobs, "pair streams alias observation streams of another length");
@@ -235,8 +261,6 @@ hipError_t launch_factor(const Batch& b, double*, hipStream_t, double*) {
 hipError_t launch_backsub(const Batch& b, bool, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
 hipError_t launch_decide(const Batch& b, bool, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
 hipError_t launch_gather(const Batch& b, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
-hipError_t launch_large_zero(const Batch& b, int, hipStream_t) { check_batch(b); return hipSuccess; }
-hipError_t launch_large_scatter(const Batch& b, int, int, hipStream_t) { check_batch(b); return hipSuccess; }
 hipError_t launch_large_factor(const Batch& b, int, int, hipStream_t) { check_batch(b); return hipSuccess; }
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t) {
   CHECK(inside(p, bytes), "fill outside an allocation");

@@ -235,12 +235,17 @@ def test_autotuner_and_chunking_leave_results_unchanged():
 
 def test_large_multicamera_window_matches_oracle():
     # 59 free key frames (reduced system of order 354, above the 40 key frames one chunk stages): the mapped
-    # path (per-chunk key frame maps, dense FP32 scatter, blocked FP64 Cholesky over several workgroups)
+    # path (per-chunk key frame maps, chunk-ordered FP64 gather, blocked FP64 Cholesky over several workgroups)
     p, _, _ = ca.scene.loop(num_keyframes=60, num_points=20000, loop_length_m=12.0, max_iterations=10)
     assert p.rig_from_world.shape[0] - p.num_fixed_key_frames > 40 and p.camera_from_rig.shape[0] == 8
     q = p.copy()
     s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
     assert s.solve(q)
+    # the gather sums every block in chunk order: a second solve gives the same bits
+    q2 = p.copy()
+    assert s.solve(q2)
+    assert np.array_equal(q.rig_from_world, q2.rig_from_world) and np.array_equal(q.points, q2.points)
+    assert q.last_cost_f64 == q2.last_cost_f64
     r = O.solve(p, "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
     assert q.iterations == r["iterations"] == 10
     assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < 1e-6
