@@ -188,7 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_obs(Batch b, Adj A) {
     const int i = lo;
     const int j = b.pair_pose[pair];
     const bool pf = i < pf_end, cf = j < ncf;
-    for (int o = b.pair_obs[pair]; o < b.pair_obs[pair + 1]; ++o) {
+    const int4 rec = b.pair_rec[pair];
+    for (int o = rec.x; o < rec.y; ++o) {
       const float2 xy = b.obs_xy[o];
       const float4 inf = b.obs_info[o];
       const int cam = __float_as_int(inf.w);

@@ -90,7 +90,14 @@ struct TraceEntry {
 };
 
 struct Tile {
-  int32_t window, point_begin, point_end, pair_begin, pair_end, pad[3];
+  int32_t window, point_begin, point_end, pair_begin, pair_end;
+  // A point of a mapped window seen from more than kMaxFreePoses free key frames has no place in the ordinary
+  // Schur tiles: its free key frames are split into groups of at most kMaxFreePoses / 2, and for every two groups
+  // a tile of its own stages their union, over a copy of all the point's pairs (pair_begin .. pair_end) in which
+  // the pairs of the other groups count as fixed. pair_shift = pair_begin - point_pair[point]; virt = 1 | the
+  // chunk keeps the blocks inside the first group << 1 | inside the second << 2 | size of the first group << 8
+  // (0 for ordinary tiles).
+  int32_t pair_shift, virt, pad;
 };
 
 struct Chunk {

@@ -389,8 +389,8 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   Tile Tn = b.tiles[ch.tile_begin];
   int pp_a = 0, pp_e = 0;  // pair range of point vp, loaded one tile ahead
   if (vp >= 0 && vp < Tn.point_end - Tn.point_begin) {
-    pp_a = b.point_pair[Tn.point_begin + vp];
-    pp_e = b.point_pair[Tn.point_begin + vp + 1];
+    pp_a = b.point_pair[Tn.point_begin + vp] + Tn.pair_shift;
+    pp_e = b.point_pair[Tn.point_begin + vp + 1] + Tn.pair_shift;
   }
   PairPre pre{};
   if (tid < Tn.pair_end - Tn.pair_begin) {
@@ -422,7 +422,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       const int i = T.point_begin + tp;
       j = pre.rec.z & 255;
       pair_pf = i < pf_end;
-      pair_cf = j < ncf;
+      pair_cf = j < ncf && jl != 255;  // a free key frame staged by this chunk
       const float4 p = sPts[(t & 1) * kTilePoints + tp];
       // the pair's terms go straight to its own scratch column (no other thread touches it): the first valid
       // observation stores, later ones (other cameras of a rig) add; a pair without one stores zeros
@@ -614,8 +614,8 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     }
     if (tid < npts_next) pn = pts[Tn.point_begin + tid];
     if (vp >= 0 && vp < npts_next) {
-      pp_a = b.point_pair[Tn.point_begin + vp];
-      pp_e = b.point_pair[Tn.point_begin + vp + 1];
+      pp_a = b.point_pair[Tn.point_begin + vp] + Tn.pair_shift;
+      pp_e = b.point_pair[Tn.point_begin + vp + 1] + Tn.pair_shift;
     }
     if (u_owner && u1 > 21) {
       // as in (c); rows -1 .. 8 around the six e_c rows are read (the pair table lies before them, the W scratch

@@ -178,7 +178,18 @@ struct WindowSort {
   std::vector<int64_t> ko;    // sorted (key << 32 | input index) of the observations
   std::vector<int64_t> priv;  // the threads' ranges, each sorted by point
   int npairs = 0;
+  // mapped windows: the points seen from more than kMaxFreePoses free key frames ("wide" points, staged by tiles
+  // of their own), the pair copies their tiles need and where those start in the pair streams
+  std::vector<int32_t> wide;
+  int64_t ndup = 0, dup_off = 0;
 };
+
+// A wide point's free key frames are split into groups of at most kWideGroup consecutive ones; every two groups
+// are staged together by one chunk (at most kMaxFreePoses key frames).
+constexpr int kWideGroup = kMaxFreePoses / 2;
+int wide_groups(int nfree) { return (nfree + kWideGroup - 1) / kWideGroup; }
+// pair_pose of a wide point's pair copy whose free key frame its tile does not stage
+constexpr int32_t kNotStaged = INT32_MAX;
 
 // The host side of a batch while it is packed. A handle keeps one and clears it per batch, so that its vectors keep
 // their capacity across calls: a repeated solve writes into memory that is already mapped (first-touch page faults
@@ -441,12 +452,16 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
       nfree[i] = f;
     }
   });
-  // a mapped window's chunks stage at most kMaxFreePoses free key frames, so no point may see more
+  // a mapped window's chunks stage at most kMaxFreePoses free key frames: points seen from more are wide
+  S.wide.clear();
+  S.ndup = 0;
   if (W.mapped)
     for (int i = 0; i < np; ++i)
-      if (nfree[i] > kMaxFreePoses)
-        return err = "window " + std::to_string(w) + ": a point is observed from more than 40 free key frames",
-               SBA_E_CAPACITY;
+      if (nfree[i] > kMaxFreePoses) {
+        const int64_t K = wide_groups(nfree[i]);
+        S.wide.push_back(i);
+        S.ndup += K * (K - 1) / 2 * npair[i];
+      }
   std::vector<int32_t>& first_pair = S.first_pair;
   first_pair.assign(np + 1, 0);
   for (int i = 0; i < np; ++i) first_pair[i + 1] = first_pair[i] + npair[i];
@@ -504,18 +519,24 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
   for (int j = 0; j < nc; ++j) put_pose(H.poses.data() + 3 * (W.pose_off + j), p.rig_from_world + 16 * j);
   for (int c = 0; c < ncam; ++c) put_pose(H.cams.data() + 3 * (W.cam_off + c), p.camera_from_rig + 16 * c);
   // tiles: consecutive points with at most kTilePairs pairs and max_points points; in a mapped window also at
-  // most kMaxFreePoses distinct free key frames (one tile is staged by one chunk)
+  // most kMaxFreePoses distinct free key frames (one tile is staged by one chunk), and without the wide points
   std::vector<uint8_t> in_tile(W.mapped ? ncf : 0);
+  auto wide = [&](int i) { return W.mapped && S.nfree[i] > kMaxFreePoses; };
   auto make_tiles = [&](std::vector<Tile>& out, int max_points) {
+    const bool schur = max_points == kTilePoints;
     int i = 0;
     while (i < np) {
+      if (schur && wide(i)) {
+        ++i;
+        continue;
+      }
       const int begin = i;
       int pairs = 0, nkf = 0;
       std::fill(in_tile.begin(), in_tile.end(), 0);
-      while (i < np && i - begin < max_points) {
+      while (i < np && i - begin < max_points && !(schur && wide(i))) {
         const int pi = S.npair[i];
         if (pairs + pi > kTilePairs && i > begin) break;
-        if (W.mapped && max_points == kTilePoints) {
+        if (W.mapped && schur) {
           const int q0 = pair_base + first_pair[i];
           int add = 0;
           for (int q = q0; q < q0 + pi; ++q) {
@@ -533,7 +554,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
         ++i;
       }
       const int pb = pair_base + first_pair[begin];
-      out.push_back(Tile{w, point_base + begin, point_base + i, pb, pb + pairs, {0, 0, 0}});
+      out.push_back(Tile{w, point_base + begin, point_base + i, pb, pb + pairs, 0, 0, 0});
     }
   };
   tiles.clear();
@@ -562,6 +583,14 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       }
     }
   });
+  // the wide points' own pairs (in no Schur tile; the back-substitution reads them)
+  for (int i : S.wide) {
+    const auto [qa, qe] = pair_range(point_base + i);
+    for (int q = qa; q < qe; ++q) {
+      const int oe = q + 1 < pair_end ? H.pair_obs[q + 1] : obs_end;
+      H.pair_rec[q] = make_int4(H.pair_obs[q], oe, H.pair_pose[q], 255 << 8);
+    }
+  }
   parallel_ranges(h, T, static_cast<int64_t>(btiles.size()), [&](int, int64_t a, int64_t e) {
     for (int64_t t = a; t < e; ++t) {
       const Tile& Tl = btiles[t];
@@ -571,6 +600,32 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       }
     }
   });
+  // The tiles of the wide points: for groups ga < gb (in this order) a copy of all the point's pairs in which the
+  // free key frames outside ga and gb are marked as not staged (pair_pose kNotStaged, local index 255). The
+  // first tile that stages a group keeps the blocks inside it (and its key frames' U, e), every tile the blocks
+  // between its two groups: each block of the point's Z Z^T is then summed once.
+  int64_t d = S.dup_off;
+  for (int i : S.wide) {
+    const auto [qa, qe] = pair_range(point_base + i);
+    const int m = S.nfree[i], K = wide_groups(m);
+    // the point's pairs are in key frame order, so its free key frames are its first m pairs
+    auto group_begin = [&](int g) { return static_cast<int>(static_cast<int64_t>(g) * m / K); };
+    for (int ga = 0; ga < K; ++ga)
+      for (int gb = ga + 1; gb < K; ++gb) {
+        const int a0 = group_begin(ga), a1 = group_begin(ga + 1), b0 = group_begin(gb), b1 = group_begin(gb + 1);
+        for (int q = qa; q < qe; ++q) {
+          const int k = q - qa, dq = static_cast<int>(d) + k;
+          const bool staged = (k >= a0 && k < a1) || (k >= b0 && k < b1);
+          H.pair_obs[dq] = H.pair_obs[q];
+          H.pair_pose[dq] = k < m && !staged ? kNotStaged : H.pair_pose[q];
+          H.pair_rec[dq] = make_int4(H.pair_rec[q].x, H.pair_rec[q].y, H.pair_rec[q].z, 255 << 8);
+        }
+        const int virt = 1 | (gb == 1 ? 2 : 0) | (ga == 0 ? 4 : 0) | ((a1 - a0) << 8);
+        tiles.push_back(Tile{w, point_base + i, point_base + i + 1, static_cast<int32_t>(d),
+                             static_cast<int32_t>(d) + (qe - qa), static_cast<int32_t>(d) - qa, virt, 0});
+        d += qe - qa;
+      }
+  }
 }
 
 // Packs the n windows into H (phases 1-3 above, phase 2 the batch-wide offsets). A single window uses the threads
@@ -601,9 +656,10 @@ int pack_windows(sba_handle* h, const sba_problem_view* problems, int n, HostPac
   }
   for (int w = 0; w < n; ++w)
     if (rc[w] != SBA_OK) return fail(h, rc[w], err[w]);
-  int64_t obs = 0, pairs = 0, points = 0, poses = 0, cams = 0;
+  int64_t obs = 0, pairs = 0, points = 0, poses = 0, cams = 0, dups = 0;
   for (int w = 0; w < n; ++w) {
     WindowDesc& W = win[w];
+    dups += sorts[w].ndup;
     W.obs_off = static_cast<int32_t>(obs);
     W.pair_off = static_cast<int32_t>(pairs);
     W.point_off = static_cast<int32_t>(points);
@@ -615,19 +671,25 @@ int pack_windows(sba_handle* h, const sba_problem_view* problems, int n, HostPac
     poses += W.nc;
     cams += W.ncam;
   }
-  if (obs > INT32_MAX || pairs > INT32_MAX) return fail(h, SBA_E_CAPACITY, "more than 2^31 observations in a batch");
+  if (obs > INT32_MAX || pairs + dups > INT32_MAX)
+    return fail(h, SBA_E_CAPACITY, "more than 2^31 observations in a batch");
+  // the wide points' pair copies follow the pairs of all windows
+  for (int w = 0, d = 0; w < n; ++w) {
+    sorts[w].dup_off = pairs + d;
+    d += static_cast<int>(sorts[w].ndup);
+  }
   H.obs_orig.resize(obs);
   H.obs_xy.resize(obs);
   H.obs_info.resize(obs);
-  H.pair_obs.resize(pairs + 1);
-  H.pair_pose.resize(pairs);
-  H.pair_rec.resize(pairs);
+  H.pair_obs.resize(pairs + dups + 1);
+  H.pair_pose.resize(pairs + dups);
+  H.pair_rec.resize(pairs + dups);
   H.point_pair.resize(points + 1);
   H.point_nfree.resize(points);
   H.pts.resize(points);
   H.poses.resize(3 * poses);
   H.cams.resize(3 * cams);
-  H.pair_obs[pairs] = static_cast<int32_t>(obs);  // end sentinels
+  H.pair_obs[pairs + dups] = static_cast<int32_t>(obs);  // end sentinels
   H.point_pair[points] = static_cast<int32_t>(pairs);
   {
     TraceRange tr("sba fill");
@@ -941,7 +1003,9 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
       const size_t begin = t;
       int64_t pairs = 0;
       std::vector<int> kfs;  // free key frames of the chunk (mapped windows)
-      while (t < H.tiles.size() && H.tiles[t].window == w && (t == begin || pairs < target)) {
+      // a wide point's tile is a chunk of its own
+      while (t < H.tiles.size() && H.tiles[t].window == w &&
+             (t == begin || (pairs < target && !H.tiles[t].virt && !H.tiles[begin].virt))) {
         const Tile& T = H.tiles[t];
         if (W.mapped) {
           std::vector<int> add;
@@ -981,7 +1045,16 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
     }
   }
   // Mapped windows: the contributions to each key-frame block (a, b), a <= b, of the reduced system, listed
-  // chunk by chunk (counting sort over the blocks), so that the gather sums every block in chunk order.
+  // chunk by chunk (counting sort over the blocks), so that the gather sums every block in chunk order. A chunk
+  // of a wide point contributes the blocks between its two groups, and those inside a group it keeps.
+  auto contributes = [&](const Chunk& c, int la, int lb) {
+    const int v = H.tiles[c.tile_begin].virt;
+    if (!v) return true;
+    const int na = v >> 8;
+    if (lb < na) return (v & 2) != 0;
+    if (la >= na) return (v & 4) != 0;
+    return true;
+  };
   {
     pinned_vector<int32_t>& gfirst = H.gfirst;
     pinned_vector<int2>& gcon = H.gcon;
@@ -997,7 +1070,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
       for (int ci = c0; ci < c1; ++ci) {
         const int32_t* m = chunk_map.data() + chunks[ci].map_off;
         for (int la = 0; la < chunks[ci].nloc; ++la)
-          for (int lb = la; lb < chunks[ci].nloc; ++lb) ++f[static_cast<size_t>(m[la]) * ncf + m[lb] + 1];
+          for (int lb = la; lb < chunks[ci].nloc; ++lb)
+            if (contributes(chunks[ci], la, lb)) ++f[static_cast<size_t>(m[la]) * ncf + m[lb] + 1];
       }
       f[0] = static_cast<int32_t>(gcon.size());
       for (size_t i = 1; i <= nb; ++i) f[i] += f[i - 1];
@@ -1007,7 +1081,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
         const int32_t* m = chunk_map.data() + chunks[ci].map_off;
         for (int la = 0; la < chunks[ci].nloc; ++la)
           for (int lb = la; lb < chunks[ci].nloc; ++lb)
-            gcon[cur[static_cast<size_t>(m[la]) * ncf + m[lb]]++] = make_int2(ci, la | (lb << 8));
+            if (contributes(chunks[ci], la, lb))
+              gcon[cur[static_cast<size_t>(m[la]) * ncf + m[lb]]++] = make_int2(ci, la | (lb << 8));
       }
     }
   }
@@ -1065,7 +1140,9 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   L.upload(d.pair_obs, H.pair_obs);
   L.upload(d.pair_pose, H.pair_pose);
   L.upload(d.pair_rec, H.pair_rec);
-  const bool pair_streams = H.obs_xy.size() != H.pair_rec.size();
+  bool copies = false;  // wide points' pair copies
+  for (int w = 0; w < n; ++w) copies = copies || H.sorts[w].ndup > 0;
+  const bool pair_streams = H.obs_xy.size() != H.pair_rec.size() || copies;
   if (pair_streams) {
     // some pair has several observations: the first observation of every pair, indexed by pair
     H.pair_xy.resize(H.pair_rec.size());

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -134,6 +135,8 @@ static void check_ranges(std::vector<Range> rs) {
   }
 }
 
+static int g_wide_points = 0;  // wide points seen in checked batches
+
 static void check_batch(const Batch& b) {
   const int nw = b.n_windows;
   CHECK(nw > 0, "no windows");
@@ -164,30 +167,66 @@ static void check_batch(const Batch& b) {
   }
   for (int c = 0; c < b.n_chunks; ++c)
     if (b.win[b.chunks[c].window].mapped) cmap = std::max<int64_t>(cmap, b.chunks[c].map_off + b.chunks[c].nloc);
-  // the gather plan of each mapped window: block (a, b) lists, in increasing chunk order, exactly the chunks of
-  // the window that stage both a and b, with their local indices
+  // the gather plan of each mapped window: block (a, b) lists its contributions in increasing chunk order, each
+  // naming a chunk of the window that stages a and b at the given local indices; every ordinary chunk that stages
+  // both contributes, and the tiles of a wide point (a point seen from more than kMaxFreePoses free key frames)
+  // contribute every block between two of its free key frames exactly once
   for (int w = 0; w < nw; ++w) {
     const WindowDesc& W = b.win[w];
     if (!W.mapped) continue;
     const int ncf = W.ncf;
     const int32_t* f = b.gfirst + W.gfirst_off;
     gc = std::max<int64_t>(gc, f[static_cast<int64_t>(ncf) * ncf]);
-    std::vector<int> want(static_cast<size_t>(ncf) * ncf, 0);
+    std::vector<int> ordinary(static_cast<size_t>(ncf) * ncf, 0);
+    std::map<int, std::set<int>> wide_kfs;  // wide point -> the free key frames its tiles stage
     for (int c = W.chunk_begin; c < W.chunk_begin + W.nchunks; ++c) {
       const Chunk& ch = b.chunks[c];
       CHECK(ch.window == w && ch.nloc <= kMaxFreePoses, "mapped chunk");
+      const Tile& T = b.tiles[ch.tile_begin];
+      if (T.virt) {
+        CHECK(ch.tile_end == ch.tile_begin + 1 && T.point_end == T.point_begin + 1, "a wide point's chunk is one tile");
+        for (int l = 0; l < ch.nloc; ++l) wide_kfs[T.point_begin].insert(b.chunk_map[ch.map_off + l]);
+        continue;
+      }
       for (int la = 0; la < ch.nloc; ++la)
         for (int lb = la; lb < ch.nloc; ++lb) {
           const int ka = b.chunk_map[ch.map_off + la], kb = b.chunk_map[ch.map_off + lb];
           CHECK(ka < kb || la == lb, "chunk map increasing");
-          const int k = f[ka * ncf + kb] + want[ka * ncf + kb]++;
-          CHECK(k < f[ka * ncf + kb + 1], "gather plan: block holds fewer contributions than chunks stage it");
-          if (k < f[ka * ncf + kb + 1])
-            CHECK(b.gcon[k].x == c && b.gcon[k].y == (la | (lb << 8)), "gather plan entry (chunk order)");
+          ++ordinary[ka * ncf + kb];
         }
     }
-    for (int k = 0; k < ncf * ncf; ++k)
-      CHECK(f[k] + want[k] == f[k + 1], "gather plan: block holds contributions of no chunk");
+    std::map<std::pair<int, int>, int> wide_count;  // (wide point, block) -> contributions
+    for (int ka = 0; ka < ncf; ++ka)
+      for (int kb = 0; kb < ncf; ++kb) {
+        const int blk = ka * ncf + kb;
+        int n_ord = 0, last = -1;
+        for (int k = f[blk]; k < f[blk + 1]; ++k) {
+          const int c = b.gcon[k].x, la = b.gcon[k].y & 255, lb = b.gcon[k].y >> 8;
+          CHECK(c > last && c >= W.chunk_begin && c < W.chunk_begin + W.nchunks, "gather plan: chunk order");
+          last = c;
+          if (c < W.chunk_begin || c >= W.chunk_begin + W.nchunks) break;
+          const Chunk& ch = b.chunks[c];
+          CHECK(la <= lb && lb < ch.nloc && b.chunk_map[ch.map_off + la] == ka && b.chunk_map[ch.map_off + lb] == kb,
+                "gather plan: local indices");
+          const Tile& T = b.tiles[ch.tile_begin];
+          if (T.virt) ++wide_count[{T.point_begin, blk}];
+          else ++n_ord;
+        }
+        CHECK(n_ord == ordinary[blk], "gather plan: the ordinary chunks that stage a block");
+      }
+    for (const auto& [pt, kfs] : wide_kfs) {
+      CHECK(kfs.size() > static_cast<size_t>(kMaxFreePoses), "a wide point sees more than kMaxFreePoses key frames");
+      for (int ka : kfs)
+        for (int kb : kfs)
+          if (ka <= kb) {
+            auto it = wide_count.find({pt, ka * ncf + kb});
+            CHECK(it != wide_count.end() && it->second == 1, "gather plan: a wide point's block summed once");
+          }
+    }
+    size_t n_wide = 0;
+    for (const auto& [pt, kfs] : wide_kfs) n_wide += kfs.size() * (kfs.size() + 1) / 2;
+    CHECK(wide_count.size() == n_wide, "gather plan: blocks of a wide point outside its key frames");
+    g_wide_points = std::max(g_wide_points, static_cast<int>(wide_kfs.size()));
   }
   for (int c = 0; c < b.n_bchunks; ++c) nbtiles = std::max<int64_t>(nbtiles, b.bchunks[c].tile_end);
   for (int64_t t = 0; t < ntiles; ++t) {
@@ -510,6 +549,75 @@ int main() {
     sba_batch_destroy(h, b);
     sba_result r = result_for(loop);
     CHECK(sba_solve(h, &loop.v, &r) == SBA_OK, sba_last_error(h));
+  }
+
+  g_case = "batch: mapped window with points seen from more than 40 free key frames";
+  std::printf("%s\n", g_case.c_str());
+  {
+    Scene loop;
+    make_loop(loop, 48, 3000);
+    // the loop window plus 5 points (before the fixed ones) seen from 41 .. 47 key frames (free and fixed)
+    const sba_problem_view& v = loop.v;
+    const int np0 = v.num_points, nfp = v.num_fixed_points, add = 5, nc = v.num_poses;
+    std::vector<float> pts(v.points, v.points + 3 * np0), xy(v.observation_xys, v.observation_xys + 2 * v.num_observations),
+        info(v.observation_infos, v.observation_infos + 4 * v.num_observations);
+    std::vector<int32_t> pid(v.point_ids, v.point_ids + v.num_observations), qid(v.pose_ids, v.pose_ids + v.num_observations);
+    std::vector<int8_t> cid(v.camera_ids, v.camera_ids + v.num_observations);
+    for (int& i : pid)
+      if (i >= np0 - nfp) i += add;  // the fixed points move behind the new ones
+    pts.insert(pts.begin() + 3 * (np0 - nfp), 3 * add, 1.0f);
+    for (int k = 0; k < add; ++k) {
+      const int i = np0 - nfp + k;
+      for (int j = 0; j < std::min(nc, 41 + k); ++j) {
+        const int jj = (j * 7 + k) % nc;  // key frames in scattered order
+        pid.push_back(i);
+        qid.push_back(jj);
+        cid.push_back(static_cast<int8_t>(j % v.num_cameras));
+        xy.push_back(0.01f * j);
+        xy.push_back(-0.01f * k);
+        for (float x : {1e4f, 0.f, 0.f, 1e4f}) info.push_back(x);
+      }
+    }
+    sba_problem_view w = v;
+    w.num_points = np0 + add;
+    w.points = pts.data();
+    w.num_observations = static_cast<int>(pid.size());
+    w.observation_xys = xy.data();
+    w.observation_infos = info.data();
+    w.point_ids = pid.data();
+    w.pose_ids = qid.data();
+    w.camera_ids = cid.data();
+    Scene small;
+    make_scene(small, 8, 500, 41);
+    sba_problem_view vs[2] = {small.v, w};
+    g_wide_points = 0;
+    std::vector<std::vector<char>> packed[2];
+    for (int threads : {1, 8}) {
+      sba_options o1 = o;
+      o1.pack_threads = threads;
+      sba_handle* h1 = nullptr;
+      CHECK(sba_create(&o1, &h1) == SBA_OK, "sba_create");
+      sba_batch* b = nullptr;
+      CHECK(sba_batch_create(h1, vs, 2, &b) == SBA_OK, sba_last_error(h1));
+      CHECK(sba_batch_run(h1, b) == SBA_OK, "run");
+      packed[threads == 8] = g_packed;
+      sba_batch_destroy(h1, b);
+      std::vector<float> rp(3 * w.num_points), rq(16 * w.num_poses);
+      sba_result r{};
+      r.points = rp.data();
+      r.rig_from_world = rq.data();
+      CHECK(sba_solve(h1, &w, &r) == SBA_OK, sba_last_error(h1));
+      sba_destroy(h1);
+    }
+    int expect = 0;  // points seen from more than 40 free key frames
+    for (int k = 0; k < add; ++k) {
+      int f = 0;
+      for (int j = 0; j < std::min(nc, 41 + k); ++j) f += (j * 7 + k) % nc < nc - v.num_fixed_key_frames;
+      expect += f > 40;
+    }
+    std::printf("  wide points %d (expected %d)\n", g_wide_points, expect);
+    CHECK(expect >= 3 && g_wide_points == expect, "the wide points have tiles of their own");
+    CHECK(packed[0] == packed[1], "packing with wide points on 1 and on 8 host threads");
   }
 
   g_case = "packing on 1 and on 8 host threads gives the same streams";

@@ -253,6 +253,64 @@ def test_large_multicamera_window_matches_oracle():
     assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
 
 
+def _add_far_points(p, gt_poses, n, seed):
+    """p with n more points 30-60 m out in the plane of the loop, observed (without noise) by every camera of every
+    key frame that sees them in front within |x/z|, |y/z| < 1; on the multi-camera loop each is seen from nearly
+    every key frame."""
+    rng = np.random.default_rng(seed)
+    centers = np.array([-(q[:3, :3].T @ q[:3, 3]) for q in gt_poses.astype(np.float64)])
+    mid = centers.mean(0)
+    up = np.argmin(centers.var(0))  # the loop's vertical axis: the one its key frames vary least along
+    flat = [k for k in range(3) if k != up]
+    ang = rng.uniform(0, 2 * np.pi, n)
+    rad = rng.uniform(30, 60, n)
+    X = np.tile(mid, (n, 1))
+    X[:, flat[0]] += rad * np.cos(ang)
+    X[:, flat[1]] += rad * np.sin(ang)
+    X[:, up] += rng.uniform(-1, 1, n)
+    pid, qid, cid, xy = [], [], [], []
+    base = p.points.shape[0]
+    for i in range(n):
+        for j in range(gt_poses.shape[0]):
+            for cam in range(p.camera_from_rig.shape[0]):
+                T = p.camera_from_rig[cam].astype(np.float64) @ gt_poses[j].astype(np.float64)
+                Xc = T[:3, :3] @ X[i] + T[:3, 3]
+                if Xc[2] > 0.5 and abs(Xc[0] / Xc[2]) < 1 and abs(Xc[1] / Xc[2]) < 1:
+                    pid.append(base + i)
+                    qid.append(j)
+                    cid.append(cam)
+                    xy.append(Xc[:2] / Xc[2])
+    q = p.copy()
+    q.points = np.concatenate([p.points, (X + rng.normal(0, 0.05, X.shape)).astype(np.float32)])
+    q.observation_xys = np.concatenate([p.observation_xys, np.array(xy, np.float32)])
+    q.observation_infos = np.concatenate([p.observation_infos, np.repeat(p.observation_infos[:1], len(xy), 0)])
+    q.point_ids = np.concatenate([p.point_ids, np.array(pid, np.int32)])
+    q.pose_ids = np.concatenate([p.pose_ids, np.array(qid, np.int32)])
+    q.camera_ids = np.concatenate([p.camera_ids, np.array(cid, np.int8)])
+    return q
+
+
+def test_large_window_with_points_seen_from_more_than_40_key_frames():
+    # the mapped window above plus 20 far points seen from (nearly) all 59 free key frames: each such point is
+    # staged by tiles of its own (its key frames in groups of at most 20, every two groups together)
+    p0, _, gt_poses = ca.scene.loop(num_keyframes=60, num_points=20000, loop_length_m=12.0, max_iterations=10)
+    p = _add_far_points(p0, gt_poses, 20, seed=3)
+    nf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
+    new = p.point_ids >= p0.points.shape[0]
+    seen = [np.unique(p.pose_ids[new & (p.point_ids == i)]) for i in range(p0.points.shape[0], p.points.shape[0])]
+    assert sum((s < nf).sum() > 40 for s in seen) >= 15
+    q = p.copy()
+    s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    assert s.solve(q)
+    r = O.solve(p, "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
+    assert q.iterations == r["iterations"] == 10
+    assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < 1e-6
+    assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
+    q2 = p.copy()
+    assert s.solve(q2)
+    assert np.array_equal(q.rig_from_world, q2.rig_from_world) and np.array_equal(q.points, q2.points)
+
+
 def test_large_window_in_a_batch_with_small_windows():
     big, _, _ = ca.scene.loop(num_keyframes=48, num_points=6000, loop_length_m=9.6, max_iterations=6)
     small = ca.scene.batched_windows(3, seed=11, max_iterations=6)
