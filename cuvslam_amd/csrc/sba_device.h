@@ -35,7 +35,7 @@ constexpr int kBlock = 256;        // threads of the per-tile kernels
 constexpr int kTilePairs = 256;    // max (point, key frame) pairs per tile (one per thread)
 constexpr int kTilePoints = 16;    // max points per tile: 48 Schur columns, 12 matrix-core k-steps
 constexpr int kMaxFreePoses = 40;  // free key frames per window (reduced system order <= 240)
-constexpr int kMaxPoses = 256;
+constexpr int kMaxPoses = 1024;  // key frames per window
 constexpr int kMaxCameras = 32;    // camera::Rig::kMaxCameras (libs/camera/rig.h:30)
 constexpr int kNT = 15;            // 16x16 tiles per side of the largest reduced system (240 / 16)
 constexpr int kZcLd = 240;         // leading dimension of the staged Z columns (bank-conflict free)
@@ -112,6 +112,7 @@ struct Chunk {
 struct Batch {
   int32_t n_windows, n_chunks, n_points, n_poses, n_cams;
   int32_t loss, fixed_iterations, max_rec;
+  int32_t max_nc;  // key frames of the largest window (sizes the back-substitution's shared memory)
   const WindowDesc* win;
   WindowState* st;
   const Tile* tiles;     // Schur tiles (<= kTilePairs pairs, <= kTilePoints points)
@@ -125,7 +126,7 @@ struct Batch {
   const int32_t* pair_obs;
   const int32_t* pair_pose;
   const int4* pair_rec;      // per pair: (first observation, end of its observations, key frame | point index
-                             // inside its back-substitution tile << 8, point index inside its Schur tile |
+                             // inside its back-substitution tile << 16, point index inside its Schur tile |
                              // local free key frame index << 8, 255 when fixed)
   const float2* pair_xy;     // per pair: its first observation (the obs streams themselves when every pair has
   const float4* pair_info;   // exactly one observation)

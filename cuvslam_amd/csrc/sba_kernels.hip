@@ -420,7 +420,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       tp = pre.rec.w & 255;
       jl = pre.rec.w >> 8;
       const int i = T.point_begin + tp;
-      j = pre.rec.z & 255;
+      j = pre.rec.z & 0xffff;
       pair_pf = i < pf_end;
       pair_cf = j < ncf && jl != 255;  // a free key frame staged by this chunk
       const float4 p = sPts[(t & 1) * kTilePoints + tp];
@@ -1080,12 +1080,15 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
 // ---------------------------------------------------------------------------------------------------------
 template <bool INITIAL>
 __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
-  __shared__ float sPose[12 * kMaxPoses];
-  __shared__ float sTrial[12 * kMaxPoses];
+  // the window's key frames (current and trial) and pose step: 30 floats per key frame of the batch's largest
+  // window, in the dynamic shared memory
+  extern __shared__ float sDyn[];
+  float* sPose = sDyn;
+  float* sTrial = sDyn + 12 * b.max_nc;
+  float* sDc = sDyn + 24 * b.max_nc;
   __shared__ float sCam[12 * kMaxCameras];
   __shared__ float sAcc[4 * kBlock];  // per tile point (up to kBlock): dp
   __shared__ float sPair[3 * kBlock];  // per pair: its part of sum_j W_ij^T dc_j
-  __shared__ float sDc[6 * kMaxPoses];
   __shared__ double sRed[3 * (kBlock / 64)];
   __shared__ int sRedI[kBlock / 64];
   __shared__ float sRedM[kBlock / 64];
@@ -1140,9 +1143,9 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
       // sum over its pairs is taken in pair order in (b): no floating-point atomics)
       if (tid < npair) {
         float acc[3] = {0.f, 0.f, 0.f};
-        const int tp = rec.z >> 8;  // point inside this back-substitution tile
+        const int tp = rec.z >> 16;  // point inside this back-substitution tile
         const int i = T.point_begin + tp;
-        const int j = rec.z & 255;
+        const int j = rec.z & 0xffff;
         if (i < pf_end && j < ncf) {
           const float4 p = pts[i];
           const float* d = sDc + 6 * j;
@@ -1222,9 +1225,9 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
     }
     // (c) one thread per pair: trial cost in FP64, and w |J_c dc + J_p dp|^2_Lambda
     if (tid < npair) {
-      const int tp = rec.z >> 8;
+      const int tp = rec.z >> 16;
       const int i = T.point_begin + tp;
-      const int j = rec.z & 255;
+      const int j = rec.z & 0xffff;
       const float4 p = pts[i];
       float dp[3] = {0.f, 0.f, 0.f}, dcj[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (!INITIAL) {
@@ -1486,11 +1489,16 @@ hipError_t launch_factor(const Batch& b, double* dbg_S, hipStream_t s, double* s
   return hipGetLastError();
 }
 
+// Dynamic shared memory of the back-substitution: current and trial key frames and the pose step of the batch's
+// largest window (the static arrays take about 9 KB besides).
+static size_t backsub_lds_bytes(const Batch& b) { return sizeof(float) * 30 * static_cast<size_t>(b.max_nc > 0 ? b.max_nc : 1); }
+static_assert(sizeof(float) * 30 * kMaxPoses + 9 * 1024 <= 160 * 1024, "back-substitution shared memory");
+
 hipError_t launch_backsub(const Batch& b, bool initial, hipStream_t s) {
   if (initial)
-    hipLaunchKernelGGL(k_backsub<true>, dim3(b.n_bchunks), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_backsub<true>, dim3(b.n_bchunks), dim3(kBlock), backsub_lds_bytes(b), s, b);
   else
-    hipLaunchKernelGGL(k_backsub<false>, dim3(b.n_bchunks), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_backsub<false>, dim3(b.n_bchunks), dim3(kBlock), backsub_lds_bytes(b), s, b);
   return hipGetLastError();
 }
 

@@ -413,10 +413,10 @@ __global__ __launch_bounds__(256) void k_syrk(Batch b, int w, int k) {
 // Backward substitution L^T x = y in one workgroup of 1024 threads, 64-row blocks from the last: for block bk,
 // r = y_bk - sum_{i past the block} L[i][block bk]^T x_i (16 groups of 64 threads, each group one row at a time
 // with its 64 lanes on the block's 64 contiguous columns), then x_bk = X_bk^T r with the kept inverse. x stays
-// in LDS (n <= 6 x 255 = 1530) and goes to rhs at the end.
+// in the dynamic shared memory (n doubles) and goes to rhs at the end.
 __global__ __launch_bounds__(1024) void k_bsolve(Batch b, int w) {
   constexpr int kG = 16;                        // row groups
-  __shared__ double sx[6 * kMaxPoses];           // y, overwritten by x block by block
+  extern __shared__ double sx[];                 // y, overwritten by x block by block
   __shared__ double part[kG * kLB];
   __shared__ double sr[kLB];
   const WindowDesc W = b.win[w];
@@ -523,7 +523,7 @@ hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s) {
     e = hipGetLastError();
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bsolve, dim3(1), dim3(1024), 0, s, b, w);
+  hipLaunchKernelGGL(k_bsolve, dim3(1), dim3(1024), sizeof(double) * n, s, b, w);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lepilogue, dim3(1), dim3(256), 0, s, b, w);

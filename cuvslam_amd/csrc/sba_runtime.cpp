@@ -327,7 +327,8 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
       (no && (!p.observation_xys || !p.observation_infos || !p.point_ids || !p.pose_ids || !p.camera_ids)))
     return err = "null buffer" + where, SBA_E_INVALID;
   const int ncf = nc - p.num_fixed_key_frames;
-  if (nc > kMaxPoses) return err = "window " + std::to_string(w) + " has more than 256 key frames", SBA_E_CAPACITY;
+  if (nc > kMaxPoses)
+    return err = "window " + std::to_string(w) + " has more than " + std::to_string(kMaxPoses) + " key frames", SBA_E_CAPACITY;
   if (ncam > kMaxCameras) return err = "more than 32 cameras", SBA_E_CAPACITY;
   W = WindowDesc{};
   W.np = np;
@@ -561,7 +562,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
   btiles.clear();
   make_tiles(tiles, kTilePoints);
   make_tiles(btiles, kBlock);
-  // pair records: observation range, key frame | point index inside its back-substitution tile << 8, point
+  // pair records: observation range, key frame | point index inside its back-substitution tile << 16, point
   // index inside its Schur tile | local key frame << 8 (ranges end at the window's own ends)
   const int pair_end = pair_base + S.npairs, obs_end = obs_base + no;
   auto pair_range = [&](int i) {  // batch-wide pair range of batch-wide point i of this window
@@ -596,7 +597,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       const Tile& Tl = btiles[t];
       for (int i = Tl.point_begin; i < Tl.point_end; ++i) {
         const auto [qa, qe] = pair_range(i);
-        for (int q = qa; q < qe; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - Tl.point_begin) << 8);
+        for (int q = qa; q < qe; ++q) H.pair_rec[q].z = H.pair_pose[q] | ((i - Tl.point_begin) << 16);
       }
     }
   });
@@ -1125,6 +1126,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   d.loss = h->opt.loss;
   d.fixed_iterations = h->opt.fixed_iterations;
   d.max_rec = max_rec;
+  d.max_nc = max_nc;
   Layout L;
   L.upload(d.win, b->win);
   L.upload(d.tiles, H.tiles);

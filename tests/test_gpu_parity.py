@@ -311,6 +311,27 @@ def test_large_window_with_points_seen_from_more_than_40_key_frames():
     assert np.array_equal(q.rig_from_world, q2.rig_from_world) and np.array_equal(q.points, q2.points)
 
 
+def test_window_of_300_key_frames_matches_oracle():
+    # more key frames than a 256-entry table would hold: 299 free key frames (reduced system of order 1794); the
+    # key frame index of a pair record and the back-substitution's key frame staging are sized for 1024. The
+    # 60 m loop with one fixed key frame is weakly constrained along it: the oracle's FP32 restatement of the
+    # method ends 5.7e-7 from the FP64 solve (with 3% outliers 1e-5), so the bound is ten times that distance,
+    # computed here as for the 200-key-frame map.
+    p, _, _ = ca.scene.loop(num_keyframes=300, num_points=40000, loop_length_m=60.0, max_iterations=5,
+                            outlier_rate=0.0)
+    assert p.rig_from_world.shape[0] == 300
+    q = p.copy()
+    s = ca.SchurComplementBundlerGpu(loss=ca.LOSS_HUBER, fixed_iterations=True)
+    assert s.solve(q)
+    r = O.solve(p, "f64", loss=ca.LOSS_HUBER, fixed_iterations=1)
+    from sensitivity import fp32_method_spread
+    spread, _, _ = fp32_method_spread(p, ca.LOSS_HUBER, reference=r)
+    assert q.iterations == r["iterations"] == 5
+    assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < max(1e-6, 10 * spread), spread
+    nf = p.rig_from_world.shape[0] - p.num_fixed_key_frames
+    assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
+
+
 def test_large_window_in_a_batch_with_small_windows():
     big, _, _ = ca.scene.loop(num_keyframes=48, num_points=6000, loop_length_m=9.6, max_iterations=6)
     small = ca.scene.batched_windows(3, seed=11, max_iterations=6)
