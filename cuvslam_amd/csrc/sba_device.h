@@ -142,10 +142,11 @@ struct Batch {
   const float4* cams;
   const int32_t* chunk_map;  // local -> window free key frame, per chunk of a mapped window
   // mapped windows: the contributions to key-frame block (a, b), a <= b, of the reduced system are
-  // gcon[gfirst[W.gfirst_off + a ncf + b] .. gfirst[... + 1]), in chunk order, each (chunk, la | lb << 8) with
-  // la, lb the chunk-local indices of a and b
+  // gcon[gfirst[W.gfirst_off + a ncf + b] .. gfirst[... + 1]), in chunk order, each (chunk, la | lb << 8 |
+  // nt << 16 | nloc << 24, part_off low and high 32 bits) with la, lb the chunk-local indices of a and b and
+  // nt, nloc, part_off those of the chunk
   const int32_t* gfirst;
-  const int2* gcon;
+  const int4* gcon;
   float* part;       // per-chunk Schur records
   float* sred;       // per-window reduced records
   double* A;

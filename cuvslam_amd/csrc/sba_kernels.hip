@@ -303,6 +303,32 @@ __device__ __forceinline__ void store_one(float* rec, int lane, const floatx4 (&
   }
 }
 
+// Mapped windows: the record holds the upper 6 x 6 key-frame blocks (la <= lb) one after another, 36 floats each
+// (row-major; of a diagonal block only the upper triangle is written), so that a key-frame block of the window's
+// system reads each chunk's part of it as one contiguous piece.
+template <int NT, int WAVE, int t>
+__device__ __forceinline__ void store_one_blocked(float* rec, int lane, int nloc,
+                                                  const floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  constexpr int T = WAVE + 4 * t;
+  if constexpr (T < NT * (NT + 1) / 2) {
+    constexpr int I = tri_i(NT, T), J = tri_j(NT, T);
+    const int cl = 16 * J + (lane & 15), lb = cl / 6;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rl = 16 * I + 4 * (lane >> 4) + v, la = rl / 6;
+      if (rl <= cl && cl < 6 * nloc)
+        rec[36 * (la * nloc - (la * (la - 1)) / 2 + (lb - la)) + 6 * (rl - 6 * la) + (cl - 6 * lb)] = acc[t][v];
+    }
+  }
+}
+
+template <int NT, int WAVE, int... ts>
+__device__ __forceinline__ void store_all_blocked(float* rec, int lane, int nloc,
+                                                  const floatx4 (&acc)[tiles_per_wave<NT>()],
+                                                  std::integer_sequence<int, ts...>) {
+  (store_one_blocked<NT, WAVE, ts>(rec, lane, nloc, acc), ...);
+}
+
 template <int NT, int WAVE, int... ts>
 __device__ __forceinline__ void store_all(float* rec, int lane, const floatx4 (&acc)[tiles_per_wave<NT>()],
                                           std::integer_sequence<int, ts...>) {
@@ -310,8 +336,10 @@ __device__ __forceinline__ void store_all(float* rec, int lane, const floatx4 (&
 }
 
 template <int NT, int WAVE>
-__device__ __forceinline__ void store_tiles(float* rec, int lane, const floatx4 (&acc)[tiles_per_wave<NT>()]) {
-  store_all<NT, WAVE>(rec, lane, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
+__device__ __forceinline__ void store_tiles(float* rec, int lane, int nloc, bool blocked,
+                                            const floatx4 (&acc)[tiles_per_wave<NT>()]) {
+  if (blocked) store_all_blocked<NT, WAVE>(rec, lane, nloc, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
+  else store_all<NT, WAVE>(rec, lane, acc, std::make_integer_sequence<int, tiles_per_wave<NT>()>{});
 }
 
 // Workgroup barrier for shared memory only: waits for this wave's LDS operations, not for its global loads, so
@@ -648,16 +676,18 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     }
     if (tid < npts_next) sPts[((t + 1) & 1) * kTilePoints + tid] = pn;
   }
-  // the chunk's record: Schur tiles (lane order), then U (21 per key frame), e (6), diag U (6)
+  // the chunk's record: Schur tiles (lane order; key-frame blocks in a mapped window), then U (21 per key frame),
+  // e (6), diag U (6)
   float* rec = b.part + ch.part_off;
+  const bool blocked = W.mapped != 0;
   if (nt > 0 && ch.nt > 0) switch (tid >> 6) {
-    case 0: store_tiles<NT, 0>(rec, lane, acc); break;
-    case 1: store_tiles<NT, 1>(rec, lane, acc); break;
-    case 2: store_tiles<NT, 2>(rec, lane, acc); break;
-    default: store_tiles<NT, 3>(rec, lane, acc); break;
+    case 0: store_tiles<NT, 0>(rec, lane, nloc, blocked, acc); break;
+    case 1: store_tiles<NT, 1>(rec, lane, nloc, blocked, acc); break;
+    case 2: store_tiles<NT, 2>(rec, lane, nloc, blocked, acc); break;
+    default: store_tiles<NT, 3>(rec, lane, nloc, blocked, acc); break;
   }
   if (u_owner) {
-    float* ru = rec + 256 * (nt * (nt + 1) / 2);
+    float* ru = rec + (blocked ? 36 * (nloc * (nloc + 1) / 2) : 256 * (nt * (nt + 1) / 2));
     constexpr int kDiag[6] = {0, 6, 11, 15, 18, 20};  // diagonal entries of the packed upper 6x6
 #pragma unroll
     for (int v = 0; v < kUGroup; ++v) {

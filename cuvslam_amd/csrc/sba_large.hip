@@ -31,6 +31,8 @@ using namespace sba_tiles;
 namespace {
 
 constexpr int kLB = 64;        // block of the large factorization
+constexpr int kGather = 8;     // contributions whose loads k_gather issues together
+constexpr int kGatherWaves = 8;  // wavefronts of k_gather that share one key-frame block's contributions
 
 struct Large {
   double* A;     // n x n, row-major; the lower triangle holds A, then L
@@ -50,18 +52,26 @@ __device__ __forceinline__ Large large_of(const Batch& b, const WindowDesc& W) {
 }  // namespace
 
 // A = U + lambda diag(U) - sum Z Z^T (lower triangle), rhs = e, du = diag(U); chol_ok set for the factor.
-// One wavefront per key-frame block (a, b), a = blockIdx.y <= b = 4 blockIdx.x + wavefront: lane 6 r + c sums
-// record element (6 la + r, 6 lb + c) over the block's contributions in their listed (chunk) order, in FP64, and
-// writes A(6 b + c, 6 a + r); in a diagonal block lanes with r <= c add U and lambda diag U, lanes 36..41 sum e
-// and lanes 42..47 diag U. The sums do not depend on the order the hardware runs the chunks or the wavefronts.
-__global__ __launch_bounds__(256) void k_gather(Batch b, int w) {
+// One workgroup per key-frame block (a, b), a <= b (blockIdx.x enumerates the upper blocks row by row). Lane 6 r + c
+// of each of the kGatherWaves wavefronts sums element (r, c) of the chunk-local block (la, lb) (36 contiguous
+// floats of a record) in FP64 over its share of the block's contributions, in their listed (chunk) order; the
+// shares' sums are added in wavefront order.
+// A(6 b + c, 6 a + r) = -sum; in a diagonal block lanes with r <= c add U and lambda diag U, lanes 36..41 sum e
+// and lanes 42..47 diag U. The sums do not depend on the order the hardware runs the chunks or the workgroups.
+__global__ __launch_bounds__(64 * kGatherWaves) void k_gather(Batch b, int w) {
+  __shared__ double red[3][kGatherWaves - 1][64];  // wavefronts 1..: their sums z, u, d
   const WindowDesc W = b.win[w];
   WindowState* St = b.st + w;
   if (St->done) return;
   const int ncf = W.ncf, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ka = blockIdx.y, kb = 4 * blockIdx.x + wave;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) St->chol_ok = 1;
-  if (kb >= ncf || kb < ka) return;
+  // upper block t = blockIdx.x: row ka holds blocks ka..ncf-1, rows before it ka ncf - ka (ka - 1) / 2 blocks
+  const int t = blockIdx.x;
+  int ka = static_cast<int>((2.0 * ncf + 1.0 - sqrt((2.0 * ncf + 1.0) * (2.0 * ncf + 1.0) - 8.0 * t)) / 2.0);
+  ka = max(0, min(ka, ncf - 1));
+  while (ka > 0 && ka * ncf - (ka * (ka - 1)) / 2 > t) --ka;
+  while (ka + 1 < ncf && (ka + 1) * ncf - ((ka + 1) * ka) / 2 <= t) ++ka;
+  const int kb = ka + t - (ka * ncf - (ka * (ka - 1)) / 2);
+  if (t == 0 && threadIdx.x == 0) St->chol_ok = 1;
   const Large L = large_of(b, W);
   const int n = L.n;
   const bool diag = ka == kb;
@@ -70,27 +80,57 @@ __global__ __launch_bounds__(256) void k_gather(Batch b, int w) {
   int kind = -1;
   if (lane < 36 && (!diag || r <= c)) kind = 0;
   else if (diag && lane >= 36 && lane < 48) kind = lane < 42 ? 1 : 2;
-  if (kind < 0) return;
   const int q = lane >= 36 ? (lane - 36) % 6 : 0;
   const int32_t* first = b.gfirst + W.gfirst_off + ka * ncf + kb;
+  const int nall = first[1] - first[0];
+  const int kbeg = first[0] + (nall * wave) / kGatherWaves, kend = first[0] + (nall * (wave + 1)) / kGatherWaves;
+  const bool want_u = kind == 0 && diag, want_d = want_u && r == c;
   double z = 0.0, u = 0.0, d = 0.0;
-  for (int k = first[0], end = first[1]; k < end; ++k) {
-    const int2 g = b.gcon[k];
-    const Chunk ch = b.chunks[g.x];
-    const float* rec = b.part + ch.part_off;
-    const int la = g.y & 255, lb = g.y >> 8, nt = ch.nt, nloc = ch.nloc;
-    const float* tail = rec + 256 * (nt * (nt + 1) / 2);
-    if (kind == 0) {
-      const int rl = 6 * la + r, cl = 6 * lb + c, I = rl >> 4, J = cl >> 4, r16 = rl & 15;
-      const int tile = I * nt - (I * (I - 1)) / 2 + (J - I);
-      z += static_cast<double>(rec[256 * tile + 64 * (r16 & 3) + 16 * (r16 >> 2) + (cl & 15)]);
-      if (diag) {
-        u += static_cast<double>(tail[21 * la + 6 * r - (r * (r - 1)) / 2 + (c - r)]);
-        if (r == c) d += static_cast<double>(tail[27 * nloc + 6 * la + r]);
+  // kGather contributions at a time: their plan entries, then their record values, all loads of a batch in
+  // flight together; the values are then added one after another in the plan's order
+  for (int k0 = kbeg; kind >= 0 && k0 < kend; k0 += kGather) {
+    int64_t oz[kGather], ou[kGather], od[kGather];
+#pragma unroll
+    for (int t = 0; t < kGather; ++t) {
+      const int4 g = b.gcon[min(k0 + t, kend - 1)];
+      const int64_t rec = (static_cast<int64_t>(g.w) << 32) | static_cast<uint32_t>(g.z);
+      const int la = g.y & 255, lb = (g.y >> 8) & 255, nloc = g.y >> 24;
+      const int64_t tail = rec + 36 * (nloc * (nloc + 1) / 2);
+      if (kind == 0) {
+        oz[t] = rec + 36 * (la * nloc - (la * (la - 1)) / 2 + (lb - la)) + 6 * r + c;
+      } else {
+        oz[t] = tail + (kind == 1 ? 21 : 27) * nloc + 6 * la + q;
       }
-    } else {
-      z += static_cast<double>(tail[(kind == 1 ? 21 : 27) * nloc + 6 * la + q]);
+      ou[t] = tail + 21 * la + (want_u ? 6 * r - (r * (r - 1)) / 2 + (c - r) : 0);
+      od[t] = tail + 27 * nloc + 6 * la + r;
     }
+    float vz[kGather], vu[kGather], vd[kGather];
+#pragma unroll
+    for (int t = 0; t < kGather; ++t) {
+      vz[t] = b.part[oz[t]];
+      vu[t] = want_u ? b.part[ou[t]] : 0.f;
+      vd[t] = want_d ? b.part[od[t]] : 0.f;
+    }
+#pragma unroll
+    for (int t = 0; t < kGather; ++t)
+      if (k0 + t < kend) {
+        z += static_cast<double>(vz[t]);
+        u += static_cast<double>(vu[t]);
+        d += static_cast<double>(vd[t]);
+      }
+  }
+  if (wave > 0) {
+    red[0][wave - 1][lane] = z;
+    red[1][wave - 1][lane] = u;
+    red[2][wave - 1][lane] = d;
+  }
+  __syncthreads();
+  if (wave > 0 || kind < 0) return;
+#pragma unroll
+  for (int v = 0; v < kGatherWaves - 1; ++v) {
+    z += red[0][v][lane];
+    u += red[1][v][lane];
+    d += red[2][v][lane];
   }
   if (kind == 0) {
     double v = u - z;
@@ -511,7 +551,7 @@ namespace sba_dev {
 // epilogue. 3 nb launches for nb 64-row blocks.
 hipError_t launch_large_factor(const Batch& b, int w, int n, hipStream_t s) {
   const int nb = (n + kLB - 1) / kLB, ncf = n / 6;
-  hipLaunchKernelGGL(k_gather, dim3((ncf + 3) / 4, ncf), dim3(256), 0, s, b, w);
+  hipLaunchKernelGGL(k_gather, dim3(ncf * (ncf + 1) / 2), dim3(64 * kGatherWaves), 0, s, b, w);
   hipError_t e = hipGetLastError();
   for (int k = 0; k < nb && e == hipSuccess; ++k) {
     hipLaunchKernelGGL(k_potrf, dim3(1), dim3(256), 0, s, b, w, k);

@@ -182,6 +182,10 @@ struct WindowSort {
   // of their own), the pair copies their tiles need and where those start in the pair streams
   std::vector<int32_t> wide;
   int64_t ndup = 0, dup_off = 0;
+  // mapped windows: packed point -> window point, the free points before the fixed ones, each class ordered by
+  // its first key frame (stable), so that a chunk's consecutive points share key frames; empty otherwise (the
+  // packed order is the window's). npair, nfree, first_pair and wide follow the packed order, start the window's.
+  std::vector<int32_t> order;
 };
 
 // A wide point's free key frames are split into groups of at most kWideGroup consecutive ones; every two groups
@@ -205,7 +209,7 @@ struct HostPack {
   pinned_vector<Tile> tiles, btiles;
   pinned_vector<Chunk> chunks, bchunks;
   pinned_vector<int32_t> chunk_map, red_first, gfirst;
-  pinned_vector<int2> gcon;
+  pinned_vector<int4> gcon;
   // per-window scratch of the packing (the sort order and pair counts, the tiles before they are joined)
   std::vector<WindowSort> sorts;
   std::vector<std::vector<Tile>> wtiles, wbtiles;
@@ -262,6 +266,7 @@ struct sba_batch {
   double* dbg_S = nullptr;  // debug buffer of the reduced system of window 0
   int32_t* obs_orig = nullptr;  // packed observation -> index in its window's input order
   std::vector<int> mapped;      // windows with more than kMaxFreePoses free key frames
+  std::vector<std::vector<int32_t>> point_order;  // per window: packed point -> window point (empty: the same)
   double* adj = nullptr;        // adjoint workspace of window 0 (allocated on first use)
   bool adj_ready = false;       // the workspace holds the factor and the FP64 state of the current solution
   hipEvent_t ev[8] = {};        // kernel-class timing (created on first use)
@@ -453,6 +458,23 @@ int sort_window(sba_handle* h, const sba_problem_view& p, int w, int T, WindowSo
       nfree[i] = f;
     }
   });
+  // mapped windows: the packed point order (a counting sort by first key frame)
+  std::vector<int32_t>& order = S.order;
+  order.clear();
+  if (W.mapped && np > 0) {
+    std::vector<int32_t> bucket(2 * static_cast<size_t>(nc) + 1, 0);
+    const int npf = W.npf;
+    auto key = [&](int o) { return (o < npf ? 0 : nc) + (start[o] < start[o + 1] ? static_cast<int>(ko[start[o]] >> 40) : 0); };
+    for (int o = 0; o < np; ++o) ++bucket[key(o) + 1];
+    for (size_t k = 1; k < bucket.size(); ++k) bucket[k] += bucket[k - 1];
+    order.resize(np);
+    for (int o = 0; o < np; ++o) order[bucket[key(o)]++] = o;
+    std::vector<int32_t> tmp(np);
+    for (int i = 0; i < np; ++i) tmp[i] = npair[order[i]];
+    npair.swap(tmp);
+    for (int i = 0; i < np; ++i) tmp[i] = nfree[order[i]];
+    nfree.swap(tmp);
+  }
   // a mapped window's chunks stage at most kMaxFreePoses free key frames: points seen from more are wide
   S.wide.clear();
   S.ndup = 0;
@@ -480,6 +502,8 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
   const int obs_base = W.obs_off, pair_base = W.pair_off, point_base = W.point_off;
   const std::vector<int32_t>&start = S.start, &first_pair = S.first_pair;
   const std::vector<int64_t>& ko = S.ko;
+  // packed point i is window point orig(i)
+  auto orig = [&](int i) { return S.order.empty() ? i : S.order[i]; };
   // the gather into sorted order reads the input at random (disjoint output ranges per thread)
   parallel_ranges(h, T, no, [&](int, int64_t a, int64_t e) {
     for (int64_t k = a; k < e; ++k) {
@@ -501,9 +525,10 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
       int q = pair_base + first_pair[i];
       H.point_pair[point_base + i] = q;
       H.point_nfree[point_base + i] = S.nfree[i];
-      H.pts[point_base + i] = make_float4(p.points[3 * i], p.points[3 * i + 1], p.points[3 * i + 2], 0.f);
+      const int o = orig(static_cast<int>(i));
+      H.pts[point_base + i] = make_float4(p.points[3 * o], p.points[3 * o + 1], p.points[3 * o + 2], 0.f);
       int last = -1;
-      for (int k = start[i]; k < start[i + 1]; ++k) {
+      for (int k = start[o]; k < start[o + 1]; ++k) {
         const int pose = static_cast<int>(ko[k] >> 40);
         if (pose != last) {
           H.pair_obs[q] = obs_base + k;
@@ -564,11 +589,13 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
   make_tiles(btiles, kBlock);
   // pair records: observation range, key frame | point index inside its back-substitution tile << 16, point
   // index inside its Schur tile | local key frame << 8 (ranges end at the window's own ends)
-  const int pair_end = pair_base + S.npairs, obs_end = obs_base + no;
+  const int pair_end = pair_base + S.npairs;
   auto pair_range = [&](int i) {  // batch-wide pair range of batch-wide point i of this window
     const int qa = H.point_pair[i];
     return std::make_pair(qa, i + 1 < point_base + np ? H.point_pair[i + 1] : pair_end);
   };
+  // observation end of pair q, the last pair of batch-wide point i ends where the point's observations end
+  auto obs_stop = [&](int q, int qe, int i) { return q + 1 < qe ? H.pair_obs[q + 1] : obs_base + start[orig(i - point_base) + 1]; };
   parallel_ranges(h, T, static_cast<int64_t>(tiles.size()), [&](int, int64_t a, int64_t e) {
     for (int64_t t = a; t < e; ++t) {
       const Tile& Tl = tiles[t];
@@ -576,7 +603,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
         const auto [qa, qe] = pair_range(i);
         for (int q = qa; q < qe; ++q) {
           const int ob = H.pair_obs[q];
-          const int oe = q + 1 < pair_end ? H.pair_obs[q + 1] : obs_end;
+          const int oe = obs_stop(q, qe, i);
           // local free key frame index: the key frame itself unless the window is mapped (set with its chunks)
           const int jl = H.pair_pose[q] < ncf ? H.pair_pose[q] : 255;
           H.pair_rec[q] = make_int4(ob, oe, H.pair_pose[q], (i - Tl.point_begin) | (jl << 8));
@@ -588,7 +615,7 @@ void fill_window(sba_handle* h, const sba_problem_view& p, int w, int T, const W
   for (int i : S.wide) {
     const auto [qa, qe] = pair_range(point_base + i);
     for (int q = qa; q < qe; ++q) {
-      const int oe = q + 1 < pair_end ? H.pair_obs[q + 1] : obs_end;
+      const int oe = obs_stop(q, qe, point_base + i);
       H.pair_rec[q] = make_int4(H.pair_obs[q], oe, H.pair_pose[q], 255 << 8);
     }
   }
@@ -968,6 +995,8 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
       a_off += static_cast<int64_t>(nn) * nn + 2 * nn + static_cast<int64_t>((nn + 63) / 64) * 64 * 64;
       b->mapped.push_back(w);
     }
+    if (b->point_order.size() != static_cast<size_t>(n)) b->point_order.resize(n);
+    b->point_order[w] = H.sorts[w].order;
     max_nc = std::max(max_nc, W.nc);
     max_ncam = std::max(max_ncam, W.ncam);
     max_ncf = std::max(max_ncf, W.ncf);
@@ -1038,7 +1067,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
         c.nt = (6 * c.nloc + 15) / 16;
         c.map_off = static_cast<int32_t>(chunk_map.size());
         chunk_map.insert(chunk_map.end(), kfs.begin(), kfs.end());
-        rec = 256 * (c.nt * (c.nt + 1) / 2) + 33 * c.nloc;
+        rec = 36 * (c.nloc * (c.nloc + 1) / 2) + 33 * c.nloc;  // key-frame blocks, then U, e, diag U
       }
       chunks.push_back(c);
       part_off += rec;
@@ -1058,7 +1087,7 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   };
   {
     pinned_vector<int32_t>& gfirst = H.gfirst;
-    pinned_vector<int2>& gcon = H.gcon;
+    pinned_vector<int4>& gcon = H.gcon;
     std::vector<int32_t> cur;
     for (int w : b->mapped) {
       WindowDesc& W = b->win[w];
@@ -1083,7 +1112,10 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
         for (int la = 0; la < chunks[ci].nloc; ++la)
           for (int lb = la; lb < chunks[ci].nloc; ++lb)
             if (contributes(chunks[ci], la, lb))
-              gcon[cur[static_cast<size_t>(m[la]) * ncf + m[lb]]++] = make_int2(ci, la | (lb << 8));
+              gcon[cur[static_cast<size_t>(m[la]) * ncf + m[lb]]++] =
+                  make_int4(ci, la | (lb << 8) | (chunks[ci].nt << 16) | (chunks[ci].nloc << 24),
+                            static_cast<int32_t>(chunks[ci].part_off & 0xffffffff),
+                            static_cast<int32_t>(chunks[ci].part_off >> 32));
       }
     }
   }
@@ -1342,13 +1374,16 @@ int sba_batch_read(sba_handle* h, sba_batch* b, sba_result* results) {
     }
     // state is written back only for a solve that ran (the reference leaves the problem untouched on failure)
     if (r.status != SBA_OK) continue;
-    if (r.points)
+    if (r.points) {
+      const std::vector<int32_t>& order = b->point_order[w];
       for (int i = 0; i < W.np; ++i) {
         const float4 q = pts[W.point_off + i];
-        r.points[3 * i] = q.x;
-        r.points[3 * i + 1] = q.y;
-        r.points[3 * i + 2] = q.z;
+        const int o = order.empty() ? i : order[i];
+        r.points[3 * o] = q.x;
+        r.points[3 * o + 1] = q.y;
+        r.points[3 * o + 2] = q.z;
       }
+    }
     if (r.rig_from_world)
       for (int j = 0; j < W.nc; ++j) {
         float* m = r.rig_from_world + 16 * j;
@@ -1445,10 +1480,16 @@ int sba_batch_winsorize(sba_handle* h, sba_batch* b, int32_t window, int32_t key
       hipStreamSynchronize(s) != hipSuccess)
     return done(fail(h, SBA_E_DEVICE, "winsorize launch failed"));
   if (out[3]) return done(fail(h, SBA_E_CAPACITY, "more than 1024 observations share the winsorizing cut"));
+  // the marks are by packed point: a window whose points were reordered maps them back
+  const std::vector<int32_t>& order = b->point_order[window];
+  std::vector<uint8_t> packed_mark(reset_mask && !order.empty() ? W.np : 0);
+  uint8_t* mark_dst = order.empty() ? reset_mask : packed_mark.data();
   if (launch_winsorize_apply(b->dev, window, W.np, W.nc, mark, s) != hipSuccess ||
-      (reset_mask && W.np > 0 && hipMemcpyAsync(reset_mask, mark, W.np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (reset_mask && W.np > 0 && hipMemcpyAsync(mark_dst, mark, W.np, hipMemcpyDeviceToHost, s) != hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess)
     return done(fail(h, SBA_E_DEVICE, "winsorize apply failed"));
+  if (reset_mask && !order.empty())
+    for (int i = 0; i < W.np; ++i) reset_mask[order[i]] = packed_mark[i];
   b->adj_ready = false;
   if (n_reset) *n_reset = out[0];
   return done(SBA_OK);

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_wins_err(Batch b, int w, int kf, double
   int n = 0;
   for (int q = b.point_pair[gi]; q < b.point_pair[gi + 1]; ++q) {
     const bool seen = b.pair_pose[q] == kf;
-    for (int o = b.pair_obs[q]; o < b.pair_obs[q + 1]; ++o) {
+    for (int o = b.pair_rec[q].x, oe = b.pair_rec[q].y; o < oe; ++o) {
       double e = -1.0;
       if (seen) {
         const float4 inf = b.obs_info[o];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_wins_apply(Batch b, int w, const uint8_
       float4* info = const_cast<float4*>(b.obs_info);
       float4* pinfo = const_cast<float4*>(b.pair_info);
       for (int q = b.point_pair[gi]; q < b.point_pair[gi + 1]; ++q) {
-        for (int o = b.pair_obs[q]; o < b.pair_obs[q + 1]; ++o) info[o] = make_float4(0.f, 0.f, 0.f, info[o].w);
+        for (int o = b.pair_rec[q].x, oe = b.pair_rec[q].y; o < oe; ++o) info[o] = make_float4(0.f, 0.f, 0.f, info[o].w);
         if (pinfo != info) pinfo[q] = make_float4(0.f, 0.f, 0.f, pinfo[q].w);
       }
     }

@@ -161,7 +161,8 @@ static void check_batch(const Batch& b) {
   for (int c = 0; c < b.n_chunks; ++c) {
     const Chunk& ch = b.chunks[c];
     ntiles = std::max<int64_t>(ntiles, ch.tile_end);
-    const int64_t rec = 256 * (ch.nt * (ch.nt + 1) / 2) + 33 * ch.nloc;
+    const int64_t rec = b.win[ch.window].mapped ? 36 * (ch.nloc * (ch.nloc + 1) / 2) + 33 * ch.nloc
+                                                : 256 * (ch.nt * (ch.nt + 1) / 2) + 33 * ch.nloc;
     part = std::max<int64_t>(part, ch.part_off + rec);
     CHECK(ch.window >= 0 && ch.window < nw, "chunk window");
   }
@@ -201,13 +202,16 @@ static void check_batch(const Batch& b) {
         const int blk = ka * ncf + kb;
         int n_ord = 0, last = -1;
         for (int k = f[blk]; k < f[blk + 1]; ++k) {
-          const int c = b.gcon[k].x, la = b.gcon[k].y & 255, lb = b.gcon[k].y >> 8;
+          const int c = b.gcon[k].x, la = b.gcon[k].y & 255, lb = (b.gcon[k].y >> 8) & 255;
           CHECK(c > last && c >= W.chunk_begin && c < W.chunk_begin + W.nchunks, "gather plan: chunk order");
           last = c;
           if (c < W.chunk_begin || c >= W.chunk_begin + W.nchunks) break;
           const Chunk& ch = b.chunks[c];
           CHECK(la <= lb && lb < ch.nloc && b.chunk_map[ch.map_off + la] == ka && b.chunk_map[ch.map_off + lb] == kb,
                 "gather plan: local indices");
+          CHECK(((b.gcon[k].y >> 16) & 255) == ch.nt && (b.gcon[k].y >> 24) == ch.nloc &&
+                ((static_cast<int64_t>(b.gcon[k].w) << 32) | static_cast<uint32_t>(b.gcon[k].z)) == ch.part_off,
+                "gather plan: the chunk's record"); 
           const Tile& T = b.tiles[ch.tile_begin];
           if (T.virt) ++wide_count[{T.point_begin, blk}];
           else ++n_ord;
@@ -252,7 +256,7 @@ static void check_batch(const Batch& b) {
       {"part", b.part, 4 * part}, {"sred", b.sred, 4 * sred}, {"dc", b.dc, 4 * dc}, {"dc64", b.dc64, 8 * dc},
       {"out_pts", b.out_pts, 16 * static_cast<size_t>(b.n_points)}, {"out_poses", b.out_poses, 48 * static_cast<size_t>(b.n_poses)},
       {"trace", b.trace, sizeof(TraceEntry) * kTrace * nw}, {"bpart", b.bpart, sizeof(CostPart) * b.n_bchunks},
-      {"A", b.A, 8 * a}, {"gfirst", b.gfirst, 4 * gf}, {"gcon", b.gcon, 8 * gc}, {"chunk_map", b.chunk_map, 4 * cmap}};
+      {"A", b.A, 8 * a}, {"gfirst", b.gfirst, 4 * gf}, {"gcon", b.gcon, 16 * gc}, {"chunk_map", b.chunk_map, 4 * cmap}};
   // the pair streams alias the observation streams when every pair has one observation
   if (b.pair_xy == b.obs_xy) {
     CHECK(npairs == nobs, "pair streams alias observation streams of another length");

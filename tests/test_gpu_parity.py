@@ -251,6 +251,8 @@ def test_large_multicamera_window_matches_oracle():
     assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < 1e-6
     nf = p.rig_from_world.shape[0] - 1
     assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
+    # the window's points are packed in the order of their first key frame; they come back in the caller's order
+    assert np.median(np.linalg.norm(q.points - r["points"], axis=1)) < 1e-5
 
 
 def _add_far_points(p, gt_poses, n, seed):
@@ -306,6 +308,7 @@ def test_large_window_with_points_seen_from_more_than_40_key_frames():
     assert q.iterations == r["iterations"] == 10
     assert abs(q.last_cost_f64 - r["last_cost"]) / r["last_cost"] < 1e-6
     assert np.abs(q.rig_from_world[:nf, :3, 3] - r["poses"][:nf, :3, 3]).max() < 1e-4
+    assert np.median(np.linalg.norm(q.points - r["points"], axis=1)) < 1e-5
     q2 = p.copy()
     assert s.solve(q2)
     assert np.array_equal(q.rig_from_world, q2.rig_from_world) and np.array_equal(q.points, q2.points)
@@ -368,6 +371,22 @@ def test_winsorize_matches_oracle_and_the_second_run_continues_without_the_reset
     assert t.iterations == ref["iterations"]
     assert abs(t.initial_cost_f64 - ref["initial_cost"]) / ref["initial_cost"] < 1e-7
     assert abs(t.last_cost_f64 - ref["last_cost"]) / ref["last_cost"] < 1e-6
+
+
+def test_winsorize_of_a_mapped_window_matches_oracle():
+    # a window of more than 40 free key frames packs its points in another order: the reset marks come back in
+    # the caller's point order
+    p, _, _ = ca.scene.loop(num_keyframes=48, num_points=6000, loop_length_m=9.6, max_iterations=6)
+    s = ca.SchurComplementBundlerGpu(loss=HUBER, fixed_iterations=True)
+    b = s.batch([p])
+    b.run()
+    q = p.copy()
+    b.read([q])
+    state = (q.points.astype(np.float64), q.rig_from_world.astype(np.float64))
+    want0, n0 = O.winsorize(p, 0, *state)
+    mask0, m0 = b.winsorize(0)
+    assert m0 == n0 > 0
+    np.testing.assert_array_equal(mask0, want0)
 
 
 def test_winsorize_needs_fifty_observations_and_checks_arguments(solver_huber):
