@@ -83,10 +83,11 @@ __device__ inline bool factor_tile(double4v& t, double* sT, double* xt, double* 
     il = il * (1.5 - 0.5 * dd * il * il);
     a[c] = i == c ? dd * il : (i > c ? a[c] * il : 0.0);
     ri = i == c ? il : ri;
-    // rows i < c2 update their a[c2] too: it lies above the diagonal there, and pivot c2 sets it to zero
+    // rows i < c2 update their a[c2] too: it lies above the diagonal there, and pivot c2 sets it to zero. Column
+    // c + 1 first: the next pivot depends only on it, so its broadcast and square root can start while the
+    // other columns are updated.
 #pragma unroll
     for (int c2 = c + 1; c2 < 16; ++c2) a[c2] -= a[c] * readlane_d(a[c], c2);  // L[c2][c]
-    __builtin_amdgcn_sched_barrier(0);  // one pivot's broadcasts live at a time
   }
   if (lane < 16) {
     rdiag[lane] = ri;
