@@ -379,7 +379,9 @@ static_assert((kUVals + 9) * kScr <= kKt * kZcLd, "the scratch aliases the stage
 //   (z) zero the staged Z columns (they alias the scratch)
 //   (d) Z = W G into the staged columns; e_c terms -Z G^T e_b into their own scratch
 //   (e) e owners sum the e_c terms; matrix cores: acc += Zc Zc^T
-template <int NT>
+// POSES_GLOBAL: the key frames are read from global memory (cached) instead of the shared copy, for batches whose
+// largest window would otherwise leave room for only one workgroup per compute unit.
+template <int NT, bool POSES_GLOBAL>
 __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, const WindowDesc& W, float lam, int cur,
                                             float* sm) {
   constexpr int nt = NT;
@@ -388,7 +390,8 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   const int lane = tid & 63;
   const int ncf = W.ncf;
   const int nloc = ch.nloc;  // free key frames staged by this chunk (local index jl; = ncf unless mapped)
-  float* sPose = sm + b.lds_pose;
+  const float* sPose = POSES_GLOBAL ? reinterpret_cast<const float*>((cur ? b.poses1 : b.poses0) + 3 * W.pose_off)
+                                    : sm + b.lds_pose;
   float* sCam = sm + b.lds_cam;
   float* sAcc = sm + b.lds_acc;  // 12 per tile point: G (9) and G^T e_b (3)
   int* sEnt = reinterpret_cast<int*>(sm + b.lds_ent);  // [tile point][free key frame]: (tile << 9) | pair
@@ -705,6 +708,30 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
   }
 }
 
+template <bool POSES_GLOBAL>
+__device__ __forceinline__ void schur_dispatch(const Batch& b, const Chunk& ch, const WindowDesc& W, float lam, int cur,
+                                               float* sm) {
+  static_assert(kNT == 15, "the dispatch below covers 1..15 tiles");
+  switch (ch.nt) {
+    case 0:  // no free key frame: only the point blocks (V*^-1, V*^-1 e_b) are needed; the record is empty
+    case 1: schur_chunk<1, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 2: schur_chunk<2, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 3: schur_chunk<3, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 4: schur_chunk<4, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 5: schur_chunk<5, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 6: schur_chunk<6, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 7: schur_chunk<7, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 8: schur_chunk<8, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 9: schur_chunk<9, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 10: schur_chunk<10, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 11: schur_chunk<11, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 12: schur_chunk<12, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 13: schur_chunk<13, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    case 14: schur_chunk<14, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+    default: schur_chunk<15, POSES_GLOBAL>(b, ch, W, lam, cur, sm); break;
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
@@ -717,27 +744,11 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   const int cur = St->cur;
   const float lam = St->lambda;
   const int tid = threadIdx.x;
-  load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+  const bool poses_global = b.lds_pose < 0;
+  if (!poses_global) load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
   load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
-  static_assert(kNT == 15, "the dispatch below covers 1..15 tiles");
-  switch (ch.nt) {
-    case 0:  // no free key frame: only the point blocks (V*^-1, V*^-1 e_b) are needed; the record is empty
-    case 1: schur_chunk<1>(b, ch, W, lam, cur, sm); break;
-    case 2: schur_chunk<2>(b, ch, W, lam, cur, sm); break;
-    case 3: schur_chunk<3>(b, ch, W, lam, cur, sm); break;
-    case 4: schur_chunk<4>(b, ch, W, lam, cur, sm); break;
-    case 5: schur_chunk<5>(b, ch, W, lam, cur, sm); break;
-    case 6: schur_chunk<6>(b, ch, W, lam, cur, sm); break;
-    case 7: schur_chunk<7>(b, ch, W, lam, cur, sm); break;
-    case 8: schur_chunk<8>(b, ch, W, lam, cur, sm); break;
-    case 9: schur_chunk<9>(b, ch, W, lam, cur, sm); break;
-    case 10: schur_chunk<10>(b, ch, W, lam, cur, sm); break;
-    case 11: schur_chunk<11>(b, ch, W, lam, cur, sm); break;
-    case 12: schur_chunk<12>(b, ch, W, lam, cur, sm); break;
-    case 13: schur_chunk<13>(b, ch, W, lam, cur, sm); break;
-    case 14: schur_chunk<14>(b, ch, W, lam, cur, sm); break;
-    default: schur_chunk<15>(b, ch, W, lam, cur, sm); break;
-  }
+  if (poses_global) schur_dispatch<true>(b, ch, W, lam, cur, sm);
+  else schur_dispatch<false>(b, ch, W, lam, cur, sm);
 }
 
 // Sums the chunk records of each window (deterministic; no atomics). A window of many chunks: the four

@@ -1211,10 +1211,15 @@ int build_batch_impl(sba_handle* h, const sba_problem_view* problems, int32_t n,
   L.zeroed(d.trace, static_cast<size_t>(n) * kTrace);
   L.zeroed(d.bpart, bchunks.size());
   d.dbg_dp = nullptr;
-  // shared-memory layout of the Schur kernel, sized for the batch's largest window
+  // shared-memory layout of the Schur kernel, sized for the batch's largest window. The key frames are staged
+  // only while the layout leaves room for two workgroups per compute unit (80 KiB each); otherwise (large
+  // windows) the kernel reads them from global memory (lds_pose = -1).
   int off = 0;
-  d.lds_pose = off;
-  off += 12 * max_nc;
+  const int fixed_words = 12 * max_ncam + 12 * kTilePoints + kTilePoints * kMaxFreePoses + 24 * (kTilePairs + 1) +
+                          2 * 4 * kTilePoints + kKt * kZcLd + 128;
+  const bool stage_poses = (fixed_words + 12 * max_nc) * 4 + 64 <= 80 * 1024;
+  d.lds_pose = stage_poses ? off : -1;
+  if (stage_poses) off += 12 * max_nc;
   d.lds_cam = off;
   off += 12 * max_ncam;
   d.lds_acc = off;
