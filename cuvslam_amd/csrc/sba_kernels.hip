@@ -734,6 +734,7 @@ __device__ __forceinline__ void schur_dispatch(const Batch& b, const Chunk& ch, 
 
 }  // namespace
 
+template <bool POSES_GLOBAL>
 __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   extern __shared__ float sm[];
   const Chunk ch = b.chunks[blockIdx.x];
@@ -744,11 +745,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur(Batch b) {
   const int cur = St->cur;
   const float lam = St->lambda;
   const int tid = threadIdx.x;
-  const bool poses_global = b.lds_pose < 0;
-  if (!poses_global) load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
+  if (!POSES_GLOBAL) load_pose_lds(sm + b.lds_pose, (cur ? b.poses1 : b.poses0) + 3 * W.pose_off, 3 * W.nc, tid, kBlock);
   load_pose_lds(sm + b.lds_cam, b.cams + 3 * W.cam_off, 3 * W.ncam, tid, kBlock);
-  if (poses_global) schur_dispatch<true>(b, ch, W, lam, cur, sm);
-  else schur_dispatch<false>(b, ch, W, lam, cur, sm);
+  schur_dispatch<POSES_GLOBAL>(b, ch, W, lam, cur, sm);
 }
 
 // Sums the chunk records of each window (deterministic; no atomics). A window of many chunks: the four
@@ -1515,7 +1514,11 @@ hipError_t launch_reset(const Batch& b, hipStream_t s) {
 }
 
 hipError_t launch_schur(const Batch& b, hipStream_t s) {
-  hipLaunchKernelGGL(k_schur, dim3(b.n_chunks), dim3(kBlock), static_cast<size_t>(b.lds_words) * 4, s, b);
+  // key frames staged in shared memory unless the layout reads them from global memory (lds_pose < 0)
+  if (b.lds_pose < 0)
+    hipLaunchKernelGGL(k_schur<true>, dim3(b.n_chunks), dim3(kBlock), static_cast<size_t>(b.lds_words) * 4, s, b);
+  else
+    hipLaunchKernelGGL(k_schur<false>, dim3(b.n_chunks), dim3(kBlock), static_cast<size_t>(b.lds_words) * 4, s, b);
   return hipGetLastError();
 }
 
