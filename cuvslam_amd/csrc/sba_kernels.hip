@@ -859,7 +859,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
   // past n are identity. Stored sum Z Z^T tiles hold element (row, col) at 64 (row & 3) + 16 (row >> 2) + col;
   // diagonal tiles are read above their diagonal.
   auto element = [&](int J, int I, int v) -> double {
-    // branch-free, every load in bounds, so that all the loads of a wavefront's tiles are in flight together
+    // every load in bounds and no branch around the Z Z^T loads, so that a wavefront's tile loads are in flight
+    // together
     const int rl = 4 * v + li, r = 16 * J + rl, c = 16 * I + lj;
     const bool inside = r < n && c < n;
     const bool mir = J == I && rl > lj;
@@ -867,8 +868,9 @@ __global__ __launch_bounds__(kFactorThreads) void k_factor(Batch b, double* dbg_
     const float zz = R[256 * upper_tile(J, I, T) + 64 * (er & 3) + 16 * (er >> 2) + ec];
     const int lo = min(r, c), hi = max(r, c), pose = lo / 6, rr = lo - 6 * pose, cc = hi - 6 * pose;
     const bool blk = inside && cc < 6;
-    const float u = RU[blk ? 21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr) : 0];
-    const float dg = RD[min(r, n - 1)];
+    // U only inside a key frame's 6 x 6 block, diag U only on the diagonal: the other lanes load nothing
+    const float u = blk ? RU[21 * pose + 6 * rr - (rr * (rr - 1)) / 2 + (cc - rr)] : 0.f;
+    const float dg = r == c ? RD[min(r, n - 1)] : 0.f;
     const double val = -static_cast<double>(zz) + (blk ? static_cast<double>(u) : 0.0) +
                        (r == c ? static_cast<double>(lam) * static_cast<double>(dg) : 0.0);
     return inside ? val : (r == c ? 1.0 : 0.0);
