@@ -267,6 +267,7 @@ static void check_batch(const Batch& b) {
 
 // The packed streams of the last batch whose solve was enqueued (copied at its first launch).
 static std::vector<std::vector<char>> g_packed;
+static std::vector<float4> g_packed_pts;  // the packed points of the last batch
 static void snapshot(const Batch& b) {
   auto grab = [](const void* p, size_t bytes) {
     const char* c = static_cast<const char*>(p);
@@ -284,6 +285,7 @@ static void snapshot(const Batch& b) {
               grab(b.pair_xy, 8 * npairs), grab(b.pair_info, 16 * npairs),
               grab(b.point_pair, 4 * (b.n_points + 1)), grab(b.point_nfree, 4 * b.n_points),
               grab(b.pts_init, 16 * b.n_points), grab(b.poses_init, 48 * b.n_poses)};
+  g_packed_pts.assign(b.pts_init, b.pts_init + b.n_points);
 }
 
 namespace sba_dev {
@@ -302,7 +304,14 @@ hipError_t launch_factor(const Batch& b, double*, hipStream_t, double*) {
   return hipSuccess;
 }
 hipError_t launch_backsub(const Batch& b, bool, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
-hipError_t launch_decide(const Batch& b, bool, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
+// The decision also writes the solution streams: here the initial state, so that a read returns the input.
+hipError_t launch_decide(const Batch& b, bool, hipStream_t) {
+  ++g_launches;
+  check_batch(b);
+  if (b.out_pts && b.pts_init) std::memcpy(b.out_pts, b.pts_init, sizeof(float4) * b.n_points);
+  if (b.out_poses && b.poses_init) std::memcpy(b.out_poses, b.poses_init, sizeof(float4) * 3 * b.n_poses);
+  return hipSuccess;
+}
 hipError_t launch_gather(const Batch& b, hipStream_t) { ++g_launches; check_batch(b); return hipSuccess; }
 hipError_t launch_large_factor(const Batch& b, int, int, hipStream_t) { check_batch(b); return hipSuccess; }
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t) {
@@ -618,6 +627,24 @@ int main() {
       int f = 0;
       for (int j = 0; j < std::min(nc, 41 + k); ++j) f += (j * 7 + k) % nc < nc - v.num_fixed_key_frames;
       expect += f > 40;
+    }
+    // the mapped window's points are packed in another order: a read (of the unchanged state) returns them in
+    // the caller's order
+    {
+      sba_batch* b = nullptr;
+      CHECK(sba_batch_create(h, &w, 1, &b) == SBA_OK, sba_last_error(h));
+      CHECK(sba_batch_run(h, b) == SBA_OK, "run");
+      std::vector<float> rp(3 * w.num_points, -1.f), rq(16 * w.num_poses);
+      sba_result r{};
+      r.points = rp.data();
+      r.rig_from_world = rq.data();
+      CHECK(sba_batch_read(h, b, &r) == SBA_OK, "read");
+      CHECK(std::memcmp(rp.data(), pts.data(), sizeof(float) * rp.size()) == 0, "points come back in the caller's order");
+      bool moved = false;  // the packed order differs from the caller's
+      for (int i = 0; i < w.num_points && !moved; ++i)
+        moved = std::memcmp(&g_packed_pts[i], &pts[3 * i], sizeof(float) * 3) != 0;
+      CHECK(moved, "the mapped window's points are packed in another order");
+      sba_batch_destroy(h, b);
     }
     std::printf("  wide points %d (expected %d)\n", g_wide_points, expect);
     CHECK(expect >= 3 && g_wide_points == expect, "the wide points have tiles of their own");
