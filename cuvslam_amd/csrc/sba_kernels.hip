@@ -633,7 +633,7 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       for (int r = 0; r < 6; ++r) sEc[r * kScr + tid] = ec[r];
     }
     lds_barrier();
-    // (e) the next tile's pair data is requested; the e owners sum the e_c terms; matrix cores: acc += Zc Zc^T;
+    // (e) the next tile's pair data is requested; matrix cores: acc += Zc Zc^T; the e owners sum the e_c terms;
     // the next tile's points go to the other half of sPts (read after the next barrier)
     const bool has_next = t + 1 < ch.tile_end;
     float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -648,6 +648,13 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
       pp_a = b.point_pair[Tn.point_begin + vp] + Tn.pair_shift;
       pp_e = b.point_pair[Tn.point_begin + vp + 1] + Tn.pair_shift;
     }
+    switch (tid >> 6) {
+      case 0: mfma_tile<NT, 0>(sZc, kpad / 4, lane, acc); break;
+      case 1: mfma_tile<NT, 1>(sZc, kpad / 4, lane, acc); break;
+      case 2: mfma_tile<NT, 2>(sZc, kpad / 4, lane, acc); break;
+      default: mfma_tile<NT, 3>(sZc, kpad / 4, lane, acc); break;
+    }
+    // after the matrix-core instructions are issued, so that these shared-memory sums overlap their execution
     if (u_owner && u1 > 21) {
       // as in (c); rows -1 .. 8 around the six e_c rows are read (the pair table lies before them, the W scratch
       // after them) and those outside 21 <= u0 + v < u1 dropped
@@ -670,12 +677,6 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
 #pragma unroll
           for (int v = 0; v < kUGroup; ++v) accU[v] += ok[j] && u0 + v >= 21 && u0 + v < u1 ? x[j][v] : 0.f;
       }
-    }
-    switch (tid >> 6) {
-      case 0: mfma_tile<NT, 0>(sZc, kpad / 4, lane, acc); break;
-      case 1: mfma_tile<NT, 1>(sZc, kpad / 4, lane, acc); break;
-      case 2: mfma_tile<NT, 2>(sZc, kpad / 4, lane, acc); break;
-      default: mfma_tile<NT, 3>(sZc, kpad / 4, lane, acc); break;
     }
     if (tid < npts_next) sPts[((t + 1) & 1) * kTilePoints + tid] = pn;
   }
