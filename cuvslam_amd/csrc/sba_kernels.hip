@@ -529,14 +529,21 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     // (c) owners sum the pair terms: U and e_a of key frame uj into registers; V, e_b of point vp, then its
     // damped pseudo-inverse V*^+ = G G^T
     if (u_owner) {
-      // four tile points at a time: their table entries, then their 20 values, each batch of loads in flight
-      // together (values past u1 are read and dropped: rows u0 + v <= 29 lie inside the scratch, whose 9 V
-      // rows follow the 27 U rows)
+      // four tile points at a time: their 20 values and the next four points' table entries, each batch of loads
+      // in flight together (values past u1 are read and dropped: rows u0 + v <= 29 lie inside the scratch, whose
+      // 9 V rows follow the 27 U rows; entries past the tile's points are read and not used)
+      int en_next[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) en_next[j] = sEnt[j * kMaxFreePoses + uj];
 #pragma unroll 1
       for (int q0 = 0; q0 < npts; q0 += 4) {
         int en[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) en[j] = sEnt[(q0 + j) * kMaxFreePoses + uj];
+        for (int j = 0; j < 4; ++j) en[j] = en_next[j];
+        if (q0 + 4 < kTilePoints) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) en_next[j] = sEnt[(q0 + 4 + j) * kMaxFreePoses + uj];
+        }
         float x[4][kUGroup];
         bool ok[4];
 #pragma unroll
