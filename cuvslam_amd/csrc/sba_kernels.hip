@@ -663,13 +663,21 @@ __device__ __forceinline__ void schur_chunk(const Batch& b, const Chunk& ch, con
     }
     // after the matrix-core instructions are issued, so that these shared-memory sums overlap their execution
     if (u_owner && u1 > 21) {
-      // as in (c); rows -1 .. 8 around the six e_c rows are read (the pair table lies before them, the W scratch
-      // after them) and those outside 21 <= u0 + v < u1 dropped
+      // as in (c), the next table entries loaded with the current values; rows -1 .. 8 around the six e_c rows
+      // are read (the pair table lies before them, the W scratch after them) and those outside 21 <= u0 + v < u1
+      // dropped
+      int en_next[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) en_next[j] = sEnt[j * kMaxFreePoses + uj];
 #pragma unroll 1
       for (int q0 = 0; q0 < npts; q0 += 4) {
         int en[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) en[j] = sEnt[(q0 + j) * kMaxFreePoses + uj];
+        for (int j = 0; j < 4; ++j) en[j] = en_next[j];
+        if (q0 + 4 < kTilePoints) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) en_next[j] = sEnt[(q0 + 4 + j) * kMaxFreePoses + uj];
+        }
         float x[4][kUGroup];
         bool ok[4];
 #pragma unroll
