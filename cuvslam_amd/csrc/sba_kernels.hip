@@ -1176,20 +1176,26 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
   float maxdp = 0.f;
   __syncthreads();  // key frames, cameras and dc in shared memory
 
+  // this thread's pair of the tile: record (first observation, end, key frame, point in the tile) and first
+  // observation; the next tile's are requested before this tile's (c), so that their loads overlap it
+  Tile Tn = b.btiles[ch.tile_begin];
+  int4 rec_n = make_int4(0, 0, 0, 0);
+  float2 xy_n = make_float2(0.f, 0.f);
+  float4 inf_n = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (tid < Tn.pair_end - Tn.pair_begin) {
+    rec_n = b.pair_rec[Tn.pair_begin + tid];
+    xy_n = b.pair_xy[Tn.pair_begin + tid];
+    inf_n = b.pair_info[Tn.pair_begin + tid];
+  }
   for (int t = ch.tile_begin; t < ch.tile_end; ++t) {
-    const Tile T = b.btiles[t];
+    const Tile T = Tn;
+    if (t + 1 < ch.tile_end) Tn = b.btiles[t + 1];
     const int p0 = T.pair_begin;
     const int npair = T.pair_end - p0;
     const int npts = T.point_end - T.point_begin;
-    // this thread's pair: record (first observation, end, key frame, point in the tile) and first observation
-    int4 rec = make_int4(0, 0, 0, 0);
-    float2 xy0 = make_float2(0.f, 0.f);
-    float4 inf0 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (tid < npair) {
-      rec = b.pair_rec[p0 + tid];
-      xy0 = b.pair_xy[p0 + tid];
-      inf0 = b.pair_info[p0 + tid];
-    }
+    const int4 rec = tid < npair ? rec_n : make_int4(0, 0, 0, 0);
+    const float2 xy0 = xy_n;
+    const float4 inf0 = inf_n;
     // No barrier here: this tile's (a) writes only sPair, which the previous tile last read before its (b)-(c)
     // barrier, and (b) writes sAcc after the (a)-(b) barrier, which every thread reaches after its previous (c).
     // the first observation's J_c dc_j, J_p and weight, kept from (a) for the predicted reduction in (c):
@@ -1280,6 +1286,11 @@ __global__ __launch_bounds__(kBlock) void k_backsub(Batch b) {
         if (b.dbg_dp && w == 0) b.dbg_dp[i - W.point_off] = make_float4(dp[0], dp[1], dp[2], 0.f);
       }
       __syncthreads();
+    }
+    if (t + 1 < ch.tile_end && tid < Tn.pair_end - Tn.pair_begin) {
+      rec_n = b.pair_rec[Tn.pair_begin + tid];
+      xy_n = b.pair_xy[Tn.pair_begin + tid];
+      inf_n = b.pair_info[Tn.pair_begin + tid];
     }
     // (c) one thread per pair: trial cost in FP64, and w |J_c dc + J_p dp|^2_Lambda
     if (tid < npair) {
