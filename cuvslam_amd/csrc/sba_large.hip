@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kLB = 64;        // block of the large factorization
 constexpr int kGather = 8;     // contributions whose loads k_gather issues together
-constexpr int kGatherWaves = 8;  // wavefronts of k_gather that share one key-frame block's contributions
+constexpr int kGatherWaves = 4;  // wavefronts of k_gather that share one key-frame block's contributions
 
 struct Large {
   double* A;     // n x n, row-major; the lower triangle holds A, then L
